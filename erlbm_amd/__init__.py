@@ -1,0 +1,18 @@
+"""erlbm_amd -- MI355X-native E-RLBM collide-stream (D2Q9 / D3Q19).
+
+HIP kernels for gfx950 behind the C ABI of ``include/erlbm.h``
+(``liberlbm.so``, built in-tree by ``python -m erlbm_amd.build``); this
+package is the Python host side over that ABI.  Importing it does not need a
+GPU; using a lattice does, and there is no CPU fallback.
+"""
+from ._lib import (COL_BGK, COL_EXT_REG, COL_MRT, COL_REG, FP16_SHIFTED, FP32_SHIFTED, FP64, POST_COLLISION,
+                   STATE, STOP_DENSITY, STOP_ENERGY, ErlbmError, device_count, lib)
+from .lattice import (Lattice, dist_eq, init_double_shear, init_taylor_green, init_tg_map, obstacle_map_point_params,
+                      obstacle_mask, tg_map_point_params)
+
+__all__ = [
+    "COL_BGK", "COL_MRT", "COL_REG", "COL_EXT_REG", "FP64", "FP32_SHIFTED", "FP16_SHIFTED", "STATE",
+    "POST_COLLISION", "STOP_DENSITY", "STOP_ENERGY", "ErlbmError", "device_count", "lib", "Lattice", "dist_eq",
+    "init_double_shear", "init_taylor_green", "init_tg_map", "obstacle_mask", "tg_map_point_params",
+    "obstacle_map_point_params",
+]

@@ -1,0 +1,93 @@
+"""ctypes binding of ``liberlbm.so`` (the HIP library's C ABI, include/erlbm.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback anywhere in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liberlbm.so")
+
+ERLBM_OK, ERLBM_ERR_ARG, ERLBM_ERR_HIP, ERLBM_ERR_STATE, ERLBM_ERR_NOMEM, ERLBM_ERR_UNSUPPORTED = range(6)
+COL_BGK, COL_MRT, COL_REG, COL_EXT_REG = 1, 2, 3, 4
+FP64, FP32_SHIFTED, FP16_SHIFTED = 0, 1, 2
+STATE, POST_COLLISION = 0, 1
+STOP_DENSITY, STOP_ENERGY = 1, 2
+
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_i, _d, _ll = C.c_int, C.c_double, C.c_longlong
+
+
+class ErlbmError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where} failed with code {code}: {msg}")
+        self.code = code
+
+
+# (name, restype, argtypes) of every C-ABI entry point declared in include/erlbm.h
+SIGNATURES = {
+    "erlbm_last_error": (C.c_char_p, []),
+    "erlbm_device_count": (_i, [C.POINTER(_i)]),
+    "erlbm_lattice_create": (_i, [_i, _i, _i, _i, _vp, _i, _vp, C.POINTER(_vp)]),
+    "erlbm_lattice_destroy": (_i, [_vp]),
+    "erlbm_lattice_fluid_count": (_i, [_vp, C.POINTER(_i)]),
+    "erlbm_set_params": (_i, [_vp, _d, _d, _d, _i]),
+    "erlbm_set_zouhe_inlet": (_i, [_vp, _i, _d]),
+    "erlbm_set_outlet_rescale": (_i, [_vp, _i, _d]),
+    "erlbm_set_density_check": (_i, [_vp, _d]),
+    "erlbm_upload_aos": (_i, [_vp, _dp]),
+    "erlbm_download_aos": (_i, [_vp, _i, _dp]),
+    "erlbm_step": (_i, [_vp, _i, _i, _d]),
+    "erlbm_step_timed": (_i, [_vp, _i, C.POINTER(C.c_float)]),
+    "erlbm_status": (_i, [_vp, C.POINTER(_ll), C.POINTER(_i)]),
+    "erlbm_synchronize": (_i, [_vp]),
+    "erlbm_kinetic_energy": (_i, [_vp, _i, _i, C.POINTER(_d)]),
+    "erlbm_density_monitor": (_i, [_vp, _i, _d, C.POINTER(_d), C.POINTER(_i)]),
+    "erlbm_dist_eq": (_i, [_i, _d, _d, _d, _d, _dp]),
+    "erlbm_init_taylor_green": (_i, [_i, _i, _d, _d, _i, _i, _dp]),
+    "erlbm_init_double_shear": (_i, [_i, _d, _dp]),
+    "erlbm_init_tg_map": (_i, [_i, _d, _d, _dp]),
+    "erlbm_tg_map_tables": (_i, [_i, _dp, _dp]),
+    "erlbm_obstacle_mask": (_i, [_ip]),
+    "erlbm_tg_map_point_params": (_i, [_i, _i, _i, _i, _i, _i, _d, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d),
+                                       C.POINTER(_i)]),
+    "erlbm_obstacle_map_point_params": (_i, [_i, _i, _i, _i, _d, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d),
+                                             C.POINTER(_i)]),
+}
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises if it was not built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m erlbm_amd.build` "
+                              "(the HIP path has no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, where: str) -> None:
+    if rc != ERLBM_OK:
+        msg = lib().erlbm_last_error()
+        raise ErlbmError(rc, where, msg.decode() if msg else "")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().erlbm_device_count(C.byref(n)), "erlbm_device_count")
+    return n.value

@@ -1,0 +1,206 @@
+// collide.hpp -- per-node FP64 collision, bitwise faithful to SimBoltz.
+//
+// Restates, in registers, the operation order of
+//   calcula        SimBoltz_Functions/LBM_functions.cpp:3517-3589
+//   dist_eq        SimBoltz_Functions/LBM_functions.cpp:2944-3000
+//   coll_Ext_Reg   SimBoltz_Functions/Collision_operators.cpp:1181-1307
+//   coll_Reg       SimBoltz_Functions/Collision_operators.cpp:859-967
+//   coll_BGK       SimBoltz_Functions/Collision_operators.cpp:11-49
+// so that, compiled with -ffp-contract=off and IEEE division, every finite
+// population equals the reference's bit for bit.  Exact simplifications only:
+//  * products by c in {-1, 0, +1} are not formed: x*(+-1) is exact, and a
+//    +-0 term added to a running sum that starts at +0.0 never changes it;
+//  * opposite directions share cv*3 and ((0.5 cv) cv 3) 3, which are odd and
+//    even in cv under IEEE round-to-nearest;
+//  * the forcing term S (source(), LBM_functions.cpp:5645-5716) is +-0 at the
+//    zero body force every driver passes, so it is not added.
+// Non-finite states stay non-finite (possibly inf where the reference has
+// NaN), which every stability test treats alike.
+#pragma once
+
+#include "vset.hpp"
+
+namespace erlbm {
+
+enum Collision { COL_BGK = 1, COL_MRT = 2, COL_REG = 3, COL_EXT_REG = 4 };
+
+struct CollideConsts {
+    double omega1;   // 1 - 1/tau   (coll_Ext_Reg :1192; coll_Reg's (1 - 1/tau))
+    double omega2;   // 1 - 1/tau_nh (coll_Ext_Reg :1194)
+    double oot;      // 1/tau        (coll_BGK :24)
+};
+
+// rho, u as calcula() computes them (sums in velocity order from 0.0,
+// u = m * (1/rho), u = 0 when rho == 0).
+template <int Q>
+__device__ __forceinline__ void moments_exact(const double (&g)[Q], double &rho, double &vx,
+                                              double &vy, double &vz)
+{
+    using V = VSet<Q>;
+    double r = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        r = r + g[i];
+        if constexpr (V::cx[i] == 1) mx = mx + g[i];
+        if constexpr (V::cx[i] == -1) mx = mx - g[i];
+        if constexpr (V::cy[i] == 1) my = my + g[i];
+        if constexpr (V::cy[i] == -1) my = my - g[i];
+        if constexpr (V::cz[i] == 1) mz = mz + g[i];
+        if constexpr (V::cz[i] == -1) mz = mz - g[i];
+    });
+    rho = r;
+    if (r != 0.0) {
+        const double oor = 1.0 / r;
+        vx = mx * oor;
+        vy = my * oor;
+        vz = (V::D == 3) ? mz * oor : 0.0;
+    } else {
+        vx = 0.0;
+        vy = 0.0;
+        vz = 0.0;
+    }
+}
+
+// c_i . u in dot_product's order (x, then y, then z), zero terms dropped.
+template <int Q, int I>
+__device__ __forceinline__ double cdotu(double vx, double vy, double vz)
+{
+    using V = VSet<Q>;
+    double cv = 0.0;
+    bool started = false;
+    if constexpr (V::cx[I] != 0) {
+        cv = V::cx[I] > 0 ? vx : -vx;
+        started = true;
+    }
+    if constexpr (V::cy[I] != 0) {
+        if (started) cv = V::cy[I] > 0 ? cv + vy : cv - vy;
+        else cv = V::cy[I] > 0 ? vy : -vy;
+        started = true;
+    }
+    if constexpr (V::cz[I] != 0) {
+        if (started) cv = V::cz[I] > 0 ? cv + vz : cv - vz;
+        else cv = V::cz[I] > 0 ? vz : -vz;
+    }
+    (void)started;
+    return cv;
+}
+
+// f_eq = (w rho)(((1 + 3 cv) + ((0.5 cv) cv 3) 3) - (0.5 u.u) 3), dist_eq's order.
+template <int Q>
+__device__ __forceinline__ void feq_exact(double rho, double vx, double vy, double vz,
+                                          double (&feq)[Q])
+{
+    using V = VSet<Q>;
+    const double vquad = (V::D == 2) ? (vx * vx + vy * vy) : (vx * vx + vy * vy + vz * vz);
+    const double s = (0.5 * vquad) * 3.0;
+    feq[0] = (V::w[0] * rho) * (1.0 - s);
+    static_for<(Q - 1) / 2>([&](auto P) {
+        constexpr int i = 2 * P + 1;  // i and i+1 are opposite
+        const double cv = cdotu<Q, i>(vx, vy, vz);
+        const double a = cv * 3.0;
+        const double q = (((0.5 * cv) * cv) * 3.0) * 3.0;
+        feq[i] = (V::w[i] * rho) * (((1.0 + a) + q) - s);
+        feq[i + 1] = (V::w[i + 1] * rho) * (((1.0 - a) + q) - s);
+    });
+}
+
+// One collision in place.  Returns nothing; g holds f* afterwards.
+// vx, vy, vz receive the pre-collision velocity (for monitors).
+template <int Q, int COL>
+__device__ __forceinline__ void collide_exact(double (&g)[Q], const CollideConsts &k, double &vx,
+                                              double &vy, double &vz)
+{
+    using V = VSet<Q>;
+    double rho;
+    moments_exact<Q>(g, rho, vx, vy, vz);
+    double feq[Q];
+    feq_exact<Q>(rho, vx, vy, vz, feq);
+
+    if constexpr (COL == COL_BGK) {
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            g[i] = g[i] + (feq[i] - g[i]) * k.oot;
+        });
+        return;
+    } else {
+        double d[Q];
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            d[i] = g[i] - feq[i];
+        });
+        // Pi^(1)_ab = sum_i (f_i - f_eq_i) c_ia c_ib, index order, from 0.0
+        double Pxx = 0.0, Pyy = 0.0, Pzz = 0.0, Pxy = 0.0, Pxz = 0.0, Pyz = 0.0;
+        bool sxx = false, syy = false, szz = false, sxy = false, sxz = false, syz = false;
+        auto acc = [](double &s, bool &st, double v, int sign) {
+            if (!st) {
+                s = sign > 0 ? v : -v;
+                st = true;
+            } else {
+                s = sign > 0 ? s + v : s - v;
+            }
+        };
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            if constexpr (V::cx[i] != 0) acc(Pxx, sxx, d[i], 1);
+            if constexpr (V::cy[i] != 0) acc(Pyy, syy, d[i], 1);
+            if constexpr (V::D == 3 && V::cz[i] != 0) acc(Pzz, szz, d[i], 1);
+            if constexpr (V::cx[i] * V::cy[i] != 0) acc(Pxy, sxy, d[i], V::cx[i] * V::cy[i]);
+            if constexpr (V::D == 3 && V::cx[i] * V::cz[i] != 0) acc(Pxz, sxz, d[i], V::cx[i] * V::cz[i]);
+            if constexpr (V::D == 3 && V::cy[i] * V::cz[i] != 0) acc(Pyz, syz, d[i], V::cy[i] * V::cz[i]);
+        });
+        // Q_aa = c_a c_a - 1/3: (1.0 - 1/3) or (0 - 1/3); 2 Q_ab = +-2 for a != b.
+        constexpr double cs2 = 1. / 3.;
+        constexpr double Q1 = 1.0 - cs2;
+        constexpr double Q0 = 0.0 - cs2;
+        const double axx1 = Q1 * Pxx, axx0 = Q0 * Pxx;
+        const double ayy1 = Q1 * Pyy, ayy0 = Q0 * Pyy;
+        const double azz1 = Q1 * Pzz, azz0 = Q0 * Pzz;
+        const double bxy = 2.0 * Pxy, bxz = 2.0 * Pxz, byz = 2.0 * Pyz;
+        double f1[Q];
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            double t = (V::cx[i] != 0 ? axx1 : axx0) + (V::cy[i] != 0 ? ayy1 : ayy0);
+            if constexpr (V::D == 3) t = t + (V::cz[i] != 0 ? azz1 : azz0);
+            if constexpr (V::cx[i] * V::cy[i] > 0) t = t + bxy;
+            if constexpr (V::cx[i] * V::cy[i] < 0) t = t - bxy;
+            if constexpr (V::D == 3 && V::cx[i] * V::cz[i] > 0) t = t + bxz;
+            if constexpr (V::D == 3 && V::cx[i] * V::cz[i] < 0) t = t - bxz;
+            if constexpr (V::D == 3 && V::cy[i] * V::cz[i] > 0) t = t + byz;
+            if constexpr (V::D == 3 && V::cy[i] * V::cz[i] < 0) t = t - byz;
+            constexpr double k45 = (0.5 * 3. * 3.) * V::w[i];
+            f1[i] = k45 * t;
+        });
+        if constexpr (COL == COL_REG) {
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                g[i] = feq[i] + k.omega1 * f1[i];
+            });
+        } else {
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                const double fgt1 = d[i] - f1[i];
+                g[i] = (feq[i] + k.omega1 * f1[i]) + k.omega2 * fgt1;
+            });
+        }
+    }
+}
+
+// density(): sum in velocity order from 0.0 (LBM_functions.cpp:5543-5552).
+template <int Q>
+__device__ __forceinline__ double density_exact(const double (&g)[Q])
+{
+    double r = 0.0;
+    static_for<Q>([&](auto I) { r = r + g[(int)I]; });
+    return r;
+}
+
+// kinectic_energy()'s per-site term 0.5 (vx^2 + vy^2 + vz^2)
+// (Other_functions.cpp:100-115); in 2D calcula sets vz = 0.
+template <int Q>
+__device__ __forceinline__ double ke_term(double vx, double vy, double vz)
+{
+    if constexpr (VSet<Q>::D == 2) return 0.5 * (vx * vx + vy * vy);
+    else return 0.5 * (vx * vx + vy * vy + vz * vz);
+}
+
+}  // namespace erlbm

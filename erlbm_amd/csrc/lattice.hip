@@ -1,0 +1,870 @@
+// lattice.hip -- plain (one lattice per handle) E-RLBM collide-stream path.
+//
+// Device layout: structure of arrays over voxels, f[i * plane + v] with
+// v = x + nx (y + ny z) and plane = n_voxels rounded up to 64 doubles.
+// Solid voxels hold no state.  A step is one fused kernel: pull-stream
+// (gather f_i from x - c_i, or the opposite population of x itself when
+// x - c_i is solid: halfway bounce-back), boundary columns in registers,
+// collision, optional density test, one coalesced store per population.
+//
+// Phase.  The reference pushes after colliding, so between steps its inif
+// holds ref_k = S(C(BC(ref_{k-1}))).  A pull kernel stores the
+// post-collision populations h_k = C(BC(ref_k)) instead, and the gather of
+// step k+1 reproduces ref_{k+1} = S(h_k) bit for bit in registers.  The first
+// step after an upload gathers with the identity (its input is ref_0), and
+// downloads/monitors of "the state" gather S(h) on the fly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "collide.hpp"
+#include "common.hpp"
+
+using namespace erlbm;
+
+namespace erlbm {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const char *what, hipError_t e)
+{
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+}
+void set_last_error_msg(const char *msg) { g_last_error = msg; }
+
+}  // namespace erlbm
+
+extern "C" const char *erlbm_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int erlbm_device_count(int *count)
+{
+    if (!count) return ERLBM_ERR_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return ERLBM_OK;
+}
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr uint32_t FLUID_BIT = 1u << 31;
+
+struct Geometry {
+    int nx, ny, nz;
+    long long plane;
+};
+
+struct BCArgs {
+    int inlet_x;    // < 0: none
+    int inlet_west; // pos_x < nx/2 branch of ZouHeD3Q19_vx_x
+    double u_in;
+    int outlet_x;   // < 0: none
+    int outlet_dx;  // +1 west half, -1 east half (devnull_rho_x :90-92)
+    double rho_out;
+};
+
+struct StepArgs {
+    const double *__restrict__ src;
+    double *__restrict__ dst;
+    Geometry g;
+    const uint32_t *__restrict__ nbr;  // general kernel only
+    CollideConsts k;
+    BCArgs bc;
+    double dens_limit;                 // <= 0: no test
+    unsigned long long *status;
+    unsigned long long step;
+};
+
+__device__ __forceinline__ bool stopped(const unsigned long long *status, unsigned long long step)
+{
+    const unsigned long long s =
+        __hip_atomic_load(&status[ST_STOP_STEP], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return step >= s;
+}
+
+__device__ __forceinline__ void flag_density(unsigned long long *status, unsigned long long step, bool bad)
+{
+    // one atomic per wavefront
+    if (__any(bad)) {
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&status[ST_STOP_STEP], step + 1);
+            atomicOr(&status[ST_REASON], (unsigned long long)ERLBM_STOP_DENSITY);
+        }
+    }
+}
+
+__device__ __forceinline__ int wrap(int a, int n)
+{
+    return a < 0 ? a + n : (a >= n ? a - n : a);
+}
+
+// ZouHeD3Q19_vx_x (Boundary_conditions.cpp:350-442) on one site, in registers.
+__device__ __forceinline__ void zouhe_d3q19(double (&f)[19], double vx, bool west)
+{
+    if (west) {
+        double sum_f_xminus = f[1] + f[7] + f[9] + f[11] + f[13];
+        double sum_f_xzero = f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18];
+        double rho = (2.0 * sum_f_xminus + sum_f_xzero) / (1.0 - vx);
+        double vy = 0.0, vz = 0.0;
+        f[1] = f[1] + rho * vx / 3.0;
+        f[7] = f[7] + rho * vx / 6.0 + 0.5 * rho * vy - 0.5 * (f[18] + f[3] + f[16] - f[15] - f[4] - f[17]);
+        f[9] = f[9] + rho * vx / 6.0 - 0.5 * rho * vy + 0.5 * (f[18] + f[3] + f[16] - f[15] - f[4] - f[17]);
+        f[11] = f[11] + rho * vx / 6.0 + 0.5 * rho * vz - 0.5 * (f[17] + f[5] + f[16] - f[15] - f[6] - f[18]);
+        f[13] = f[13] + rho * vx / 6.0 - 0.5 * rho * vz + 0.5 * (f[17] + f[5] + f[16] - f[15] - f[6] - f[18]);
+    } else {
+        double sum_f_xplus = f[2] + f[8] + f[10] + f[12] + f[14];
+        double sum_f_xzero = f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18];
+        double rho = (2.0 * sum_f_xplus + sum_f_xzero) / (1.0 - vx);
+        double vy = 0.0, vz = 0.0;
+        f[2] = f[2] - rho * vx / 3.0;
+        f[10] = f[10] - rho * vx / 6.0 + 0.5 * rho * vy - 0.5 * (f[18] + f[3] + f[16] - f[15] - f[4] - f[17]);
+        f[8] = f[8] - rho * vx / 6.0 - 0.5 * rho * vy + 0.5 * (f[18] + f[3] + f[16] - f[15] - f[4] - f[17]);
+        f[14] = f[14] - rho * vx / 6.0 + 0.5 * rho * vz - 0.5 * (f[17] + f[5] + f[16] - f[15] - f[6] - f[18]);
+        f[12] = f[12] - rho * vx / 6.0 - 0.5 * rho * vz + 0.5 * (f[17] + f[5] + f[16] - f[15] - f[6] - f[18]);
+    }
+}
+
+// ZouHeD2Q9_vx_x (Boundary_conditions.cpp:710-735) on one site.
+__device__ __forceinline__ void zouhe_d2q9(double (&f)[9], double vx)
+{
+    double rho = (f[0] + f[3] + f[4] + 2 * (f[2] + f[8] + f[6])) / (1. - vx);
+    f[1] = (rho / 9.0) * (1.0 + 3.0 * vx + 3.0 * vx * vx);
+    f[5] = (rho / 36.0) * (1.0 + 3.0 * vx + 3.0 * vx * vx);
+    f[7] = (rho / 36.0) * (1.0 + 3.0 * vx + 3.0 * vx * vx);
+}
+
+template <int Q>
+__device__ __forceinline__ void zouhe(double (&f)[Q], double vx, bool west)
+{
+    if constexpr (Q == 19) zouhe_d3q19(f, vx, west);
+    else zouhe_d2q9(f, vx);
+}
+
+// Gather the pre-collision populations of voxel (x,y,z): identity (first step
+// after an upload) or pull-stream with periodic wrap and bounce-back.
+template <int Q, bool STREAM, bool SOLID>
+__device__ __forceinline__ void gather(const double *__restrict__ src, const Geometry &g, uint32_t nb,
+                                       int x, int y, int z, double (&f)[Q])
+{
+    using V = VSet<Q>;
+    const long long v = x + (long long)g.nx * (y + (long long)g.ny * z);
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        if constexpr (!STREAM || i == 0) {
+            f[i] = src[i * g.plane + v];
+        } else {
+            if (SOLID && (nb & (1u << i))) {
+                f[i] = src[V::opp[i] * g.plane + v];
+            } else {
+                const int xs = V::cx[i] == 0 ? x : wrap(x - V::cx[i], g.nx);
+                const int ys = V::cy[i] == 0 ? y : wrap(y - V::cy[i], g.ny);
+                const int zs = V::cz[i] == 0 ? z : wrap(z - V::cz[i], g.nz);
+                f[i] = src[i * g.plane + xs + (long long)g.nx * (ys + (long long)g.ny * zs)];
+            }
+        }
+    });
+}
+
+// Boundary columns in registers: inlet, then outlet (the drivers' order,
+// main_stability-map_obstacle.cpp:92-94).  The outlet copies its neighbour
+// column's gathered (and, if that column is the inlet, Zou-He'd) state.
+template <int Q, bool STREAM>
+__device__ __forceinline__ void apply_bc(const StepArgs &a, int x, int y, int z, double (&f)[Q])
+{
+    using V = VSet<Q>;
+    if (x == a.bc.inlet_x) zouhe<Q>(f, a.bc.u_in, a.bc.inlet_west);
+    if (x == a.bc.outlet_x) {
+        const int xn = x + a.bc.outlet_dx;
+        const long long vn = xn + (long long)a.g.nx * (y + (long long)a.g.ny * z);
+        const uint32_t nbn = a.nbr[vn];
+        if (nbn & FLUID_BIT) {
+            double fa[Q];
+            gather<Q, STREAM, true>(a.src, a.g, nbn, xn, y, z, fa);
+            if (xn == a.bc.inlet_x) zouhe<Q>(fa, a.bc.u_in, a.bc.inlet_west);
+            const double rho_adj = density_exact<Q>(fa);
+            const double fator = a.bc.rho_out / rho_adj;
+            static_for<Q>([&](auto I) { f[(int)I] = fa[(int)I] * fator; });
+        } else {
+            static_for<Q>([&](auto I) { f[(int)I] = V::w[(int)I] * a.bc.rho_out; });
+        }
+    }
+}
+
+// General kernel: any geometry, boundary columns, 1-D voxel mapping.
+template <int Q, int COL, bool STREAM>
+__global__ __launch_bounds__(BLOCK) void k_step_general(StepArgs a)
+{
+    if (stopped(a.status, a.step)) return;
+    const long long nvox = (long long)a.g.nx * a.g.ny * a.g.nz;
+    const long long v = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    bool bad = false;
+    if (v < nvox) {
+        const uint32_t nb = a.nbr[v];
+        if (nb & FLUID_BIT) {
+            const int x = (int)(v % a.g.nx);
+            const long long yz = v / a.g.nx;
+            const int y = (int)(yz % a.g.ny);
+            const int z = (int)(yz / a.g.ny);
+            double f[Q];
+            gather<Q, STREAM, true>(a.src, a.g, nb, x, y, z, f);
+            apply_bc<Q, STREAM>(a, x, y, z, f);
+            double vx, vy, vz;
+            collide_exact<Q, COL>(f, a.k, vx, vy, vz);
+            if (a.dens_limit > 0.0) {
+                const double d = density_exact<Q>(f);
+                bad = !(fabs(d) <= a.dens_limit);
+            }
+            static_for<Q>([&](auto I) {
+                __builtin_nontemporal_store(f[(int)I], &a.dst[(int)I * a.g.plane + v]);
+            });
+        }
+    }
+    flag_density(a.status, a.step, bad);
+}
+
+// Periodic all-fluid kernel: 2-D grid (x chunks, rows), XCD-aware block order.
+template <int Q, int COL, bool STREAM>
+__global__ __launch_bounds__(BLOCK) void k_step_periodic(StepArgs a)
+{
+    if (stopped(a.status, a.step)) return;
+    // Blocks are dispatched round-robin over the 8 XCDs; renumber them so
+    // each XCD walks one contiguous band of rows (neighbouring blocks, which
+    // share the cache lines at x-chunk edges, then sit in one L2).
+    const unsigned nbx = gridDim.x, nb = gridDim.x * gridDim.y;
+    unsigned b = blockIdx.y * nbx + blockIdx.x;
+    if ((nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    const int bx = (int)(b % nbx);
+    const int row = (int)(b / nbx);
+    const int x = bx * BLOCK + (int)threadIdx.x;
+    bool bad = false;
+    if (x < a.g.nx) {
+        const int y = row % a.g.ny;
+        const int z = row / a.g.ny;
+        double f[Q];
+        gather<Q, STREAM, false>(a.src, a.g, 0u, x, y, z, f);
+        double vx, vy, vz;
+        collide_exact<Q, COL>(f, a.k, vx, vy, vz);
+        if (a.dens_limit > 0.0) {
+            const double d = density_exact<Q>(f);
+            bad = !(fabs(d) <= a.dens_limit);
+        }
+        const long long v = x + (long long)a.g.nx * row;
+        static_for<Q>([&](auto I) {
+            __builtin_nontemporal_store(f[(int)I], &a.dst[(int)I * a.g.plane + v]);
+        });
+    }
+    flag_density(a.status, a.step, bad);
+}
+
+// ---- monitors -------------------------------------------------------------
+struct MonArgs {
+    const double *__restrict__ src;
+    Geometry g;
+    const uint32_t *__restrict__ nbr;   // null: all fluid
+    const int *__restrict__ vox_of_fluid;  // null: identity
+    int n_fluid;
+    double *ke_node;      // per fluid site 0.5|u|^2 (ordered mode), or null
+    double *partial_ke;   // per block
+    double *partial_rho;  // per block max |rho|
+    int *nonfinite;       // set if some rho non-finite
+};
+
+template <int Q, bool STREAM>
+__global__ __launch_bounds__(BLOCK) void k_monitor(MonArgs a)
+{
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    double e = 0.0, r_abs = 0.0;
+    bool nf = false;
+    if (p < a.n_fluid) {
+        const long long v = a.vox_of_fluid ? a.vox_of_fluid[p] : p;
+        const int x = (int)(v % a.g.nx);
+        const long long yz = v / a.g.nx;
+        const int y = (int)(yz % a.g.ny);
+        const int z = (int)(yz / a.g.ny);
+        double f[Q];
+        if (a.nbr) gather<Q, STREAM, true>(a.src, a.g, a.nbr[v], x, y, z, f);
+        else gather<Q, STREAM, false>(a.src, a.g, 0u, x, y, z, f);
+        double rho, vx, vy, vz;
+        moments_exact<Q>(f, rho, vx, vy, vz);
+        e = ke_term<Q>(vx, vy, vz);
+        const double d = density_exact<Q>(f);
+        r_abs = fabs(d);
+        nf = !isfinite(d);
+        if (a.ke_node) a.ke_node[p] = e;
+    }
+    if (__any(nf) && (threadIdx.x & 63) == 0) atomicOr(a.nonfinite, 1);
+    __shared__ double s_e[BLOCK / 64], s_r[BLOCK / 64];
+    e = wave_sum(e);
+    r_abs = wave_max(r_abs);
+    if ((threadIdx.x & 63) == 0) {
+        s_e[threadIdx.x >> 6] = e;
+        s_r[threadIdx.x >> 6] = r_abs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double te = 0.0, tr = 0.0;
+        for (int w = 0; w < BLOCK / 64; w++) {
+            te += s_e[w];
+            tr = fmax(tr, s_r[w]);
+        }
+        a.partial_ke[blockIdx.x] = te;
+        a.partial_rho[blockIdx.x] = tr;
+    }
+}
+
+// Final reduction.  ordered: sequential sum of ke_node in fluid order (the
+// reference's loop); else: sum of the block partials in block order.
+// If check_step != ULLONG_MAX, applies the Taylor-Green map test
+// (E > factor * E0 or non-finite -> stop before step check_step) with E0
+// read from *e0.  out[0] = E, out[1] = max |rho|.
+__global__ void k_finish(const double *ke_node, const double *partial_ke, const double *partial_rho,
+                         int n_fluid, int n_blocks, int ordered, double *out, const double *e0,
+                         double factor, unsigned long long check_step, unsigned long long *status)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double e = 0;
+    if (ordered) {
+        for (int p = 0; p < n_fluid; p++) e = e + ke_node[p];
+    } else {
+        for (int b = 0; b < n_blocks; b++) e = e + partial_ke[b];
+    }
+    double r = 0.0;
+    for (int b = 0; b < n_blocks; b++) r = fmax(r, partial_rho[b]);
+    out[0] = e;
+    out[1] = r;
+    if (check_step != NO_STOP) {
+        if (e > factor * (*e0) || !isfinite(e)) {
+            atomicMin(&status[ST_STOP_STEP], check_step);
+            atomicOr(&status[ST_REASON], (unsigned long long)ERLBM_STOP_ENERGY);
+        }
+    }
+}
+
+// AoS (reference fluid order) <-> SoA voxel layout, optionally streaming.
+template <int Q, bool STREAM>
+__global__ __launch_bounds__(BLOCK) void k_to_aos(const double *__restrict__ src, Geometry g,
+                                                  const uint32_t *__restrict__ nbr,
+                                                  const int *__restrict__ vox_of_fluid, int n_fluid,
+                                                  double *__restrict__ aos)
+{
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= n_fluid) return;
+    const long long v = vox_of_fluid ? vox_of_fluid[p] : p;
+    const int x = (int)(v % g.nx);
+    const long long yz = v / g.nx;
+    const int y = (int)(yz % g.ny);
+    const int z = (int)(yz / g.ny);
+    double f[Q];
+    if (nbr) gather<Q, STREAM, true>(src, g, nbr[v], x, y, z, f);
+    else gather<Q, STREAM, false>(src, g, 0u, x, y, z, f);
+    static_for<Q>([&](auto I) { aos[(long long)p * Q + (int)I] = f[(int)I]; });
+}
+
+template <int Q>
+__global__ __launch_bounds__(BLOCK) void k_from_aos(const double *__restrict__ aos, Geometry g,
+                                                    const int *__restrict__ vox_of_fluid, int n_fluid,
+                                                    double *__restrict__ dst)
+{
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= n_fluid) return;
+    const long long v = vox_of_fluid ? vox_of_fluid[p] : p;
+    static_for<Q>([&](auto I) { dst[(int)I * g.plane + v] = aos[(long long)p * Q + (int)I]; });
+}
+
+__global__ void k_reset_status(unsigned long long *status)
+{
+    if (threadIdx.x == 0) {
+        status[ST_STOP_STEP] = NO_STOP;
+        status[ST_REASON] = 0;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct erlbm_lattice {
+    int q = 0, nx = 0, ny = 0, nz = 0;
+    long long nvox = 0, plane = 0;
+    int n_fluid = 0;
+    bool all_fluid = true;
+    int precision = ERLBM_FP64;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    double *buf[2] = {nullptr, nullptr};
+    int base = 0;             // buffer holding the uploaded state
+    bool uploaded = false;
+    long long launched = 0;   // steps enqueued since upload
+
+    uint32_t *nbr = nullptr;          // per voxel (always allocated)
+    int *vox_of_fluid = nullptr;      // null when all fluid
+    double *ke_node = nullptr, *partial_ke = nullptr, *partial_rho = nullptr;
+    double *scalars = nullptr;        // [0] E, [1] max|rho|, [2] E0
+    int *nonfinite = nullptr;
+    unsigned long long *status = nullptr;
+
+    double tau = 1.0, tau_nh = 1.0, rho0 = 1.0;
+    int col = ERLBM_COL_EXT_REG;
+    int inlet_x = -1;
+    double u_in = 0.0;
+    int outlet_x = -1;
+    double rho_out = 1.0;
+    double dens_limit = 0.0;
+    bool e0_valid = false;
+};
+
+namespace {
+
+int mon_blocks(const erlbm_lattice *h) { return (h->n_fluid + BLOCK - 1) / BLOCK; }
+
+Geometry geom(const erlbm_lattice *h) { return Geometry{h->nx, h->ny, h->nz, h->plane}; }
+
+// Steps actually run since the upload (launched, capped by a stop); syncs.
+int steps_run(erlbm_lattice *h, long long *out, int *reason)
+{
+    unsigned long long st[ST_WORDS];
+    ERLBM_HIP_TRY(hipMemcpyAsync(st, h->status, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+    ERLBM_HIP_TRY(hipStreamSynchronize(h->stream));
+    long long r = h->launched;
+    if (st[ST_STOP_STEP] != NO_STOP && (long long)st[ST_STOP_STEP] < r) r = (long long)st[ST_STOP_STEP];
+    *out = r;
+    if (reason) *reason = (int)st[ST_REASON];
+    return ERLBM_OK;
+}
+
+template <int Q>
+int launch_monitor(erlbm_lattice *h, const double *src, bool stream, bool ordered, double *out_dev,
+                   unsigned long long check_step, double factor)
+{
+    MonArgs m;
+    m.src = src;
+    m.g = geom(h);
+    m.nbr = h->all_fluid ? nullptr : h->nbr;
+    m.vox_of_fluid = h->vox_of_fluid;
+    m.n_fluid = h->n_fluid;
+    m.ke_node = ordered ? h->ke_node : nullptr;
+    m.partial_ke = h->partial_ke;
+    m.partial_rho = h->partial_rho;
+    m.nonfinite = h->nonfinite;
+    const int nb = mon_blocks(h);
+    ERLBM_HIP_TRY(hipMemsetAsync(h->nonfinite, 0, sizeof(int), h->stream));
+    if (stream) hipLaunchKernelGGL((k_monitor<Q, true>), dim3(nb), dim3(BLOCK), 0, h->stream, m);
+    else hipLaunchKernelGGL((k_monitor<Q, false>), dim3(nb), dim3(BLOCK), 0, h->stream, m);
+    ERLBM_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, h->stream, h->ke_node, h->partial_ke, h->partial_rho,
+                       h->n_fluid, nb, ordered ? 1 : 0, out_dev, h->scalars + 2, factor, check_step, h->status);
+    ERLBM_HIP_TRY(hipGetLastError());
+    return ERLBM_OK;
+}
+
+int monitor(erlbm_lattice *h, const double *src, bool stream, bool ordered, double *out_dev,
+            unsigned long long check_step = NO_STOP, double factor = 2.0)
+{
+    return h->q == 9 ? launch_monitor<9>(h, src, stream, ordered, out_dev, check_step, factor)
+                     : launch_monitor<19>(h, src, stream, ordered, out_dev, check_step, factor);
+}
+
+template <int Q, int COL>
+int launch_step_q(erlbm_lattice *h, const StepArgs &a, bool stream)
+{
+    if (h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0) {
+        const int gx = (h->nx + BLOCK - 1) / BLOCK;
+        const dim3 grid(gx, h->ny * h->nz);
+        if (stream) hipLaunchKernelGGL((k_step_periodic<Q, COL, true>), grid, dim3(BLOCK), 0, h->stream, a);
+        else hipLaunchKernelGGL((k_step_periodic<Q, COL, false>), grid, dim3(BLOCK), 0, h->stream, a);
+    } else {
+        const long long nb = (h->nvox + BLOCK - 1) / BLOCK;
+        if (stream) hipLaunchKernelGGL((k_step_general<Q, COL, true>), dim3((unsigned)nb), dim3(BLOCK), 0, h->stream, a);
+        else hipLaunchKernelGGL((k_step_general<Q, COL, false>), dim3((unsigned)nb), dim3(BLOCK), 0, h->stream, a);
+    }
+    ERLBM_HIP_TRY(hipGetLastError());
+    return ERLBM_OK;
+}
+
+template <int Q>
+int launch_step(erlbm_lattice *h, const StepArgs &a, bool stream)
+{
+    switch (h->col) {
+    case ERLBM_COL_BGK: return launch_step_q<Q, COL_BGK>(h, a, stream);
+    case ERLBM_COL_REG: return launch_step_q<Q, COL_REG>(h, a, stream);
+    case ERLBM_COL_EXT_REG: return launch_step_q<Q, COL_EXT_REG>(h, a, stream);
+    default: set_last_error_msg("collision operator not supported on this path"); return ERLBM_ERR_UNSUPPORTED;
+    }
+}
+
+}  // namespace
+
+// ---- C ABI ------------------------------------------------------------------
+extern "C" int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *mask, int precision,
+                                    void *stream, erlbm_lattice **out)
+{
+    if (!out || (q != 9 && q != 19) || nx < 1 || ny < 1 || nz < 1) {
+        set_last_error_msg("erlbm_lattice_create: bad q or size");
+        return ERLBM_ERR_ARG;
+    }
+    if (q == 9 && nz != 1) {
+        set_last_error_msg("erlbm_lattice_create: D2Q9 needs nz == 1");
+        return ERLBM_ERR_ARG;
+    }
+    if (precision != ERLBM_FP64) {
+        set_last_error_msg("erlbm_lattice_create: the plain lattice path stores FP64 (shifted precisions are "
+                           "served by the batched map path)");
+        return ERLBM_ERR_UNSUPPORTED;
+    }
+    const long long nvox = (long long)nx * ny * nz;
+    if (nvox > (1ll << 31) - 1) {
+        set_last_error_msg("erlbm_lattice_create: lattice too large");
+        return ERLBM_ERR_ARG;
+    }
+    auto *h = new erlbm_lattice();
+    h->q = q;
+    h->nx = nx;
+    h->ny = ny;
+    h->nz = nz;
+    h->nvox = nvox;
+    h->plane = (nvox + 63) / 64 * 64;
+    h->precision = precision;
+    h->stream = (hipStream_t)stream;
+    hipGetDevice(&h->device);
+
+    // bounce mask: bit i set when the pull source x - c_i is solid
+    std::vector<uint32_t> nbr(nvox, 0);
+    std::vector<int> vof;
+    int nf = 0;
+    const int *cx = q == 9 ? VSet<9>::cx : VSet<19>::cx;
+    const int *cy = q == 9 ? VSet<9>::cy : VSet<19>::cy;
+    const int *cz = q == 9 ? VSet<9>::cz : VSet<19>::cz;
+    for (long long v = 0; v < nvox; v++) {
+        if (mask && !mask[v]) continue;
+        nf++;
+        const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((long long)nx * ny));
+        uint32_t b = FLUID_BIT;
+        for (int i = 1; i < q; i++) {
+            const int xs = ((x - cx[i]) % nx + nx) % nx;
+            const int ys = ((y - cy[i]) % ny + ny) % ny;
+            const int zs = ((z - cz[i]) % nz + nz) % nz;
+            const long long vs = xs + (long long)nx * (ys + (long long)ny * zs);
+            if (mask && !mask[vs]) b |= 1u << i;
+        }
+        nbr[v] = b;
+    }
+    h->n_fluid = nf;
+    h->all_fluid = (nf == nvox);
+    if (!h->all_fluid) {
+        vof.reserve(nf);
+        for (long long v = 0; v < nvox; v++)
+            if (nbr[v] & FLUID_BIT) vof.push_back((int)v);
+    }
+    if (nf == 0) {
+        delete h;
+        set_last_error_msg("erlbm_lattice_create: no fluid site");
+        return ERLBM_ERR_ARG;
+    }
+    auto fail = [&](hipError_t e, const char *what) {
+        set_last_error(what, e);
+        erlbm_lattice_destroy(h);
+        return ERLBM_ERR_NOMEM;
+    };
+    hipError_t e;
+    const size_t fbytes = sizeof(double) * (size_t)q * h->plane;
+    if ((e = hipMalloc(&h->buf[0], fbytes)) != hipSuccess) return fail(e, "hipMalloc populations");
+    if ((e = hipMalloc(&h->buf[1], fbytes)) != hipSuccess) return fail(e, "hipMalloc populations");
+    if ((e = hipMalloc(&h->nbr, sizeof(uint32_t) * nvox)) != hipSuccess) return fail(e, "hipMalloc mask");
+    if (!h->all_fluid && (e = hipMalloc(&h->vox_of_fluid, sizeof(int) * nf)) != hipSuccess)
+        return fail(e, "hipMalloc map");
+    const int nb = mon_blocks(h);
+    if ((e = hipMalloc(&h->ke_node, sizeof(double) * nf)) != hipSuccess) return fail(e, "hipMalloc monitor");
+    if ((e = hipMalloc(&h->partial_ke, sizeof(double) * nb)) != hipSuccess) return fail(e, "hipMalloc monitor");
+    if ((e = hipMalloc(&h->partial_rho, sizeof(double) * nb)) != hipSuccess) return fail(e, "hipMalloc monitor");
+    if ((e = hipMalloc(&h->scalars, sizeof(double) * 4)) != hipSuccess) return fail(e, "hipMalloc monitor");
+    if ((e = hipMalloc(&h->nonfinite, sizeof(int))) != hipSuccess) return fail(e, "hipMalloc monitor");
+    if ((e = hipMalloc(&h->status, sizeof(unsigned long long) * ST_WORDS)) != hipSuccess)
+        return fail(e, "hipMalloc status");
+    if ((e = hipMemcpyAsync(h->nbr, nbr.data(), sizeof(uint32_t) * nvox, hipMemcpyHostToDevice, h->stream)) !=
+        hipSuccess)
+        return fail(e, "upload mask");
+    if (!h->all_fluid &&
+        (e = hipMemcpyAsync(h->vox_of_fluid, vof.data(), sizeof(int) * nf, hipMemcpyHostToDevice, h->stream)) !=
+            hipSuccess)
+        return fail(e, "upload map");
+    if ((e = hipMemsetAsync(h->buf[0], 0, fbytes, h->stream)) != hipSuccess) return fail(e, "clear");
+    if ((e = hipMemsetAsync(h->buf[1], 0, fbytes, h->stream)) != hipSuccess) return fail(e, "clear");
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, h->stream, h->status);
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return fail(e, "sync");
+    *out = h;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_lattice_destroy(erlbm_lattice *h)
+{
+    if (!h) return ERLBM_OK;
+    if (h->stream || h->buf[0]) hipStreamSynchronize(h->stream);
+    hipFree(h->buf[0]);
+    hipFree(h->buf[1]);
+    hipFree(h->nbr);
+    hipFree(h->vox_of_fluid);
+    hipFree(h->ke_node);
+    hipFree(h->partial_ke);
+    hipFree(h->partial_rho);
+    hipFree(h->scalars);
+    hipFree(h->nonfinite);
+    hipFree(h->status);
+    delete h;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_lattice_fluid_count(const erlbm_lattice *h, int *n_fluid)
+{
+    if (!h || !n_fluid) return ERLBM_ERR_ARG;
+    *n_fluid = h->n_fluid;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_set_params(erlbm_lattice *h, double tau, double tau_nh, double rho0, int collision)
+{
+    if (!h) return ERLBM_ERR_ARG;
+    if (collision != ERLBM_COL_BGK && collision != ERLBM_COL_REG && collision != ERLBM_COL_EXT_REG) {
+        set_last_error_msg("erlbm_set_params: collision must be BGK, REG or EXT_REG");
+        return collision == ERLBM_COL_MRT ? ERLBM_ERR_UNSUPPORTED : ERLBM_ERR_ARG;
+    }
+    if (!(tau > 0.0) || !(tau_nh > 0.0)) {
+        set_last_error_msg("erlbm_set_params: tau and tau_nh must be positive");
+        return ERLBM_ERR_ARG;
+    }
+    h->tau = tau;
+    h->tau_nh = tau_nh;
+    h->rho0 = rho0;
+    h->col = collision;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_set_zouhe_inlet(erlbm_lattice *h, int pos_x, double vx)
+{
+    if (!h || pos_x >= h->nx) return ERLBM_ERR_ARG;
+    h->inlet_x = pos_x < 0 ? -1 : pos_x;
+    h->u_in = vx;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_set_outlet_rescale(erlbm_lattice *h, int pos_x, double rho)
+{
+    if (!h || pos_x >= h->nx) return ERLBM_ERR_ARG;
+    if (pos_x >= 0) {
+        const int dx = pos_x < h->nx / 2 ? 1 : -1;
+        if (pos_x + dx < 0 || pos_x + dx >= h->nx) {
+            set_last_error_msg("erlbm_set_outlet_rescale: neighbour column outside the lattice");
+            return ERLBM_ERR_ARG;
+        }
+    }
+    h->outlet_x = pos_x < 0 ? -1 : pos_x;
+    h->rho_out = rho;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_set_density_check(erlbm_lattice *h, double limit)
+{
+    if (!h) return ERLBM_ERR_ARG;
+    h->dens_limit = limit > 0 ? limit : 0.0;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_upload_aos(erlbm_lattice *h, const double *inif)
+{
+    if (!h || !inif) return ERLBM_ERR_ARG;
+    // stage in buffer 1, transpose into buffer 0
+    const size_t bytes = sizeof(double) * (size_t)h->n_fluid * h->q;
+    ERLBM_HIP_TRY(hipMemcpyAsync(h->buf[1], inif, bytes, hipMemcpyHostToDevice, h->stream));
+    const int nb = mon_blocks(h);
+    if (h->q == 9)
+        hipLaunchKernelGGL(k_from_aos<9>, dim3(nb), dim3(BLOCK), 0, h->stream, h->buf[1], geom(h), h->vox_of_fluid,
+                           h->n_fluid, h->buf[0]);
+    else
+        hipLaunchKernelGGL(k_from_aos<19>, dim3(nb), dim3(BLOCK), 0, h->stream, h->buf[1], geom(h),
+                           h->vox_of_fluid, h->n_fluid, h->buf[0]);
+    ERLBM_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, h->stream, h->status);
+    ERLBM_HIP_TRY(hipGetLastError());
+    h->base = 0;
+    h->launched = 0;
+    h->uploaded = true;
+    // E0 for the energy test: the reference's energy_ini (ordered sum)
+    int rc = monitor(h, h->buf[0], false, true, h->scalars + 2);
+    if (rc) return rc;
+    ERLBM_HIP_TRY(hipStreamSynchronize(h->stream));
+    h->e0_valid = true;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, double energy_factor)
+{
+    if (!h || n_steps < 0) return ERLBM_ERR_ARG;
+    if (!h->uploaded) {
+        set_last_error_msg("erlbm_step: upload populations first");
+        return ERLBM_ERR_STATE;
+    }
+    StepArgs a;
+    a.g = geom(h);
+    a.nbr = h->nbr;
+    a.k.omega1 = 1.0 - 1.0 / h->tau;
+    a.k.omega2 = 1.0 - 1.0 / h->tau_nh;
+    a.k.oot = 1.0 / h->tau;
+    a.bc.inlet_x = h->inlet_x;
+    a.bc.inlet_west = h->inlet_x >= 0 && h->inlet_x < h->nx / 2;
+    a.bc.u_in = h->u_in;
+    a.bc.outlet_x = h->outlet_x;
+    a.bc.outlet_dx = (h->outlet_x >= 0 && h->outlet_x < h->nx / 2) ? 1 : -1;
+    a.bc.rho_out = h->rho_out;
+    a.dens_limit = h->dens_limit;
+    a.status = h->status;
+    for (int s = 0; s < n_steps; s++) {
+        const long long k = h->launched;
+        const int cur = (h->base + (int)(k & 1)) & 1;
+        if (energy_interval > 0 && k > 0 && k % energy_interval == 0) {
+            const bool ordered = h->n_fluid <= (1 << 20);
+            int rc = monitor(h, h->buf[cur], true, ordered, h->scalars, (unsigned long long)k, energy_factor);
+            if (rc) return rc;
+        }
+        a.src = h->buf[cur];
+        a.dst = h->buf[cur ^ 1];
+        a.step = (unsigned long long)k;
+        int rc = h->q == 9 ? launch_step<9>(h, a, k > 0) : launch_step<19>(h, a, k > 0);
+        if (rc) return rc;
+        h->launched++;
+    }
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_step_timed(erlbm_lattice *h, int n_steps, float *ms)
+{
+    if (!h || !ms || n_steps < 0) return ERLBM_ERR_ARG;
+    hipEvent_t e0, e1;
+    ERLBM_HIP_TRY(hipEventCreate(&e0));
+    ERLBM_HIP_TRY(hipEventCreate(&e1));
+    int rc = ERLBM_OK;
+    if (hipEventRecord(e0, h->stream) != hipSuccess) rc = ERLBM_ERR_HIP;
+    if (rc == ERLBM_OK) rc = erlbm_step(h, n_steps, 0, 0.0);
+    if (rc == ERLBM_OK && hipEventRecord(e1, h->stream) != hipSuccess) rc = ERLBM_ERR_HIP;
+    if (rc == ERLBM_OK && hipEventSynchronize(e1) != hipSuccess) rc = ERLBM_ERR_HIP;
+    if (rc == ERLBM_OK && hipEventElapsedTime(ms, e0, e1) != hipSuccess) rc = ERLBM_ERR_HIP;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc == ERLBM_ERR_HIP) set_last_error_msg("erlbm_step_timed: HIP event failure");
+    return rc;
+}
+
+extern "C" int erlbm_status(erlbm_lattice *h, long long *steps_done, int *stop_reason)
+{
+    if (!h) return ERLBM_ERR_ARG;
+    long long r = 0;
+    int reason = 0;
+    int rc = steps_run(h, &r, &reason);
+    if (rc) return rc;
+    if (steps_done) *steps_done = r;
+    if (stop_reason) *stop_reason = reason;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_synchronize(erlbm_lattice *h)
+{
+    if (!h) return ERLBM_ERR_ARG;
+    ERLBM_HIP_TRY(hipStreamSynchronize(h->stream));
+    return ERLBM_OK;
+}
+
+namespace {
+
+// Buffer holding h_{r-1} (post-collision of the last step) or, if no step
+// ran, the uploaded ref_0.
+int current(erlbm_lattice *h, long long *run, int *cur)
+{
+    int rc = steps_run(h, run, nullptr);
+    if (rc) return rc;
+    *cur = (h->base + (int)(*run & 1)) & 1;
+    return ERLBM_OK;
+}
+
+}  // namespace
+
+extern "C" int erlbm_download_aos(erlbm_lattice *h, int which, double *inif)
+{
+    if (!h || !inif) return ERLBM_ERR_ARG;
+    if (!h->uploaded) return ERLBM_ERR_STATE;
+    long long run;
+    int cur;
+    int rc = current(h, &run, &cur);
+    if (rc) return rc;
+    if (which == ERLBM_POST_COLLISION && run == 0) {
+        set_last_error_msg("erlbm_download_aos: no step has run since the upload");
+        return ERLBM_ERR_STATE;
+    }
+    const bool stream = (which == ERLBM_STATE) && run > 0;
+    double *stage = h->buf[cur ^ 1];
+    const int nb = mon_blocks(h);
+    const uint32_t *nbr = h->all_fluid ? nullptr : h->nbr;
+    if (h->q == 9) {
+        if (stream)
+            hipLaunchKernelGGL((k_to_aos<9, true>), dim3(nb), dim3(BLOCK), 0, h->stream, h->buf[cur], geom(h), nbr,
+                               h->vox_of_fluid, h->n_fluid, stage);
+        else
+            hipLaunchKernelGGL((k_to_aos<9, false>), dim3(nb), dim3(BLOCK), 0, h->stream, h->buf[cur], geom(h), nbr,
+                               h->vox_of_fluid, h->n_fluid, stage);
+    } else {
+        if (stream)
+            hipLaunchKernelGGL((k_to_aos<19, true>), dim3(nb), dim3(BLOCK), 0, h->stream, h->buf[cur], geom(h), nbr,
+                               h->vox_of_fluid, h->n_fluid, stage);
+        else
+            hipLaunchKernelGGL((k_to_aos<19, false>), dim3(nb), dim3(BLOCK), 0, h->stream, h->buf[cur], geom(h),
+                               nbr, h->vox_of_fluid, h->n_fluid, stage);
+    }
+    ERLBM_HIP_TRY(hipGetLastError());
+    ERLBM_HIP_TRY(hipMemcpyAsync(inif, stage, sizeof(double) * (size_t)h->n_fluid * h->q, hipMemcpyDeviceToHost,
+                                 h->stream));
+    ERLBM_HIP_TRY(hipStreamSynchronize(h->stream));
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_kinetic_energy(erlbm_lattice *h, int which, int ordered, double *energy)
+{
+    if (!h || !energy) return ERLBM_ERR_ARG;
+    if (!h->uploaded) return ERLBM_ERR_STATE;
+    long long run;
+    int cur;
+    int rc = current(h, &run, &cur);
+    if (rc) return rc;
+    if (which == ERLBM_POST_COLLISION && run == 0) return ERLBM_ERR_STATE;
+    const bool stream = (which == ERLBM_STATE) && run > 0;
+    rc = monitor(h, h->buf[cur], stream, ordered != 0, h->scalars);
+    if (rc) return rc;
+    ERLBM_HIP_TRY(hipMemcpyAsync(energy, h->scalars, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    ERLBM_HIP_TRY(hipStreamSynchronize(h->stream));
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_density_monitor(erlbm_lattice *h, int which, double thresh, double *max_abs_rho,
+                                     int *violated)
+{
+    if (!h) return ERLBM_ERR_ARG;
+    if (!h->uploaded) return ERLBM_ERR_STATE;
+    long long run;
+    int cur;
+    int rc = current(h, &run, &cur);
+    if (rc) return rc;
+    if (which == ERLBM_POST_COLLISION && run == 0) return ERLBM_ERR_STATE;
+    const bool stream = (which == ERLBM_STATE) && run > 0;
+    rc = monitor(h, h->buf[cur], stream, false, h->scalars);
+    if (rc) return rc;
+    double s[2];
+    int nf = 0;
+    ERLBM_HIP_TRY(hipMemcpyAsync(s, h->scalars, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+    ERLBM_HIP_TRY(hipMemcpyAsync(&nf, h->nonfinite, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    ERLBM_HIP_TRY(hipStreamSynchronize(h->stream));
+    if (max_abs_rho) *max_abs_rho = s[1];
+    if (violated) *violated = (nf != 0) || (thresh > 0 && s[1] > thresh);
+    return ERLBM_OK;
+}

@@ -1,0 +1,160 @@
+"""Host-side mirror of the SimBoltz lattice interface over the HIP C ABI.
+
+The reference's drivers own ``GEOMETRY``/``LATTICE``/``PARAMETERS`` structs
+(SimBoltz_Functions/Definitions.cpp:34-239) and call per-site functions in
+their step loops.  ``Lattice`` keeps those concepts -- geometry mask, fluid
+numbering, tau/tau_nh/rho_ini, collision choice, the two column boundary
+conditions, the energy and density monitors -- but a step is one call into
+``liberlbm.so`` (include/erlbm.h) that runs the whole sweep on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, lib
+
+
+class Lattice:
+    """One lattice resident in HBM (FP64, SoA over voxels).
+
+    q: 9 (D2Q9) or 19 (D3Q19).  mask[x + y nx + z nx ny] = 1 fluid / 0 solid
+    (None: all fluid, periodic in every direction).  Populations cross in the
+    reference's AoS fluid order, shape (n_fluid, q).
+    """
+
+    def __init__(self, q: int, nx: int, ny: int, nz: int = 1, mask: Optional[np.ndarray] = None,
+                 stream: int = 0):
+        self.q, self.nx, self.ny, self.nz = q, nx, ny, nz
+        h = C.c_void_p()
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.int32).reshape(-1)
+        if m is not None and m.size != nx * ny * nz:
+            raise ValueError("mask size does not match nx*ny*nz")
+        mp = None if m is None else m.ctypes.data_as(C.c_void_p)
+        check(lib().erlbm_lattice_create(q, nx, ny, nz, mp, L.FP64, C.c_void_p(stream or None),
+                                         C.byref(h)), "erlbm_lattice_create")
+        self._h = h
+        self._mask = m
+        n = C.c_int(0)
+        check(lib().erlbm_lattice_fluid_count(h, C.byref(n)), "erlbm_lattice_fluid_count")
+        self.n_fluid = n.value
+
+    # -- lifecycle --------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().erlbm_lattice_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- parameters / boundaries -----------------------------------------------
+    def set_params(self, tau: float, tau_nh: float, rho0: float = 1.0, collision: int = L.COL_EXT_REG) -> None:
+        check(lib().erlbm_set_params(self._h, tau, tau_nh, rho0, collision), "erlbm_set_params")
+
+    def set_zouhe_inlet(self, pos_x: int, vx: float) -> None:
+        check(lib().erlbm_set_zouhe_inlet(self._h, pos_x, vx), "erlbm_set_zouhe_inlet")
+
+    def set_outlet_rescale(self, pos_x: int, rho: float) -> None:
+        check(lib().erlbm_set_outlet_rescale(self._h, pos_x, rho), "erlbm_set_outlet_rescale")
+
+    def set_density_check(self, limit: float) -> None:
+        check(lib().erlbm_set_density_check(self._h, limit), "erlbm_set_density_check")
+
+    # -- transfer ----------------------------------------------------------------
+    def upload(self, inif: np.ndarray) -> None:
+        a = np.ascontiguousarray(inif, dtype=np.float64).reshape(-1)
+        if a.size != self.n_fluid * self.q:
+            raise ValueError(f"expected {self.n_fluid * self.q} populations, got {a.size}")
+        check(lib().erlbm_upload_aos(self._h, a), "erlbm_upload_aos")
+
+    def download(self, which: int = L.STATE) -> np.ndarray:
+        out = np.empty(self.n_fluid * self.q, dtype=np.float64)
+        check(lib().erlbm_download_aos(self._h, which, out), "erlbm_download_aos")
+        return out.reshape(self.n_fluid, self.q)
+
+    # -- stepping ----------------------------------------------------------------
+    def step(self, n: int = 1, energy_interval: int = 0, energy_factor: float = 2.0) -> None:
+        check(lib().erlbm_step(self._h, n, energy_interval, energy_factor), "erlbm_step")
+
+    def step_timed(self, n: int) -> float:
+        """n steps between two HIP events on the lattice's stream; milliseconds."""
+        ms = C.c_float(0)
+        check(lib().erlbm_step_timed(self._h, n, C.byref(ms)), "erlbm_step_timed")
+        return ms.value
+
+    def status(self):
+        s, r = C.c_longlong(0), C.c_int(0)
+        check(lib().erlbm_status(self._h, C.byref(s), C.byref(r)), "erlbm_status")
+        return s.value, r.value
+
+    def synchronize(self) -> None:
+        check(lib().erlbm_synchronize(self._h), "erlbm_synchronize")
+
+    # -- monitors ----------------------------------------------------------------
+    def kinetic_energy(self, which: int = L.STATE, ordered: bool = True) -> float:
+        e = C.c_double(0)
+        check(lib().erlbm_kinetic_energy(self._h, which, int(ordered), C.byref(e)), "erlbm_kinetic_energy")
+        return e.value
+
+    def density_monitor(self, which: int = L.STATE, thresh: float = 0.0):
+        r, v = C.c_double(0), C.c_int(0)
+        check(lib().erlbm_density_monitor(self._h, which, thresh, C.byref(r), C.byref(v)), "erlbm_density_monitor")
+        return r.value, bool(v.value)
+
+
+# ---- host-side driver helpers (C ABI, libm on the host) ------------------------
+def dist_eq(q: int, vx: float, vy: float, vz: float, rho: float) -> np.ndarray:
+    out = np.empty(q)
+    check(lib().erlbm_dist_eq(q, vx, vy, vz, rho, out), "erlbm_dist_eq")
+    return out
+
+
+def init_taylor_green(q: int, L_: int, u0: float, tau: float, poisson: bool = True, neq: bool = True) -> np.ndarray:
+    f = np.empty(L_ * L_ * q)
+    check(lib().erlbm_init_taylor_green(q, L_, u0, tau, int(poisson), int(neq), f), "erlbm_init_taylor_green")
+    return f.reshape(L_ * L_, q)
+
+
+def init_double_shear(L_: int, rho: float = 1.0) -> np.ndarray:
+    f = np.empty(L_ * L_ * 9)
+    check(lib().erlbm_init_double_shear(L_, rho, f), "erlbm_init_double_shear")
+    return f.reshape(L_ * L_, 9)
+
+
+def init_tg_map(L_: int, u0: float, rho: float = 1.0) -> np.ndarray:
+    f = np.empty(L_ * L_ * 19)
+    check(lib().erlbm_init_tg_map(L_, u0, rho, f), "erlbm_init_tg_map")
+    return f.reshape(L_ * L_, 19)
+
+
+def obstacle_mask() -> np.ndarray:
+    m = np.empty(22 * 12, np.int32)
+    check(lib().erlbm_obstacle_mask(m), "erlbm_obstacle_mask")
+    return m
+
+
+def tg_map_point_params(i, j, n_ma=400, n_nu=400, L_=16, collision=L.COL_EXT_REG, tau_nh=0.65):
+    u, t, tn, n = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+    check(lib().erlbm_tg_map_point_params(i, j, n_ma, n_nu, L_, collision, tau_nh, C.byref(u), C.byref(t),
+                                          C.byref(tn), C.byref(n)), "erlbm_tg_map_point_params")
+    return u.value, t.value, tn.value, n.value
+
+
+def obstacle_map_point_params(i, j, n_tau=400, n_nu=400, ma=0.1):
+    u, t, tn, n = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+    check(lib().erlbm_obstacle_map_point_params(i, j, n_tau, n_nu, ma, C.byref(u), C.byref(t), C.byref(tn),
+                                                C.byref(n)), "erlbm_obstacle_map_point_params")
+    return u.value, t.value, tn.value, n.value

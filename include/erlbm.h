@@ -1,0 +1,156 @@
+/*
+ * erlbm.h -- C ABI of the MI355X-native E-RLBM collide-stream library.
+ *
+ * The reference (loesantos/ERLBM, SimBoltz) exposes per-site C++ functions
+ * that its drivers call inside their own step loops.  A per-site call has no
+ * meaning on a GPU, so this boundary sits at step-loop granularity: one
+ * lattice handle owns its populations in HBM and a call runs the drivers'
+ * whole step  [Zou-He inlet] -> [outlet rescale] -> collide (+check) ->
+ * stream -> swap.  Each entry point names the reference interface it
+ * replaces (path:line in the reference checkout).
+ *
+ * Conventions
+ *   - every function returns ERLBM_OK (0) or an ERLBM_ERR_* code; the text of
+ *     the last error is available from erlbm_last_error();
+ *   - host arrays are borrowed for the duration of the call only;
+ *   - a handle is bound to the HIP stream given at creation (NULL = default
+ *     stream); distinct handles may be driven from distinct host threads;
+ *   - populations cross the boundary in the reference's own layout: AoS,
+ *     f[pto * q + i], pto = fluid index in x-fastest voxel order
+ *     (numera_fluido, src/erlbm_common.h:89-99), velocity order of
+ *     def_lattice_d2q9 / def_lattice_d3q19.
+ */
+#ifndef ERLBM_H
+#define ERLBM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ERLBM_OK = 0,
+    ERLBM_ERR_ARG = 1,     /* invalid argument */
+    ERLBM_ERR_HIP = 2,     /* HIP runtime error */
+    ERLBM_ERR_STATE = 3,   /* call not valid in the handle's current state */
+    ERLBM_ERR_NOMEM = 4,   /* device memory exhausted / over budget */
+    ERLBM_ERR_UNSUPPORTED = 5
+};
+
+/* Collision operators: the COLISAO enum of src/erlbm_common.h:42. */
+enum { ERLBM_COL_BGK = 1, ERLBM_COL_MRT = 2, ERLBM_COL_REG = 3, ERLBM_COL_EXT_REG = 4 };
+
+/* Population storage precision. */
+enum { ERLBM_FP64 = 0, ERLBM_FP32_SHIFTED = 1, ERLBM_FP16_SHIFTED = 2 };
+
+/* Which populations a monitor reads. */
+enum {
+    ERLBM_STATE = 0,          /* the populations the reference's inif holds between steps */
+    ERLBM_POST_COLLISION = 1  /* post-collision populations of the last step, before the
+                                 push (what main_double-shear-layer.cpp:187-196 and
+                                 main_taylor-green.cpp:176-213 measure) */
+};
+
+/* Stop reasons reported by erlbm_status. */
+enum { ERLBM_STOP_DENSITY = 1, ERLBM_STOP_ENERGY = 2 };
+
+typedef struct erlbm_lattice erlbm_lattice;
+
+/* Text of the last error on this thread ("" if none). */
+const char *erlbm_last_error(void);
+
+/* Number of HIP devices visible (0 when no GPU). */
+int erlbm_device_count(int *count);
+
+/* ---- lattice lifecycle ---------------------------------------------------
+ * Replaces geometria_periodica / numera_fluido / define_rede /
+ * aloca_populacoes (src/erlbm_common.h:89-148) and def_dir_prop
+ * (SimBoltz_Functions/LBM_functions.cpp:1967-2311): the stream map becomes a
+ * per-voxel bounce mask (periodic wrap, halfway bounce-back off solids).
+ * q in {9 (D2Q9, nz = 1), 19 (D3Q19)}; mask[x + y nx + z nx ny] = 1 fluid,
+ * 0 solid; mask = NULL means all fluid.  stream: a hipStream_t or NULL.   */
+int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *mask, int precision,
+                         void *stream, erlbm_lattice **out);
+/* libera_populacoes, src/erlbm_common.h:150-157. */
+int erlbm_lattice_destroy(erlbm_lattice *h);
+/* geometry.fluid (Definitions.cpp:44). */
+int erlbm_lattice_fluid_count(const erlbm_lattice *h, int *n_fluid);
+
+/* ---- parameters and boundary conditions ----------------------------------
+ * PARAMETERS.tau / tau_nh / rho_ini (Definitions.cpp:119-239) and the
+ * collision choice of colisao_sitio (src/erlbm_common.h:68-81).            */
+int erlbm_set_params(erlbm_lattice *h, double tau, double tau_nh, double rho0, int collision);
+/* ZouHeD3Q19_vx_x (Boundary_conditions.cpp:350-442) / ZouHeD2Q9_vx_x
+ * (:710-735) applied at column pos_x before every collision; pos_x < 0
+ * disables it.                                                              */
+int erlbm_set_zouhe_inlet(erlbm_lattice *h, int pos_x, double vx);
+/* devnull_rho_x (Boundary_conditions.cpp:68-141) at column pos_x after the
+ * inlet and before every collision; pos_x < 0 disables it.                */
+int erlbm_set_outlet_rescale(erlbm_lattice *h, int pos_x, double rho);
+/* Per-step post-collision density test of main_stability-map_obstacle.cpp:
+ * 104-106: a step whose some site has |rho| > limit or non-finite rho stops
+ * the run after that step completes.  limit <= 0 disables it.             */
+int erlbm_set_density_check(erlbm_lattice *h, double limit);
+
+/* ---- transfer --------------------------------------------------------------
+ * Host AoS in reference fluid order, n_fluid * q doubles.  Upload resets the
+ * run (step counter, stop state).                                          */
+int erlbm_upload_aos(erlbm_lattice *h, const double *inif);
+/* which = ERLBM_STATE or ERLBM_POST_COLLISION (the latter needs >= 1 step). */
+int erlbm_download_aos(erlbm_lattice *h, int which, double *inif);
+
+/* ---- stepping ----------------------------------------------------------------
+ * n_steps repetitions of the drivers' step (e.g.
+ * main_double-shear-layer.cpp:162-229, main_stability-map_obstacle.cpp:
+ * 88-118): [inlet] -> [outlet] -> collide (+density check) -> stream ->
+ * swap.  Asynchronous on the handle's stream.  If energy_interval > 0, the
+ * Taylor-Green map check of main_stability-map_taylor-green.cpp:71-81 runs
+ * before every step whose global index is a positive multiple of
+ * energy_interval: the run stops there if E > energy_factor * E0 or E is not
+ * finite, E0 being the energy of the uploaded state.                       */
+int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, double energy_factor);
+/* erlbm_step(h, n_steps, 0, 0) bracketed by HIP events recorded on the
+ * handle's stream; *ms = elapsed milliseconds (synchronises).            */
+int erlbm_step_timed(erlbm_lattice *h, int n_steps, float *ms);
+/* Global steps run so far and the stop reason (0 if none); synchronises. */
+int erlbm_status(erlbm_lattice *h, long long *steps_done, int *stop_reason);
+int erlbm_synchronize(erlbm_lattice *h);
+
+/* ---- monitors ------------------------------------------------------------------
+ * kinectic_energy (Other_functions.cpp:100-115): sum over fluid sites of
+ * 0.5 |u|^2, FP64.  ordered != 0 sums in fluid order exactly as the
+ * reference does (bitwise equal); ordered == 0 uses a fixed-shape
+ * wavefront/block tree (reproducible, differs in the last bits).         */
+int erlbm_kinetic_energy(erlbm_lattice *h, int which, int ordered, double *energy);
+/* density (LBM_functions.cpp:5543-5552) over all fluid sites of `which`:
+ * max |rho| and whether some rho is non-finite or above thresh.          */
+int erlbm_density_monitor(erlbm_lattice *h, int which, double thresh, double *max_abs_rho,
+                          int *violated);
+
+/* ---- host-side driver helpers ---------------------------------------------
+ * Initial conditions, geometries and map axes of the reference drivers,
+ * computed on the host with the reference's libm calls and operation order
+ * (the GPU then starts from bitwise the reference's populations).          */
+/* dist_eq (SimBoltz_Functions/LBM_functions.cpp:2944-3000): q doubles. */
+int erlbm_dist_eq(int q, double vx, double vy, double vz, double rho, double *feq);
+/* inicializa_taylor_green (src/main_taylor-green.cpp:57-100), L*L*q doubles. */
+int erlbm_init_taylor_green(int q, int L, double u_0, double tau, int poisson, int neq, double *inif);
+/* initial_double_shear_layer (src/main_double-shear-layer.cpp:48-66), D2Q9. */
+int erlbm_init_double_shear(int L, double rho_ini, double *inif);
+/* Taylor-Green map point (src/main_stability-map_taylor-green.cpp:49-61), D3Q19. */
+int erlbm_init_tg_map(int L, double u_0, double rho_ini, double *inif);
+/* sin / cos tables of that field (L entries each). */
+int erlbm_tg_map_tables(int L, double *sin_tab, double *cos_tab);
+/* geometria_obstaculo (src/main_stability-map_obstacle.cpp:42-70): 22*12 ints. */
+int erlbm_obstacle_mask(int *mask);
+/* Map axes (src/main_stability-map_taylor-green.cpp:134-164, :63;
+ * src/main_stability-map_obstacle.cpp:147-172, :84).                       */
+int erlbm_tg_map_point_params(int i, int j, int n_ma, int n_nu, int L, int collision, double tau_nh,
+                              double *u_0, double *tau, double *tau_nh_out, int *n_steps);
+int erlbm_obstacle_map_point_params(int i, int j, int n_tau, int n_nu, double Ma, double *u_imp, double *tau,
+                                    double *tau_nh, int *n_steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ERLBM_H */

@@ -1,0 +1,198 @@
+"""Writes the golden fixtures under tests/golden/ (run in the container that
+holds the reference checkout; the fixtures are committed, this script is how
+they were made).
+
+Sources, all of them the reference's own:
+  * the reference library and drivers compiled from source into oracle/_ref
+    (oracle/Makefile `ref` target): strict (-ffp-contract=off) and fma builds;
+  * the published data files under data/ (tables copied as CSV text, the
+    three 400 x 400 stability maps bit-packed).
+
+    python tests/golden/make_golden.py [--maps]
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import shutil
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+
+REF = "/root/reference"
+SEED = 20261203
+
+
+def read_vtk_map(path):
+    lines = open(path).read().split("\n")
+    nx, ny = [int(v) for v in next(ln for ln in lines if ln.startswith("DIMENSIONS")).split()[1:3]]
+    k = next(n for n, ln in enumerate(lines) if ln.startswith("LOOKUP_TABLE"))
+    vals = np.array(" ".join(lines[k + 1:]).split(), dtype=float)
+    return vals[: nx * ny].reshape(ny, nx).astype(np.uint8)
+
+
+def published():
+    for name in ("fig3_taylor_green_Re50.csv", "fig3_taylor_green_Re50_feq_init.csv",
+                 "fig4_double_shear_Re80000.csv", "kappa4_check_L64.csv"):
+        shutil.copyfile(os.path.join(REF, "data", name), os.path.join(HERE, name))
+    maps = {}
+    for key, name in (("tg_erlbm065", "fig1_map_erlbm065.vtk"), ("tg_reg", "fig1_map_reg.vtk"),
+                      ("obstacle", "fig2_map_obstacle.vtk")):
+        m = read_vtk_map(os.path.join(REF, "data", name))
+        assert m.shape == (400, 400)
+        maps[key] = np.packbits(m.reshape(-1))
+    np.savez_compressed(os.path.join(HERE, "published_maps.npz"), **maps)
+
+
+def perturbed_sites(q, n, rng, o):
+    """Seeded non-equilibrium sites: f_eq(rho, u) (1 + 1e-3 xi), xi ~ U(-1, 1)."""
+    out = np.zeros((n, q))
+    for s in range(n):
+        rho = 1.0 + 0.05 * rng.uniform(-1, 1)
+        u = 0.1 * rng.uniform(-1, 1, 3)
+        if q == 9:
+            u[2] = 0.0
+        out[s] = o.dist_eq(q, u[0], u[1], u[2], rho) * (1 + 1e-3 * rng.uniform(-1, 1, q))
+    return out
+
+
+def per_function():
+    o = O.Oracle()
+    rng = np.random.default_rng(SEED)
+    data = {}
+    for q in (9, 19):
+        rs, rf = O.Reference(q, fma=False), O.Reference(q, fma=True)
+        f = perturbed_sites(q, 64, rng, o)
+        data[f"sites_q{q}"] = f
+        for col in (O.COL_BGK, O.COL_REG, O.COL_EXT_REG):
+            for tag, acc in (("", (0.0, 0.0, 0.0)), ("_forced", (1e-4, -2e-4, 3e-5 if q == 19 else 0.0))):
+                data[f"coll_q{q}_c{col}{tag}_strict"] = rs.collide(f, col, 0.62, 0.65, acc)
+                data[f"coll_q{q}_c{col}{tag}_fma"] = rf.collide(f, col, 0.62, 0.65, acc)
+        data[f"calcula_q{q}"] = np.array([rs.calcula(s) for s in f])
+        data[f"density_q{q}"] = np.array([rs.density(s) for s in f])
+        data[f"dist_eq_q{q}"] = np.array([rs.dist_eq(0.03, -0.04, 0.02 if q == 19 else 0.0, 1.02)])
+        # stream maps: periodic, random masks, obstacle
+        geoms = [(4, 4, 1, None), (7, 5, 1, (rng.uniform(size=35) > 0.3).astype(np.int32))]
+        if q == 19:
+            geoms.append((5, 4, 3, (rng.uniform(size=60) > 0.3).astype(np.int32)))
+            ini, n = rs.obstacle_geometry()
+            geoms.append((22, 12, 1, (ini > 0).astype(np.int32)))
+        for k, (nx, ny, nz, m) in enumerate(geoms):
+            ini, st, nf = rs.stream_map(nx, ny, nz, m)
+            data[f"stream_q{q}_{k}_dims"] = np.array([nx, ny, nz, nf])
+            data[f"stream_q{q}_{k}_mask"] = np.ones(nx * ny * nz, np.int32) if m is None else m
+            data[f"stream_q{q}_{k}_ini"] = ini
+            data[f"stream_q{q}_{k}_map"] = st
+    # boundary routines on a 6 x 4 x 2 (D3Q19) / 6 x 4 (D2Q9) channel
+    for q in (9, 19):
+        rs = O.Reference(q)
+        nz = 2 if q == 19 else 1
+        nx, ny = 6, 4
+        m = np.ones(nx * ny * nz, np.int32)
+        m[0 + nx * 1] = 0  # a solid voxel next to the west outlet column x=1
+        ini, st, nf = rs.stream_map(nx, ny, nz, m)
+        f = perturbed_sites(q, nf, rng, o)
+        data[f"bc_q{q}_mask"] = m
+        data[f"bc_q{q}_in"] = f
+        data[f"bc_q{q}_devnull_east"] = rs.boundary(0, nx, ny, nz, m, f, nx - 2, 1.0)
+        data[f"bc_q{q}_devnull_west"] = rs.boundary(0, nx, ny, nz, m, f, 1, 1.01)
+        if q == 19:
+            data["bc_q19_zouhe_west"] = rs.boundary(1, nx, ny, nz, m, f, 1, 0.05)
+            data["bc_q19_zouhe_east"] = rs.boundary(1, nx, ny, nz, m, f, nx - 2, 0.05)
+        else:
+            data["bc_q9_zouhe"] = rs.boundary(2, nx, ny, nz, m, f, 1, 0.05)
+        data[f"bc_q{q}_ke"] = np.array([rs.kinetic_energy(nx, ny, nz, m, f)])
+    np.savez_compressed(os.path.join(HERE, "ref_functions.npz"), **data)
+
+
+def trajectories():
+    """Multi-step runs of the reference's step loop (strict and fma builds)."""
+    o = O.Oracle()
+    data = {}
+    r9s, r9f = O.Reference(9), O.Reference(9, fma=True)
+    r19s, r19f = O.Reference(19), O.Reference(19, fma=True)
+    # D2Q9 Taylor-Green L=16, f_eq + f1, E-RLBM, 64 steps
+    f0 = r9s.init_taylor_green(16, 2 / 16, 0.62)
+    data["tg16_in"] = f0
+    for tag, r in (("strict", r9s), ("fma", r9f)):
+        data[f"tg16_out_{tag}"] = r.run_steps(16, 16, 1, None, f0, 64, O.COL_EXT_REG, 0.62, 0.65)[0]
+    # D2Q9 double shear layer L=32, tau from Re=80000, 200 steps, RLBM and E-RLBM
+    f0 = r9s.init_double_shear(32)
+    tau = 0.5 + 3. * (1. / 32.) * 32 / 80000.
+    data["dsl32_in"] = f0
+    data["dsl32_tau"] = np.array([tau])
+    for col in (O.COL_REG, O.COL_EXT_REG):
+        for tag, r in (("strict", r9s), ("fma", r9f)):
+            data[f"dsl32_c{col}_out_{tag}"] = r.run_steps(32, 32, 1, None, f0, 200, col, tau,
+                                                          1.0 if col == O.COL_REG else 0.65)[0]
+    # D2Q9 channel with Zou-He inlet, outlet rescale and a bounce-back block
+    nx, ny = 24, 10
+    m = np.ones(nx * ny, np.int32)
+    m[[x + y * nx for y in range(ny) for x in (0, nx - 1)]] = 0
+    m[[x + y * nx for y in range(3, 7) for x in (8, 9)]] = 0
+    ini, st, nf = r9s.stream_map(nx, ny, 1, m)
+    f0 = np.array([o.dist_eq(9, 0, 0, 0, 1.0)] * nf)
+    data["ch9_mask"] = m
+    data["ch9_in"] = f0
+    for tag, r in (("strict", r9s), ("fma", r9f)):
+        out, n, v = r.run_steps(nx, ny, 1, m, f0, 300, O.COL_EXT_REG, 0.55, 0.7, inlet_x=1, u_in=0.05,
+                                outlet_x=nx - 2, rho_out=1.0, dens_limit=10.0)
+        data[f"ch9_out_{tag}"] = out
+        data[f"ch9_steps_{tag}"] = np.array([n, v])
+    # D3Q19 Taylor-Green map point field, 16x16x1, 300 steps, RLBM and E-RLBM
+    f0 = o.init_tg_map_d3q19(16, 0.2)
+    data["tgmap_in"] = f0
+    for col, tnh in ((O.COL_REG, 1.0), (O.COL_EXT_REG, 0.65)):
+        for tag, r in (("strict", r19s), ("fma", r19f)):
+            data[f"tgmap_c{col}_out_{tag}"] = r.run_steps(16, 16, 1, None, f0, 300, col, 0.503, tnh)[0]
+    # D3Q19 obstacle channel with the obstacle driver's boundary conditions
+    ini, nf = r19s.obstacle_geometry()
+    m = (ini > 0).astype(np.int32)
+    f0 = o.init_rest_d3q19(nf)
+    u_imp = 0.1 / np.sqrt(3.0)
+    data["obst_in"] = f0
+    for name, tau, tnh, nst in (("stable", 0.5 + 3 * 1e-2, 0.8, 400), ("unstable", 0.5 + 3e-5, 0.5, 4000)):
+        for tag, r in (("strict", r19s), ("fma", r19f)):
+            out, n, v = r.run_steps(22, 12, 1, m, f0, nst, O.COL_EXT_REG, tau, tnh, inlet_x=1, u_in=u_imp,
+                                    outlet_x=20, rho_out=1.0, dens_limit=10.0)
+            data[f"obst_{name}_out_{tag}"] = out
+            data[f"obst_{name}_steps_{tag}"] = np.array([n, v])
+        data[f"obst_{name}_params"] = np.array([tau, tnh, nst])
+    np.savez_compressed(os.path.join(HERE, "ref_trajectories.npz"), **data)
+
+
+def map_subsamples():
+    """Classifications by the reference's roda_ponto on the README's 20 x 20
+    subsample positions of the 400 x 400 maps (i = 20k + 19, j = 20k)."""
+    idx_i = np.array([20 * k + 19 for k in range(20) for _ in range(20)], np.int32)
+    idx_j = np.array([20 * k for _ in range(20) for k in range(20)], np.int32)
+    data = {"i": idx_i, "j": idx_j}
+    r19s, r19f = O.Reference(19), O.Reference(19, fma=True)
+    for tag, r in (("strict", r19s), ("fma", r19f)):
+        t = time.time()
+        data[f"tg_erlbm065_{tag}"] = r.tg_map_points(O.COL_EXT_REG, 0.65, idx_i, idx_j)
+        data[f"tg_reg_{tag}"] = r.tg_map_points(O.COL_REG, 1.0, idx_i, idx_j)
+        data[f"obstacle_{tag}"] = r.obstacle_map_points(idx_i, idx_j)
+        print(f"  map subsample ({tag}) {time.time() - t:.1f}s", flush=True)
+    np.savez_compressed(os.path.join(HERE, "ref_map_subsample.npz"), **data)
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--maps", action="store_true", help="also rerun the 20x20 map subsamples (minutes)")
+    a = ap.parse_args()
+    if not (O.Reference.available(9) and O.Reference.available(19, fma=True)):
+        O.build(ref=True)
+    published()
+    per_function()
+    trajectories()
+    if a.maps:
+        map_subsamples()
+    print("golden fixtures written to", HERE)

@@ -1,0 +1,148 @@
+"""Parity of the plain (one lattice per handle) HIP path against the oracle.
+
+FP64 integer-exact bar: every population bitwise equal to the strict oracle
+(the reference's operation order without FMA contraction); against the
+FMA-contracted build, max relative population error <= 1e-10 after 1000
+steps, as the target states.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_FMA = 1e-10
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def _periodic(gpu, q, L, tau, tnh, col, f0, nz=1):
+    lat = gpu.Lattice(q, L, L, nz)
+    lat.set_params(tau, tnh, 1.0, col)
+    lat.upload(f0)
+    return lat
+
+
+@pytest.mark.parametrize("col", [O.COL_BGK, O.COL_REG, O.COL_EXT_REG])
+def test_d2q9_double_shear_bitwise(gpu, orc, col):
+    L, n = 64, 200
+    print(f"d2q9 dsl L={L} col={col} steps={n}", flush=True)
+    tau = 0.5 + 3. * (1. / 32.) * L / 80000.
+    tnh = {O.COL_BGK: tau, O.COL_REG: 1.0, O.COL_EXT_REG: 0.65}[col]
+    f0 = gpu.init_double_shear(L)
+    ini, st, nf = orc.stream_map(9, L, L, 1)
+    want = orc.run_steps(9, f0, st, n, col, tau, tnh, nthreads=4)
+    with _periodic(gpu, 9, L, tau, tnh, col, f0) as lat:
+        lat.step(n)
+        got = lat.download()
+        assert lat.status() == (n, 0)
+    assert np.array_equal(got, want)
+
+
+def test_upload_download_roundtrip(gpu, orc):
+    f0 = gpu.init_double_shear(48)
+    with _periodic(gpu, 9, 48, 0.6, 0.65, O.COL_EXT_REG, f0) as lat:
+        assert np.array_equal(lat.download(), f0)
+        lat.step(1)
+        post = lat.download(gpu.POST_COLLISION)
+        want = orc.collide(f0, O.COL_EXT_REG, 0.6, 0.65)
+        assert np.array_equal(post, want)
+
+
+def test_taylor_green_published_error(gpu, orc):
+    """Config: D2Q9 TG L=64, tau=0.62, E-RLBM tau_nh=0.65, f_eq+f1 init, t=T.
+    data/fig3_taylor_green_Re50.csv:7 publishes E_vector 0.000298896."""
+    L = 64
+    u0 = 2.0 / L
+    visc = u0 * L / 50.0
+    tau = 0.5 + 3.0 * visc
+    nT = int(round(L / u0))
+    f0 = gpu.init_taylor_green(9, L, u0, tau)
+    with _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0) as lat:
+        lat.step(nT + 1)
+        post = lat.download(gpu.POST_COLLISION)
+    ev, em, _ = orc.tg_error_d2q9(post, L, u0, visc, nT)
+    assert "%.6g" % ev == "0.000298896"
+    assert "%.6g" % em == "0.000298724"
+
+
+@pytest.mark.parametrize("col,tnh", [(O.COL_REG, 1.0), (O.COL_EXT_REG, 0.65)])
+def test_d3q19_tg_map_field_bitwise_and_energy(gpu, orc, col, tnh):
+    L, n = 16, 500
+    f0 = gpu.init_tg_map(L, 0.3)
+    ini, st, nf = orc.stream_map(19, L, L, 1)
+    want = orc.run_steps(19, f0, st, n, col, 0.5003, tnh)
+    with _periodic(gpu, 19, L, 0.5003, tnh, col, f0) as lat:
+        assert lat.kinetic_energy(gpu.STATE, ordered=True) == orc.kinetic_energy(19, f0)
+        lat.step(n)
+        got = lat.download()
+        assert np.array_equal(got, want)
+        assert lat.kinetic_energy(gpu.STATE, ordered=True) == orc.kinetic_energy(19, want)
+        e_fast = lat.kinetic_energy(gpu.STATE, ordered=False)
+        assert abs(e_fast - orc.kinetic_energy(19, want)) <= 1e-13 * abs(e_fast)
+
+
+def test_obstacle_channel_bitwise(gpu, orc, golden):
+    g = golden("ref_trajectories.npz")
+    m = gpu.obstacle_mask()
+    for name in ("stable", "unstable"):
+        tau, tnh, nst = g[f"obst_{name}_params"]
+        print(f"obstacle {name}", flush=True)
+        with gpu.Lattice(19, 22, 12, 1, mask=m) as lat:
+            lat.set_params(tau, tnh, 1.0, O.COL_EXT_REG)
+            lat.set_zouhe_inlet(1, 0.1 / np.sqrt(3.0))
+            lat.set_outlet_rescale(20, 1.0)
+            lat.set_density_check(10.0)
+            lat.upload(g["obst_in"])
+            lat.step(int(nst))
+            steps, reason = lat.status()
+            got = lat.download()
+        want_steps, want_viol = g[f"obst_{name}_steps_strict"]
+        assert steps == want_steps
+        assert (reason == gpu.STOP_DENSITY) == bool(want_viol)
+        assert np.array_equal(got, g[f"obst_{name}_out_strict"], equal_nan=True)
+
+
+def test_d2q9_channel_with_boundaries(gpu, orc, golden):
+    g = golden("ref_trajectories.npz")
+    with gpu.Lattice(9, 24, 10, 1, mask=g["ch9_mask"]) as lat:
+        lat.set_params(0.55, 0.7, 1.0, O.COL_EXT_REG)
+        lat.set_zouhe_inlet(1, 0.05)
+        lat.set_outlet_rescale(22, 1.0)
+        lat.set_density_check(10.0)
+        lat.upload(g["ch9_in"])
+        lat.step(300)
+        assert lat.status() == (300, 0)
+        assert np.array_equal(lat.download(), g["ch9_out_strict"])
+
+
+def test_energy_check_stops_like_map_driver(gpu, orc):
+    """Unstable Taylor-Green map point: same stop step as roda_ponto."""
+    u0, tau, tnh = orc.tg_map_params(380, 5, col=O.COL_REG, tau_nh=1.0)
+    unstable, stop, _ = orc.run_tg_map_point(16, O.COL_REG, u0, tau, tnh)
+    assert unstable == 1
+    n_steps = int((100 * 16) / u0)
+    with _periodic(gpu, 19, 16, tau, tnh, O.COL_REG, gpu.init_tg_map(16, u0)) as lat:
+        lat.step(n_steps, energy_interval=100, energy_factor=2.0)
+        steps, reason = lat.status()
+    assert reason == gpu.STOP_ENERGY
+    assert steps == stop
+
+
+def test_fma_reference_tolerance_1000_steps(gpu, golden):
+    """1e-10 max relative population error against the FMA-contracted
+    reference after 1000 steps (D2Q9 DSL L=128)."""
+    import oracle
+    ofma = oracle.Oracle(fma=True)
+    L = 128
+    tau = 0.5 + 3. * (1. / 32.) * L / 80000.
+    f0 = gpu.init_double_shear(L)
+    ini, st, nf = ofma.stream_map(9, L, L, 1)
+    want = ofma.run_steps(9, f0, st, 1000, O.COL_EXT_REG, tau, 0.65, nthreads=8)
+    with _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0) as lat:
+        lat.step(1000)
+        got = lat.download()
+    assert _rel(got, want) <= TOL_FMA
