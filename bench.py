@@ -45,7 +45,7 @@ def cpu_baseline(seconds: float):
     """The reference's OpenMP step loop on a bounded sample of the same
     workload (the first n steps of the L=2048 run), all host cores."""
     import oracle
-    n_threads = os.cpu_count() or 1
+    n_threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
     f0 = oracle.Oracle().init_double_shear_d2q9(L)
     if oracle.Reference.available(9, fma=True):
         ref = oracle.Reference(9, fma=True)
@@ -75,6 +75,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="run only the calibration copy kernel (302 MB read + 302 MB written per launch)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -86,6 +88,13 @@ def main() -> int:
     if E.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
+    # plain copy of the bytes one step moves (half read, half written)
+    copy_bytes = BYTES_PER_NODE * L * L // 2
+    copy_ms = E.hbm_copy_bench(copy_bytes, 5 if a.copy_only else 20)
+    copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
+    if a.copy_only:
+        print(json.dumps({"copy_bytes_each_way": copy_bytes, "ms": copy_ms, "GBs": copy_gbs}))
+        return 0
     f0 = E.init_double_shear(L)
     lat = E.Lattice(9, L, L, 1)
     lat.set_params(TAU, TAU_NH, 1.0, E.COL_EXT_REG)
@@ -133,6 +142,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
+        "copy_kernel_GBs": round(copy_gbs, 1),
         "mass_rel_error": mass_err,
     }
     print(json.dumps(line))

@@ -26,6 +26,30 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _vp = C.c_void_p
 _i, _d, _ll = C.c_int, C.c_double, C.c_longlong
 
+INIT_REST, INIT_TAYLOR_GREEN = 0, 1
+FP_AUTO = 3
+
+
+class Job(C.Structure):
+    """erlbm_job (include/erlbm.h)."""
+    _fields_ = [("tau", C.c_double), ("tau_nh", C.c_double), ("u0", C.c_double), ("rho0", C.c_double),
+                ("energy_factor", C.c_double), ("dens_limit", C.c_double), ("n_steps", C.c_longlong),
+                ("check_every", C.c_int), ("init", C.c_int), ("collision", C.c_int), ("precision", C.c_int)]
+
+
+class JobResult(C.Structure):
+    """erlbm_job_result (include/erlbm.h)."""
+    _fields_ = [("unstable", C.c_int), ("steps", C.c_int), ("done", C.c_int), ("precision", C.c_int),
+                ("e0", C.c_double), ("energy", C.c_double)]
+
+
+JOB_DTYPE = np.dtype([("tau", "f8"), ("tau_nh", "f8"), ("u0", "f8"), ("rho0", "f8"), ("energy_factor", "f8"),
+                      ("dens_limit", "f8"), ("n_steps", "i8"), ("check_every", "i4"), ("init", "i4"),
+                      ("collision", "i4"), ("precision", "i4")])
+RESULT_DTYPE = np.dtype([("unstable", "i4"), ("steps", "i4"), ("done", "i4"), ("precision", "i4"), ("e0", "f8"),
+                         ("energy", "f8")])
+assert JOB_DTYPE.itemsize == C.sizeof(Job) and RESULT_DTYPE.itemsize == C.sizeof(JobResult)
+
 
 class ErlbmError(RuntimeError):
     def __init__(self, code: int, where: str, msg: str):
@@ -52,6 +76,13 @@ SIGNATURES = {
     "erlbm_synchronize": (_i, [_vp]),
     "erlbm_kinetic_energy": (_i, [_vp, _i, _i, C.POINTER(_d)]),
     "erlbm_density_monitor": (_i, [_vp, _i, _d, C.POINTER(_d), C.POINTER(_i)]),
+    "erlbm_hbm_copy_bench": (_i, [_ll, _i, C.POINTER(C.c_float)]),
+    "erlbm_batch_create": (_i, [_i, _i, _vp, _i, _i, _vp, C.POINTER(_vp)]),
+    "erlbm_batch_destroy": (_i, [_vp]),
+    "erlbm_batch_run": (_i, [_vp, _vp, _i, _vp]),
+    "erlbm_batch_set_launch": (_i, [_vp, _i, _i]),
+    "erlbm_tg_map_jobs": (_i, [_i, _i, _i, _d, _i, _i, _i, _i, _i, _vp]),
+    "erlbm_obstacle_map_jobs": (_i, [_i, _i, _d, _i, _i, _i, _i, _i, _vp]),
     "erlbm_dist_eq": (_i, [_i, _d, _d, _d, _d, _dp]),
     "erlbm_init_taylor_green": (_i, [_i, _i, _d, _d, _i, _i, _dp]),
     "erlbm_init_double_shear": (_i, [_i, _d, _dp]),
@@ -85,6 +116,13 @@ def check(rc: int, where: str) -> None:
     if rc != ERLBM_OK:
         msg = lib().erlbm_last_error()
         raise ErlbmError(rc, where, msg.decode() if msg else "")
+
+
+def hbm_copy_bench(nbytes: int, iters: int = 20) -> float:
+    """Milliseconds per launch of a plain copy of nbytes (read nbytes + write nbytes)."""
+    ms = C.c_float(0)
+    check(lib().erlbm_hbm_copy_bench(nbytes, iters, C.byref(ms)), "erlbm_hbm_copy_bench")
+    return ms.value
 
 
 def device_count() -> int:

@@ -1,7 +1,10 @@
 // lattice.hip -- plain (one lattice per handle) E-RLBM collide-stream path.
 //
-// Device layout: structure of arrays over voxels, f[i * plane + v] with
-// v = x + nx (y + ny z) and plane = n_voxels rounded up to 64 doubles.
+// Device layout: populations indexed f[i * si + row * sr + x], row = y + ny z.
+// Two layouts share this form: population planes (si = n_voxels padded,
+// sr = nx) and row blocks (si = nx, sr = q nx padded: the q populations of
+// one lattice row are contiguous, so a step streams through a few adjacent
+// row blocks instead of 2q far-apart planes).
 // Solid voxels hold no state.  A step is one fused kernel: pull-stream
 // (gather f_i from x - c_i, or the opposite population of x itself when
 // x - c_i is solid: halfway bounce-back), boundary columns in registers,
@@ -17,6 +20,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -57,8 +61,14 @@ constexpr uint32_t FLUID_BIT = 1u << 31;
 
 struct Geometry {
     int nx, ny, nz;
-    long long plane;
+    long long si;   // population stride
+    long long sr;   // row stride
 };
+
+__host__ __device__ __forceinline__ long long at(const Geometry &g, int i, long long row, int x)
+{
+    return i * g.si + row * g.sr + x;
+}
 
 struct BCArgs {
     int inlet_x;    // < 0: none
@@ -77,7 +87,8 @@ struct StepArgs {
     CollideConsts k;
     BCArgs bc;
     double dens_limit;                 // <= 0: no test
-    unsigned long long *status;
+    unsigned long long *status;        // written (atomics) by this launch
+    const unsigned long long *__restrict__ stop_ro;  // the same word, read as of the launch
     unsigned long long step;
 };
 
@@ -153,19 +164,19 @@ __device__ __forceinline__ void gather(const double *__restrict__ src, const Geo
                                        int x, int y, int z, double (&f)[Q])
 {
     using V = VSet<Q>;
-    const long long v = x + (long long)g.nx * (y + (long long)g.ny * z);
+    const long long row = y + (long long)g.ny * z;
     static_for<Q>([&](auto I) {
         constexpr int i = I;
         if constexpr (!STREAM || i == 0) {
-            f[i] = src[i * g.plane + v];
+            f[i] = src[at(g, i, row, x)];
         } else {
             if (SOLID && (nb & (1u << i))) {
-                f[i] = src[V::opp[i] * g.plane + v];
+                f[i] = src[at(g, V::opp[i], row, x)];
             } else {
                 const int xs = V::cx[i] == 0 ? x : wrap(x - V::cx[i], g.nx);
                 const int ys = V::cy[i] == 0 ? y : wrap(y - V::cy[i], g.ny);
                 const int zs = V::cz[i] == 0 ? z : wrap(z - V::cz[i], g.nz);
-                f[i] = src[i * g.plane + xs + (long long)g.nx * (ys + (long long)g.ny * zs)];
+                f[i] = src[at(g, i, ys + (long long)g.ny * zs, xs)];
             }
         }
     });
@@ -221,42 +232,126 @@ __global__ __launch_bounds__(BLOCK) void k_step_general(StepArgs a)
                 bad = !(fabs(d) <= a.dens_limit);
             }
             static_for<Q>([&](auto I) {
-                __builtin_nontemporal_store(f[(int)I], &a.dst[(int)I * a.g.plane + v]);
+                __builtin_nontemporal_store(f[(int)I], &a.dst[at(a.g, (int)I, yz, x)]);
             });
         }
     }
     flag_density(a.status, a.step, bad);
 }
 
-// Periodic all-fluid kernel: 2-D grid (x chunks, rows), XCD-aware block order.
-template <int Q, int COL, bool STREAM>
-__global__ __launch_bounds__(BLOCK) void k_step_periodic(StepArgs a)
+// Periodic all-fluid kernel: 2-D grid (x chunks of BS*NPT nodes, rows).
+// NPT nodes per thread (stride BS, so every load stays coalesced) raise the
+// loads in flight per wave; NTL selects nontemporal loads.  The stop test
+// reads the launch-time value of the stop word after the loads are issued,
+// so it costs no extra memory round trip.
+template <int Q, int COL, bool STREAM, int NPT, bool NTL, int BS>
+__global__ __launch_bounds__(BS) void k_step_periodic(StepArgs a, int remap)
 {
-    if (stopped(a.status, a.step)) return;
-    // Blocks are dispatched round-robin over the 8 XCDs; renumber them so
-    // each XCD walks one contiguous band of rows (neighbouring blocks, which
-    // share the cache lines at x-chunk edges, then sit in one L2).
+    using V = VSet<Q>;
     const unsigned nbx = gridDim.x, nb = gridDim.x * gridDim.y;
     unsigned b = blockIdx.y * nbx + blockIdx.x;
-    if ((nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
     const int bx = (int)(b % nbx);
     const int row = (int)(b / nbx);
-    const int x = bx * BLOCK + (int)threadIdx.x;
-    bool bad = false;
-    if (x < a.g.nx) {
-        const int y = row % a.g.ny;
-        const int z = row / a.g.ny;
-        double f[Q];
-        gather<Q, STREAM, false>(a.src, a.g, 0u, x, y, z, f);
-        double vx, vy, vz;
-        collide_exact<Q, COL>(f, a.k, vx, vy, vz);
-        if (a.dens_limit > 0.0) {
-            const double d = density_exact<Q>(f);
-            bad = !(fabs(d) <= a.dens_limit);
+    const int y = row % a.g.ny;
+    const int z = row / a.g.ny;
+    double f[NPT][Q];
+    int xs_[NPT];
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+        const int x = bx * BS * NPT + k * BS + (int)threadIdx.x;
+        xs_[k] = x;
+        if (x < a.g.nx) {
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                long long src;
+                if constexpr (!STREAM || i == 0) {
+                    src = at(a.g, i, row, x);
+                } else {
+                    const int xs = V::cx[i] == 0 ? x : wrap(x - V::cx[i], a.g.nx);
+                    const int ys = V::cy[i] == 0 ? y : wrap(y - V::cy[i], a.g.ny);
+                    const int zs = V::cz[i] == 0 ? z : wrap(z - V::cz[i], a.g.nz);
+                    src = at(a.g, i, ys + (long long)a.g.ny * zs, xs);
+                }
+                const double *p = &a.src[src];
+                if constexpr (NTL) f[k][i] = __builtin_nontemporal_load(p);
+                else f[k][i] = *p;
+            });
         }
-        const long long v = x + (long long)a.g.nx * row;
-        static_for<Q>([&](auto I) {
-            __builtin_nontemporal_store(f[(int)I], &a.dst[(int)I * a.g.plane + v]);
+    }
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+        const int x = xs_[k];
+        if (x < a.g.nx) {
+            double vx, vy, vz;
+            collide_exact<Q, COL>(f[k], a.k, vx, vy, vz);
+            if (a.dens_limit > 0.0) {
+                const double d = density_exact<Q>(f[k]);
+                bad = bad || !(fabs(d) <= a.dens_limit);
+            }
+            static_for<Q>([&](auto I) {
+                __builtin_nontemporal_store(f[k][(int)I], &a.dst[at(a.g, (int)I, row, x)]);
+            });
+        }
+    }
+    flag_density(a.status, a.step, bad);
+}
+
+// D2Q9 periodic kernel, two x-adjacent nodes per thread: every population
+// plane is read with one 16-byte load per thread (8-byte aligned when the
+// plane is x-shifted; multi-dword global loads need 4-byte alignment) and
+// written with one aligned 16-byte store.  nx must be even.
+typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+typedef double d2a __attribute__((ext_vector_type(2)));
+
+template <int COL, bool STREAM, bool NTL>
+__global__ __launch_bounds__(BLOCK) void k_step_d2q9_pair(StepArgs a, int remap)
+{
+    using V = VSet<9>;
+    if (stopped(a.status, a.step)) return;
+    const unsigned nbx = gridDim.x, nb = gridDim.x * gridDim.y;
+    unsigned b = blockIdx.y * nbx + blockIdx.x;
+    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    const int bx = (int)(b % nbx);
+    const int y = (int)(b / nbx);
+    const int x0 = 2 * (bx * BLOCK + (int)threadIdx.x);
+    bool bad = false;
+    if (x0 < a.g.nx) {
+        const int nx = a.g.nx, ny = a.g.ny;
+        double f[2][9];
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            int ys = y;
+            if constexpr (STREAM && V::cy[i] != 0) ys = wrap(y - V::cy[i], ny);
+            const double *row = a.src + at(a.g, i, ys, 0);
+            const int xs = (STREAM ? x0 - V::cx[i] : x0);
+            if (xs >= 0 && xs + 1 < nx) {
+                const d2u *p = reinterpret_cast<const d2u *>(row + xs);
+                d2u v;
+                if constexpr (NTL) v = __builtin_nontemporal_load(p);
+                else v = *p;
+                f[0][i] = v.x;
+                f[1][i] = v.y;
+            } else {
+                f[0][i] = row[wrap(xs, nx)];
+                f[1][i] = row[wrap(xs + 1, nx)];
+            }
+        });
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            double vx, vy, vz;
+            collide_exact<9, COL>(f[k], a.k, vx, vy, vz);
+            if (a.dens_limit > 0.0) {
+                const double d = density_exact<9>(f[k]);
+                bad = bad || !(fabs(d) <= a.dens_limit);
+            }
+        }
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            d2a w = {f[0][i], f[1][i]};
+            __builtin_nontemporal_store(w, reinterpret_cast<d2a *>(&a.dst[at(a.g, i, y, x0)]));
         });
     }
     flag_density(a.status, a.step, bad);
@@ -374,7 +469,9 @@ __global__ __launch_bounds__(BLOCK) void k_from_aos(const double *__restrict__ a
     const int p = blockIdx.x * BLOCK + threadIdx.x;
     if (p >= n_fluid) return;
     const long long v = vox_of_fluid ? vox_of_fluid[p] : p;
-    static_for<Q>([&](auto I) { dst[(int)I * g.plane + v] = aos[(long long)p * Q + (int)I]; });
+    const int x = (int)(v % g.nx);
+    const long long row = v / g.nx;
+    static_for<Q>([&](auto I) { dst[at(g, (int)I, row, x)] = aos[(long long)p * Q + (int)I]; });
 }
 
 __global__ void k_reset_status(unsigned long long *status)
@@ -390,7 +487,8 @@ __global__ void k_reset_status(unsigned long long *status)
 // ---------------------------------------------------------------------------
 struct erlbm_lattice {
     int q = 0, nx = 0, ny = 0, nz = 0;
-    long long nvox = 0, plane = 0;
+    long long nvox = 0, si = 0, sr = 0, extent = 0;
+    int layout = 1;
     int n_fluid = 0;
     bool all_fluid = true;
     int precision = ERLBM_FP64;
@@ -417,13 +515,23 @@ struct erlbm_lattice {
     double rho_out = 1.0;
     double dens_limit = 0.0;
     bool e0_valid = false;
+
+    // launch tuning (environment overrides read at creation):
+    //   ERLBM_PLANE_PAD  doubles added to each population plane's stride
+    //   ERLBM_NPT        nodes per thread of the periodic D2Q9 kernel (1, 2, 4)
+    //   ERLBM_NT_LOADS   1: nontemporal loads in that kernel
+    //   ERLBM_XCD_REMAP  1: renumber blocks so each XCD walks a band of rows
+    //   ERLBM_PAIR       1: D2Q9 kernel with two adjacent nodes per thread
+    long long tune_pad = 0;
+    //   ERLBM_BLOCK      threads per block of that kernel (128, 256, 512)
+    int tune_npt = 1, tune_ntl = 1, tune_remap = 0, tune_pair = 0, tune_block = 256;
 };
 
 namespace {
 
 int mon_blocks(const erlbm_lattice *h) { return (h->n_fluid + BLOCK - 1) / BLOCK; }
 
-Geometry geom(const erlbm_lattice *h) { return Geometry{h->nx, h->ny, h->nz, h->plane}; }
+Geometry geom(const erlbm_lattice *h) { return Geometry{h->nx, h->ny, h->nz, h->si, h->sr}; }
 
 // Steps actually run since the upload (launched, capped by a stop); syncs.
 int steps_run(erlbm_lattice *h, long long *out, int *reason)
@@ -470,14 +578,53 @@ int monitor(erlbm_lattice *h, const double *src, bool stream, bool ordered, doub
                      : launch_monitor<19>(h, src, stream, ordered, out_dev, check_step, factor);
 }
 
+template <int Q, int COL, bool STREAM, int NPT, bool NTL, int BS>
+void launch_periodic_bs(erlbm_lattice *h, const StepArgs &a)
+{
+    const int gx = (h->nx + BS * NPT - 1) / (BS * NPT);
+    const dim3 grid(gx, h->ny * h->nz);
+    hipLaunchKernelGGL((k_step_periodic<Q, COL, STREAM, NPT, NTL, BS>), grid, dim3(BS), 0, h->stream, a,
+                       h->tune_remap);
+}
+
+template <int Q, int COL, bool STREAM, int NPT, bool NTL>
+void launch_periodic(erlbm_lattice *h, const StepArgs &a)
+{
+    if constexpr (Q == 9 && NPT == 1) {
+        if (h->tune_block == 128) return launch_periodic_bs<Q, COL, STREAM, NPT, NTL, 128>(h, a);
+        if (h->tune_block == 512) return launch_periodic_bs<Q, COL, STREAM, NPT, NTL, 512>(h, a);
+    }
+    launch_periodic_bs<Q, COL, STREAM, NPT, NTL, BLOCK>(h, a);
+}
+
+template <int Q, int COL, bool STREAM>
+void launch_periodic_tuned(erlbm_lattice *h, const StepArgs &a)
+{
+    if constexpr (Q == 9) {
+        const bool ntl = h->tune_ntl != 0;
+        if (h->tune_pair && (h->nx % 2) == 0 && h->nz == 1) {
+            const int gx = (h->nx / 2 + BLOCK - 1) / BLOCK;
+            const dim3 grid(gx, h->ny);
+            if (ntl) hipLaunchKernelGGL((k_step_d2q9_pair<COL, STREAM, true>), grid, dim3(BLOCK), 0, h->stream, a, h->tune_remap);
+            else hipLaunchKernelGGL((k_step_d2q9_pair<COL, STREAM, false>), grid, dim3(BLOCK), 0, h->stream, a, h->tune_remap);
+            return;
+        }
+        switch (h->tune_npt) {
+        case 2: ntl ? launch_periodic<Q, COL, STREAM, 2, true>(h, a) : launch_periodic<Q, COL, STREAM, 2, false>(h, a); return;
+        case 4: ntl ? launch_periodic<Q, COL, STREAM, 4, true>(h, a) : launch_periodic<Q, COL, STREAM, 4, false>(h, a); return;
+        default: ntl ? launch_periodic<Q, COL, STREAM, 1, true>(h, a) : launch_periodic<Q, COL, STREAM, 1, false>(h, a); return;
+        }
+    } else {
+        launch_periodic<Q, COL, STREAM, 1, false>(h, a);
+    }
+}
+
 template <int Q, int COL>
 int launch_step_q(erlbm_lattice *h, const StepArgs &a, bool stream)
 {
     if (h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0) {
-        const int gx = (h->nx + BLOCK - 1) / BLOCK;
-        const dim3 grid(gx, h->ny * h->nz);
-        if (stream) hipLaunchKernelGGL((k_step_periodic<Q, COL, true>), grid, dim3(BLOCK), 0, h->stream, a);
-        else hipLaunchKernelGGL((k_step_periodic<Q, COL, false>), grid, dim3(BLOCK), 0, h->stream, a);
+        if (stream) launch_periodic_tuned<Q, COL, true>(h, a);
+        else launch_periodic_tuned<Q, COL, false>(h, a);
     } else {
         const long long nb = (h->nvox + BLOCK - 1) / BLOCK;
         if (stream) hipLaunchKernelGGL((k_step_general<Q, COL, true>), dim3((unsigned)nb), dim3(BLOCK), 0, h->stream, a);
@@ -528,7 +675,28 @@ extern "C" int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *ma
     h->ny = ny;
     h->nz = nz;
     h->nvox = nvox;
-    h->plane = (nvox + 63) / 64 * 64;
+    auto env_ll = [](const char *name, long long dflt) {
+        const char *v = getenv(name);
+        return (v && *v) ? atoll(v) : dflt;
+    };
+    h->tune_pad = env_ll("ERLBM_PLANE_PAD", h->tune_pad);
+    h->tune_npt = (int)env_ll("ERLBM_NPT", h->tune_npt);
+    h->tune_ntl = (int)env_ll("ERLBM_NT_LOADS", h->tune_ntl);
+    h->tune_remap = (int)env_ll("ERLBM_XCD_REMAP", h->tune_remap);
+    h->tune_pair = (int)env_ll("ERLBM_PAIR", h->tune_pair);
+    h->tune_block = (int)env_ll("ERLBM_BLOCK", h->tune_block);
+    if (h->tune_pad < 0 || h->tune_pad > (1 << 24)) h->tune_pad = 0;
+    h->layout = (int)env_ll("ERLBM_LAYOUT", h->layout);
+    const long long pad = (h->tune_pad + 63) / 64 * 64;
+    const long long nrows = (long long)ny * nz;
+    if (h->layout == 0) {   // population planes
+        h->si = (nvox + 63) / 64 * 64 + pad;
+        h->sr = nx;
+    } else {                // row blocks
+        h->si = nx;
+        h->sr = (long long)q * nx + pad;
+    }
+    h->extent = (q - 1) * h->si + (nrows - 1) * h->sr + nx;
     h->precision = precision;
     h->stream = (hipStream_t)stream;
     hipGetDevice(&h->device);
@@ -572,7 +740,7 @@ extern "C" int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *ma
         return ERLBM_ERR_NOMEM;
     };
     hipError_t e;
-    const size_t fbytes = sizeof(double) * (size_t)q * h->plane;
+    const size_t fbytes = sizeof(double) * (size_t)h->extent;
     if ((e = hipMalloc(&h->buf[0], fbytes)) != hipSuccess) return fail(e, "hipMalloc populations");
     if ((e = hipMalloc(&h->buf[1], fbytes)) != hipSuccess) return fail(e, "hipMalloc populations");
     if ((e = hipMalloc(&h->nbr, sizeof(uint32_t) * nvox)) != hipSuccess) return fail(e, "hipMalloc mask");
@@ -693,8 +861,9 @@ extern "C" int erlbm_upload_aos(erlbm_lattice *h, const double *inif)
     h->base = 0;
     h->launched = 0;
     h->uploaded = true;
-    // E0 for the energy test: the reference's energy_ini (ordered sum)
-    int rc = monitor(h, h->buf[0], false, true, h->scalars + 2);
+    // E0 for the energy test: the reference's energy_ini (ordered sum up to
+    // 2^20 sites, the fixed-shape tree above that)
+    int rc = monitor(h, h->buf[0], false, h->n_fluid <= (1 << 20), h->scalars + 2);
     if (rc) return rc;
     ERLBM_HIP_TRY(hipStreamSynchronize(h->stream));
     h->e0_valid = true;
@@ -722,6 +891,7 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
     a.bc.rho_out = h->rho_out;
     a.dens_limit = h->dens_limit;
     a.status = h->status;
+    a.stop_ro = h->status;
     for (int s = 0; s < n_steps; s++) {
         const long long k = h->launched;
         const int cur = (h->base + (int)(k & 1)) & 1;

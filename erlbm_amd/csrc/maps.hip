@@ -1,0 +1,673 @@
+// maps.hip -- batched tiny-lattice runs (stability maps and small requests).
+//
+// Thousands of independent tiny D3Q19 lattices (the Taylor-Green map's
+// 16x16x1 and the obstacle map's 22x12x1 channel, src/main_stability-map_*.cpp)
+// run in one persistent kernel.  Each workgroup owns one lattice at a time,
+// keeps its populations resident in LDS (19 x 256 x 8 B = 38.9 KB in FP64,
+// half that in FP32), runs it step after step and, on retirement (stop test
+// fired or step budget used), takes the next job from a device-side queue:
+// one atomic per job, no host round trip, early exit frees the workgroup at
+// once.  A launch may also stop after a budget of steps per job and save the
+// job to HBM, so a host loop can admit new work between chunks.
+//
+// Per step (the plain path's operation order): pull-gather from LDS with
+// periodic wrap and halfway bounce-back, inlet / outlet columns, collision,
+// the map driver's stop test -- the energy test every check_every steps
+// (ordered FP64 sum over the sites, main_stability-map_taylor-green.cpp:71-81)
+// or the post-collision density test every step
+// (main_stability-map_obstacle.cpp:104-106).  FP64 jobs are bitwise those of
+// the reference; FP32 jobs store shifted deviations (collide_f32.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "collide.hpp"
+#include "collide_f32.hpp"
+#include "common.hpp"
+#include "maps.hpp"
+
+using namespace erlbm;
+
+namespace {
+
+constexpr int Q = 19;
+constexpr int TPB = 256;  // threads per workgroup = max fluid sites of a tiny lattice
+constexpr uint32_t FLUID = 1u << 31;
+
+struct TinyGeom {
+    int nx, ny;                  // nz = 1
+    int n_fluid;                 // <= TPB
+    int nvp;                     // LDS plane stride (voxels, padded)
+    int general;                 // solids or boundary columns present
+    int inlet_x, outlet_x, outlet_dx;
+    const int *vox_of_fluid;     // [n_fluid]
+    const int *fluid_of_vox;     // [nx*ny], -1 = solid
+    const uint32_t *nbr;         // [nx*ny]: bit i = x - c_i solid, FLUID
+    const double *sin_tab;       // sin(2 pi x / L), x < max(nx, ny)
+    const double *cos_tab;
+};
+
+struct RunArgs {
+    TinyGeom g;
+    const MapJob *jobs;
+    MapResult *results;
+    const int *job_idx;          // jobs of this launch
+    int n_jobs;
+    unsigned int *next_job;      // queue head
+};
+
+// ---- storage policies -------------------------------------------------------
+struct F64 {
+    using T = double;
+    struct K {
+        CollideConsts c;
+    };
+    __device__ static K consts(const MapJob &J)
+    {
+        K k;
+        k.c.omega1 = 1.0 - 1.0 / J.tau;
+        k.c.omega2 = 1.0 - 1.0 / J.tau_nh;
+        k.c.oot = 1.0 / J.tau;
+        return k;
+    }
+    __device__ static void init(double (&f)[Q], double rho0, double ux, double uy)
+    {
+        feq_exact<Q>(rho0, ux, uy, 0.0, f);
+    }
+    __device__ static double ke(const double (&f)[Q], double)
+    {
+        double rho, vx, vy, vz;
+        moments_exact<Q>(f, rho, vx, vy, vz);
+        return ke_term<Q>(vx, vy, vz);
+    }
+    template <int COL>
+    __device__ static void collide(double (&f)[Q], const K &k)
+    {
+        double vx, vy, vz;
+        collide_exact<Q, COL>(f, k.c, vx, vy, vz);
+    }
+    __device__ static double density(const double (&f)[Q], double) { return density_exact<Q>(f); }
+    __device__ static void to_abs(const double (&f)[Q], double (&a)[Q], double)
+    {
+        for (int i = 0; i < Q; i++) a[i] = f[i];
+    }
+    __device__ static void from_abs(const double (&a)[Q], double (&f)[Q], double)
+    {
+        for (int i = 0; i < Q; i++) f[i] = a[i];
+    }
+};
+
+struct F32 {
+    using T = float;
+    struct K {
+        CollideConstsF32 c;
+    };
+    __device__ static K consts(const MapJob &J)
+    {
+        K k;
+        k.c.omega1 = (float)(1.0 - 1.0 / J.tau);
+        k.c.omega2 = (float)(1.0 - 1.0 / J.tau_nh);
+        k.c.oot = (float)(1.0 / J.tau);
+        k.c.rho0 = (float)J.rho0;
+        return k;
+    }
+    __device__ static void init(float (&d)[Q], double rho0, double ux, double uy)
+    {
+        double f[Q];
+        feq_exact<Q>(rho0, ux, uy, 0.0, f);
+        for (int i = 0; i < Q; i++) d[i] = (float)(f[i] - VSet<Q>::w[i] * rho0);
+    }
+    __device__ static double ke(const float (&d)[Q], double rho0)
+    {
+        float drho, rho, vx, vy, vz;
+        moments_shifted<Q>(d, (float)rho0, drho, rho, vx, vy, vz);
+        const double x = vx, y = vy, z = vz;
+        return 0.5 * (x * x + y * y + z * z);
+    }
+    template <int COL>
+    __device__ static void collide(float (&d)[Q], const K &k)
+    {
+        float drho, vx, vy, vz;
+        collide_shifted_f32<Q, COL>(d, k.c, drho, vx, vy, vz);
+    }
+    __device__ static double density(const float (&d)[Q], double rho0) { return density_shifted_f64<Q>(d, rho0); }
+    __device__ static void to_abs(const float (&d)[Q], double (&a)[Q], double rho0)
+    {
+        for (int i = 0; i < Q; i++) a[i] = (double)d[i] + VSet<Q>::w[i] * rho0;
+    }
+    __device__ static void from_abs(const double (&a)[Q], float (&d)[Q], double rho0)
+    {
+        for (int i = 0; i < Q; i++) d[i] = (float)(a[i] - VSet<Q>::w[i] * rho0);
+    }
+};
+
+// ZouHeD3Q19_vx_x (Boundary_conditions.cpp:350-442), west branch, on
+// absolute FP64 populations.
+__device__ __forceinline__ void zouhe_west(double (&f)[Q], double vx)
+{
+    double sum_f_xminus = f[1] + f[7] + f[9] + f[11] + f[13];
+    double sum_f_xzero = f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18];
+    double rho = (2.0 * sum_f_xminus + sum_f_xzero) / (1.0 - vx);
+    double vy = 0.0, vz = 0.0;
+    f[1] = f[1] + rho * vx / 3.0;
+    f[7] = f[7] + rho * vx / 6.0 + 0.5 * rho * vy - 0.5 * (f[18] + f[3] + f[16] - f[15] - f[4] - f[17]);
+    f[9] = f[9] + rho * vx / 6.0 - 0.5 * rho * vy + 0.5 * (f[18] + f[3] + f[16] - f[15] - f[4] - f[17]);
+    f[11] = f[11] + rho * vx / 6.0 + 0.5 * rho * vz - 0.5 * (f[17] + f[5] + f[16] - f[15] - f[6] - f[18]);
+    f[13] = f[13] + rho * vx / 6.0 - 0.5 * rho * vz + 0.5 * (f[17] + f[5] + f[16] - f[15] - f[6] - f[18]);
+}
+
+__device__ __forceinline__ void zouhe_east(double (&f)[Q], double vx)
+{
+    double sum_f_xplus = f[2] + f[8] + f[10] + f[12] + f[14];
+    double sum_f_xzero = f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18];
+    double rho = (2.0 * sum_f_xplus + sum_f_xzero) / (1.0 - vx);
+    double vy = 0.0, vz = 0.0;
+    f[2] = f[2] - rho * vx / 3.0;
+    f[10] = f[10] - rho * vx / 6.0 + 0.5 * rho * vy - 0.5 * (f[18] + f[3] + f[16] - f[15] - f[4] - f[17]);
+    f[8] = f[8] - rho * vx / 6.0 - 0.5 * rho * vy + 0.5 * (f[18] + f[3] + f[16] - f[15] - f[4] - f[17]);
+    f[14] = f[14] - rho * vx / 6.0 + 0.5 * rho * vz - 0.5 * (f[17] + f[5] + f[16] - f[15] - f[6] - f[18]);
+    f[12] = f[12] - rho * vx / 6.0 - 0.5 * rho * vz + 0.5 * (f[17] + f[5] + f[16] - f[15] - f[6] - f[18]);
+}
+
+// Pull-gather of voxel (x, y) from the voxel-indexed LDS populations
+// s[i * nvp + v]: x - c_i with periodic wrap, or the opposite population of
+// the voxel itself when x - c_i is solid (bit i of `bounce`).  stream =
+// false reads the voxel's own populations (the first step after init).
+template <typename T>
+__device__ __forceinline__ void gather(const T *s, int nvp, int x, int y, int nx, int ny, uint32_t bounce,
+                                       bool stream, T (&f)[Q])
+{
+    using V = VSet<Q>;
+    const int v = x + nx * y;
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        if (!stream || (V::cx[i] == 0 && V::cy[i] == 0)) {
+            f[i] = s[i * nvp + v];
+        } else if (bounce & (1u << i)) {
+            f[i] = s[V::opp[i] * nvp + v];
+        } else {
+            const int xs = V::cx[i] == 0 ? x : (V::cx[i] > 0 ? xm : xp);
+            const int ys = V::cy[i] == 0 ? y : (V::cy[i] > 0 ? ym : yp);
+            f[i] = s[i * nvp + xs + nx * ys];
+        }
+    });
+}
+
+// Ordered sum of n doubles held in LDS (the reference's sequential loop).
+__device__ __forceinline__ double ordered_sum(const double *v, int n)
+{
+    double s = 0;
+    for (int p = 0; p < n; p++) s = s + v[p];
+    return s;
+}
+
+// Energy of the gathered state, ordered over the sites; `scratch` aliases the
+// population LDS, so every gather of this step must be complete (barrier).
+template <class P>
+__device__ __forceinline__ double ordered_energy(typename P::T (&f)[Q], bool active, double rho0, double *scratch,
+                                                 int n_fluid, double *bcast)
+{
+    __syncthreads();
+    if (active) scratch[threadIdx.x] = P::ke(f, rho0);   // fluid order = thread order
+    __syncthreads();
+    if (threadIdx.x == 0) bcast[0] = ordered_sum(scratch, n_fluid);
+    __syncthreads();
+    return bcast[0];
+}
+
+// One job in one workgroup; thread p owns fluid site p (voxel vox_of_fluid[p]).
+// GENERAL: solids and/or boundary columns.
+template <class P, int COL, bool GENERAL>
+__device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typename P::T *sf, double *bcast)
+{
+    using T = typename P::T;
+    using V = VSet<Q>;
+    const int tid = threadIdx.x;
+    const int nf = a.g.n_fluid;
+    const int nx = a.g.nx, ny = a.g.ny, nvp = a.g.nvp;
+    const bool active = tid < nf;
+    const auto k = P::consts(J);
+    const double rho0 = J.rho0;
+
+    int my_v = 0, my_x = 0, my_y = 0;
+    uint32_t my_bounce = 0, out_bounce = 0;
+    bool out_fluid = false;
+    if (active) {
+        my_v = GENERAL ? a.g.vox_of_fluid[tid] : tid;
+        my_x = my_v % nx;
+        my_y = my_v / nx;
+        if constexpr (GENERAL) {
+            my_bounce = a.g.nbr[my_v] & ~FLUID;
+            if (my_x == a.g.outlet_x) {
+                const uint32_t nb = a.g.nbr[my_v + a.g.outlet_dx];
+                out_fluid = (nb & FLUID) != 0;
+                out_bounce = nb & ~FLUID;
+            }
+        }
+    }
+    // scratch for the ordered energy sums: the population LDS itself (only
+    // used between a completed gather and the next write)
+    double *scratch = reinterpret_cast<double *>(sf);
+
+    // ---- initial state and E0
+    T f[Q];
+    if (active) {
+        double ux = 0.0, uy = 0.0;
+        if (J.init == INIT_TAYLOR_GREEN) {
+            ux = J.u0 * a.g.sin_tab[my_x] * a.g.cos_tab[my_y];
+            uy = -J.u0 * a.g.cos_tab[my_x] * a.g.sin_tab[my_y];
+        }
+        P::init(f, rho0, ux, uy);
+    }
+    double e0 = 0.0;
+    if (J.check_every > 0) e0 = ordered_energy<P>(f, active, rho0, scratch, nf, bcast);
+    __syncthreads();
+    if (active)
+        for (int i = 0; i < Q; i++) sf[i * nvp + my_v] = f[i];
+    __syncthreads();
+
+    long long s = 0;
+    int unstable = 0;
+    double e_last = e0;
+    for (; s < J.n_steps; s++) {
+        const bool stream = s > 0;
+        double fator = 0.0;   // outlet column: rho0 / rho(upstream neighbour)
+        if (active) {
+            if (GENERAL && my_x == a.g.outlet_x && out_fluid) {
+                // devnull_rho_x copies the upstream neighbour (Boundary_conditions.cpp:117-126)
+                gather(sf, nvp, my_x + a.g.outlet_dx, my_y, nx, ny, out_bounce, stream, f);
+                double fn[Q];
+                P::to_abs(f, fn, rho0);
+                fator = rho0 / density_exact<Q>(fn);
+            } else {
+                gather(sf, nvp, my_x, my_y, nx, ny, my_bounce, stream, f);
+            }
+        }
+        if (J.check_every > 0 && s > 0 && s % J.check_every == 0) {
+            // (energy tests run on lattices without an outlet column; checked on the host)
+            const double e = ordered_energy<P>(f, active, rho0, scratch, nf, bcast);
+            e_last = e;
+            if (e > J.energy_factor * e0 || !isfinite(e)) {
+                unstable = 1;
+                break;
+            }
+        }
+        bool bad = false;
+        if (active) {
+            if constexpr (GENERAL) {
+                if (my_x == a.g.outlet_x) {
+                    double fd[Q];
+                    if (out_fluid) {
+                        P::to_abs(f, fd, rho0);
+                        for (int i = 0; i < Q; i++) fd[i] = fd[i] * fator;
+                    } else {
+                        for (int i = 0; i < Q; i++) fd[i] = V::w[i] * rho0;   // dist_eq(f, rho)
+                    }
+                    P::from_abs(fd, f, rho0);
+                } else if (my_x == a.g.inlet_x) {
+                    double fd[Q];
+                    P::to_abs(f, fd, rho0);
+                    if (a.g.inlet_x < nx / 2) zouhe_west(fd, J.u0);
+                    else zouhe_east(fd, J.u0);
+                    P::from_abs(fd, f, rho0);
+                }
+            }
+            P::template collide<COL>(f, k);
+            if (J.dens_limit > 0.0) {
+                const double d = P::density(f, rho0);
+                bad = !(fabs(d) <= J.dens_limit);
+            }
+        }
+        __syncthreads();  // every gather of this step is done
+        if (active)
+            for (int i = 0; i < Q; i++) sf[i * nvp + my_v] = f[i];
+        const int any_bad = __syncthreads_or(bad ? 1 : 0);
+        if (any_bad) {
+            unstable = 1;
+            s++;
+            break;
+        }
+    }
+    if (tid == 0) {
+        MapResult r;
+        r.unstable = unstable;
+        r.steps = (int)s;
+        r.done = 1;
+        r.precision = std::is_same<T, double>::value ? ERLBM_FP64 : ERLBM_FP32_SHIFTED;
+        r.e0 = e0;
+        r.energy = e_last;
+        *res = r;
+    }
+    __syncthreads();
+}
+
+template <class P, int COL, bool GENERAL>
+__global__ __launch_bounds__(TPB) void k_tiny(RunArgs a)
+{
+    extern __shared__ double lds_raw[];   // Q * nvp populations of type P::T
+    __shared__ double bcast[2];
+    __shared__ int sjob;
+    typename P::T *sf = reinterpret_cast<typename P::T *>(lds_raw);
+    for (;;) {
+        if (threadIdx.x == 0) sjob = (int)atomicAdd(a.next_job, 1u);
+        __syncthreads();
+        const int k = sjob;
+        __syncthreads();
+        if (k >= a.n_jobs) return;
+        const int j = a.job_idx[k];
+        const MapJob J = a.jobs[j];
+        run_job<P, COL, GENERAL>(a, J, &a.results[j], sf, bcast);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct erlbm_batch {
+    int nx = 0, ny = 0, n_fluid = 0;
+    bool general = false;
+    int inlet_x = -1, outlet_x = -1, outlet_dx = -1;
+    hipStream_t stream = nullptr;
+    int *vox_of_fluid = nullptr, *fluid_of_vox = nullptr;
+    uint32_t *nbr = nullptr;
+    double *tabs = nullptr;         // sin | cos tables
+    int tab_n = 0;
+    unsigned int *counters = nullptr;
+    int wg_per_cu[2] = {0, 0};      // FP64, FP32 (0 = occupancy max)
+    int n_cu = 0;
+    // job buffers (grown on demand)
+    MapJob *d_jobs = nullptr;
+    MapResult *d_results = nullptr;
+    int *d_idx = nullptr;
+    int cap = 0;
+};
+
+namespace {
+
+template <class P, int COL, bool GENERAL>
+int launch_group(erlbm_batch *b, const RunArgs &a, unsigned int *counter)
+{
+    int per_cu = 0;
+    const size_t lds = sizeof(typename P::T) * Q * (size_t)a.g.nvp;
+    ERLBM_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, GENERAL>, TPB, lds));
+    const int want = b->wg_per_cu[std::is_same<typename P::T, double>::value ? 0 : 1];
+    if (want > 0 && want < per_cu) per_cu = want;
+    if (per_cu < 1) per_cu = 1;
+    int grid = b->n_cu * per_cu;
+    if (grid > a.n_jobs) grid = a.n_jobs;
+    RunArgs ra = a;
+    ra.next_job = counter;
+    hipLaunchKernelGGL((k_tiny<P, COL, GENERAL>), dim3(grid), dim3(TPB), lds, b->stream, ra);
+    ERLBM_HIP_TRY(hipGetLastError());
+    return ERLBM_OK;
+}
+
+template <class P>
+int launch_col(erlbm_batch *b, int col, const RunArgs &a, unsigned int *counter)
+{
+    const bool g = b->general;
+    switch (col) {
+    case ERLBM_COL_BGK: return g ? launch_group<P, COL_BGK, true>(b, a, counter) : launch_group<P, COL_BGK, false>(b, a, counter);
+    case ERLBM_COL_REG: return g ? launch_group<P, COL_REG, true>(b, a, counter) : launch_group<P, COL_REG, false>(b, a, counter);
+    case ERLBM_COL_EXT_REG:
+        return g ? launch_group<P, COL_EXT_REG, true>(b, a, counter) : launch_group<P, COL_EXT_REG, false>(b, a, counter);
+    default: return ERLBM_ERR_UNSUPPORTED;
+    }
+}
+
+int resolve_precision(const MapJob &j)
+{
+    if (j.precision == ERLBM_FP_AUTO) return (j.tau - 0.5 < 1e-3) ? ERLBM_FP64 : ERLBM_FP32_SHIFTED;
+    return j.precision;
+}
+
+}  // namespace
+
+extern "C" int erlbm_batch_create(int nx, int ny, const int *mask, int inlet_x, int outlet_x, void *stream,
+                                  erlbm_batch **out)
+{
+    if (!out || nx < 1 || ny < 1 || inlet_x >= nx || outlet_x >= nx) return ERLBM_ERR_ARG;
+    const int nv = nx * ny;
+    std::vector<int> fov(nv, -1), vof;
+    std::vector<uint32_t> nbr(nv, 0);
+    for (int v = 0; v < nv; v++)
+        if (!mask || mask[v]) {
+            fov[v] = (int)vof.size();
+            vof.push_back(v);
+        }
+    const int nf = (int)vof.size();
+    if (nf < 1 || nf > TPB || nv > 400) {
+        set_last_error_msg("erlbm_batch_create: a tiny lattice needs 1..256 fluid sites in at most 400 voxels");
+        return ERLBM_ERR_ARG;
+    }
+    for (int v : vof) {
+        const int x = v % nx, y = v / nx;
+        uint32_t bits = FLUID;
+        for (int i = 1; i < Q; i++) {
+            const int xs = ((x - VSet<Q>::cx[i]) % nx + nx) % nx;
+            const int ys = ((y - VSet<Q>::cy[i]) % ny + ny) % ny;
+            if (fov[xs + nx * ys] < 0) bits |= 1u << i;
+        }
+        nbr[v] = bits;
+    }
+    auto *b = new erlbm_batch();
+    b->nx = nx;
+    b->ny = ny;
+    b->n_fluid = nf;
+    b->inlet_x = inlet_x;
+    b->outlet_x = outlet_x;
+    b->outlet_dx = (outlet_x >= 0 && outlet_x < nx / 2) ? 1 : -1;
+    if (outlet_x >= 0 && (outlet_x + b->outlet_dx < 0 || outlet_x + b->outlet_dx >= nx)) {
+        delete b;
+        set_last_error_msg("erlbm_batch_create: outlet neighbour outside the lattice");
+        return ERLBM_ERR_ARG;
+    }
+    b->general = (nf != nv) || inlet_x >= 0 || outlet_x >= 0;
+    b->stream = (hipStream_t)stream;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (b->n_cu < 1) b->n_cu = 1;
+    b->tab_n = std::max(nx, ny);
+    std::vector<double> tabs(2 * b->tab_n);
+    // sin / cos (2 pi x / L) of the Taylor-Green field (L = nx; the map is square)
+    erlbm_tg_map_tables(nx, tabs.data(), tabs.data() + b->tab_n);
+    hipError_t e = hipSuccess;
+    auto ok = [&](hipError_t r) {
+        if (r != hipSuccess && e == hipSuccess) e = r;
+    };
+    ok(hipMalloc(&b->vox_of_fluid, sizeof(int) * nf));
+    ok(hipMalloc(&b->fluid_of_vox, sizeof(int) * nv));
+    ok(hipMalloc(&b->nbr, sizeof(uint32_t) * nv));
+    ok(hipMalloc(&b->tabs, sizeof(double) * 2 * b->tab_n));
+    ok(hipMalloc(&b->counters, sizeof(unsigned int) * 16));
+    if (e == hipSuccess) {
+        ok(hipMemcpy(b->vox_of_fluid, vof.data(), sizeof(int) * nf, hipMemcpyHostToDevice));
+        ok(hipMemcpy(b->fluid_of_vox, fov.data(), sizeof(int) * nv, hipMemcpyHostToDevice));
+        ok(hipMemcpy(b->nbr, nbr.data(), sizeof(uint32_t) * nv, hipMemcpyHostToDevice));
+        ok(hipMemcpy(b->tabs, tabs.data(), sizeof(double) * 2 * b->tab_n, hipMemcpyHostToDevice));
+    }
+    if (e != hipSuccess) {
+        set_last_error("erlbm_batch_create", e);
+        erlbm_batch_destroy(b);
+        return ERLBM_ERR_NOMEM;
+    }
+    *out = b;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_batch_destroy(erlbm_batch *b)
+{
+    if (!b) return ERLBM_OK;
+    (void)hipStreamSynchronize(b->stream);
+    (void)hipFree(b->vox_of_fluid);
+    (void)hipFree(b->fluid_of_vox);
+    (void)hipFree(b->nbr);
+    (void)hipFree(b->tabs);
+    (void)hipFree(b->counters);
+    (void)hipFree(b->d_jobs);
+    (void)hipFree(b->d_results);
+    (void)hipFree(b->d_idx);
+    delete b;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_batch_set_launch(erlbm_batch *b, int wg_per_cu_f64, int wg_per_cu_f32)
+{
+    if (!b || wg_per_cu_f64 < 0 || wg_per_cu_f32 < 0) return ERLBM_ERR_ARG;
+    b->wg_per_cu[0] = wg_per_cu_f64;
+    b->wg_per_cu[1] = wg_per_cu_f32;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_batch_run(erlbm_batch *b, const erlbm_job *jobs, int n_jobs, erlbm_job_result *results)
+{
+    if (!b || (n_jobs > 0 && (!jobs || !results)) || n_jobs < 0) return ERLBM_ERR_ARG;
+    if (n_jobs == 0) return ERLBM_OK;
+    for (int k = 0; k < n_jobs; k++) {
+        const MapJob &j = jobs[k];
+        if (!(j.tau > 0) || !(j.tau_nh > 0) || j.n_steps < 0 || j.n_steps > 2000000000LL ||
+            (j.collision != ERLBM_COL_BGK && j.collision != ERLBM_COL_REG && j.collision != ERLBM_COL_EXT_REG) ||
+            (j.init != ERLBM_INIT_REST && j.init != ERLBM_INIT_TAYLOR_GREEN) ||
+            (j.precision != ERLBM_FP64 && j.precision != ERLBM_FP32_SHIFTED && j.precision != ERLBM_FP_AUTO)) {
+            set_last_error_msg("erlbm_batch_run: invalid job");
+            return ERLBM_ERR_ARG;
+        }
+        if (j.check_every > 0 && b->outlet_x >= 0) {
+            set_last_error_msg("erlbm_batch_run: the energy test is not defined on a lattice with an outlet column");
+            return ERLBM_ERR_UNSUPPORTED;
+        }
+        if (j.init == ERLBM_INIT_TAYLOR_GREEN && b->nx != b->ny) {
+            set_last_error_msg("erlbm_batch_run: the Taylor-Green field needs a square lattice");
+            return ERLBM_ERR_ARG;
+        }
+    }
+    if (n_jobs > b->cap) {
+        (void)hipFree(b->d_jobs);
+        (void)hipFree(b->d_results);
+        (void)hipFree(b->d_idx);
+        b->d_jobs = nullptr;
+        b->d_results = nullptr;
+        b->d_idx = nullptr;
+        b->cap = 0;
+        ERLBM_HIP_TRY(hipMalloc(&b->d_jobs, sizeof(MapJob) * n_jobs));
+        ERLBM_HIP_TRY(hipMalloc(&b->d_results, sizeof(MapResult) * n_jobs));
+        ERLBM_HIP_TRY(hipMalloc(&b->d_idx, sizeof(int) * n_jobs));
+        b->cap = n_jobs;
+    }
+    // group jobs by (precision, collision); longest budgets first within a
+    // group so the queue tail is made of short runs
+    std::vector<int> idx(n_jobs);
+    std::vector<int> prec(n_jobs);
+    for (int k = 0; k < n_jobs; k++) {
+        idx[k] = k;
+        prec[k] = resolve_precision(jobs[k]);
+    }
+    auto key = [&](int k) { return prec[k] * 8 + jobs[k].collision; };
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+        if (key(x) != key(y)) return key(x) < key(y);
+        return jobs[x].n_steps > jobs[y].n_steps;
+    });
+    ERLBM_HIP_TRY(hipMemcpyAsync(b->d_jobs, jobs, sizeof(MapJob) * n_jobs, hipMemcpyHostToDevice, b->stream));
+    ERLBM_HIP_TRY(hipMemcpyAsync(b->d_idx, idx.data(), sizeof(int) * n_jobs, hipMemcpyHostToDevice, b->stream));
+    ERLBM_HIP_TRY(hipMemsetAsync(b->counters, 0, sizeof(unsigned int) * 16, b->stream));
+    RunArgs a;
+    a.g.nx = b->nx;
+    a.g.ny = b->ny;
+    a.g.n_fluid = b->n_fluid;
+    a.g.nvp = (b->nx * b->ny + 7) / 8 * 8;
+    a.g.general = b->general ? 1 : 0;
+    a.g.inlet_x = b->inlet_x;
+    a.g.outlet_x = b->outlet_x;
+    a.g.outlet_dx = b->outlet_dx;
+    a.g.vox_of_fluid = b->vox_of_fluid;
+    a.g.fluid_of_vox = b->fluid_of_vox;
+    a.g.nbr = b->nbr;
+    a.g.sin_tab = b->tabs;
+    a.g.cos_tab = b->tabs + b->tab_n;
+    a.jobs = b->d_jobs;
+    a.results = b->d_results;
+    int start = 0, group = 0;
+    while (start < n_jobs) {
+        int end = start;
+        while (end < n_jobs && key(idx[end]) == key(idx[start])) end++;
+        if (group >= 16) return ERLBM_ERR_UNSUPPORTED;
+        a.job_idx = b->d_idx + start;
+        a.n_jobs = end - start;
+        const int p = prec[idx[start]];
+        const int col = jobs[idx[start]].collision;
+        int rc = (p == ERLBM_FP64) ? launch_col<F64>(b, col, a, b->counters + group)
+                                   : launch_col<F32>(b, col, a, b->counters + group);
+        if (rc) return rc;
+        start = end;
+        group++;
+    }
+    ERLBM_HIP_TRY(hipMemcpyAsync(results, b->d_results, sizeof(MapResult) * n_jobs, hipMemcpyDeviceToHost,
+                                 b->stream));
+    ERLBM_HIP_TRY(hipStreamSynchronize(b->stream));
+    return ERLBM_OK;
+}
+
+// ---- job lists of the two published maps ----------------------------------------
+extern "C" int erlbm_tg_map_jobs(int n_ma, int n_nu, int collision, double tau_nh, int precision, int i0, int i1,
+                                 int j0, int j1, erlbm_job *jobs)
+{
+    if (!jobs || i0 < 0 || j0 < 0 || i1 > n_ma || j1 > n_nu || i0 > i1 || j0 > j1) return ERLBM_ERR_ARG;
+    int k = 0;
+    for (int j = j0; j < j1; j++)
+        for (int i = i0; i < i1; i++) {
+            erlbm_job J;
+            int n = 0;
+            double u0, tau, tnh;
+            int rc = erlbm_tg_map_point_params(i, j, n_ma, n_nu, 16, collision, tau_nh, &u0, &tau, &tnh, &n);
+            if (rc) return rc;
+            J.tau = tau;
+            J.tau_nh = tnh;
+            J.u0 = u0;
+            J.rho0 = 1.0;
+            J.energy_factor = 2.0;
+            J.dens_limit = 0.0;
+            J.n_steps = n;
+            J.check_every = 100;
+            J.init = ERLBM_INIT_TAYLOR_GREEN;
+            J.collision = collision;
+            J.precision = precision;
+            jobs[k++] = J;
+        }
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_obstacle_map_jobs(int n_tau, int n_nu, double Ma, int precision, int i0, int i1, int j0,
+                                       int j1, erlbm_job *jobs)
+{
+    if (!jobs || i0 < 0 || j0 < 0 || i1 > n_tau || j1 > n_nu || i0 > i1 || j0 > j1) return ERLBM_ERR_ARG;
+    int k = 0;
+    for (int j = j0; j < j1; j++)
+        for (int i = i0; i < i1; i++) {
+            erlbm_job J;
+            int n = 0;
+            double u, tau, tnh;
+            int rc = erlbm_obstacle_map_point_params(i, j, n_tau, n_nu, Ma, &u, &tau, &tnh, &n);
+            if (rc) return rc;
+            J.tau = tau;
+            J.tau_nh = tnh;
+            J.u0 = u;
+            J.rho0 = 1.0;
+            J.energy_factor = 0.0;
+            J.dens_limit = 10. * 1.0;
+            J.n_steps = n;
+            J.check_every = 0;
+            J.init = ERLBM_INIT_REST;
+            J.collision = ERLBM_COL_EXT_REG;
+            J.precision = precision;
+            jobs[k++] = J;
+        }
+    return ERLBM_OK;
+}

@@ -1,0 +1,116 @@
+"""Batched tiny-lattice runs: the stability-map drivers on the GPU.
+
+The reference's map drivers (src/main_stability-map_taylor-green.cpp,
+src/main_stability-map_obstacle.cpp) loop over map points with
+``omp parallel for`` and run ``roda_ponto`` on a private lattice per point.
+Here a ``Batch`` holds the geometry once and ``run`` executes any number of
+jobs (one per map point) in the persistent LDS-resident kernel of
+``liberlbm.so`` (erlbm_batch_run in include/erlbm.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import JOB_DTYPE, RESULT_DTYPE, check, lib
+
+
+class Batch:
+    """A shared D3Q19 tiny geometry (nz = 1, <= 256 fluid sites)."""
+
+    def __init__(self, nx: int, ny: int, mask: Optional[np.ndarray] = None, inlet_x: int = -1,
+                 outlet_x: int = -1, stream: int = 0):
+        h = C.c_void_p()
+        m = None if mask is None else np.ascontiguousarray(mask, np.int32).reshape(-1)
+        mp = None if m is None else m.ctypes.data_as(C.c_void_p)
+        check(lib().erlbm_batch_create(nx, ny, mp, inlet_x, outlet_x, C.c_void_p(stream or None), C.byref(h)),
+              "erlbm_batch_create")
+        self._h = h
+        self.nx, self.ny = nx, ny
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().erlbm_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_launch(self, wg_per_cu_f64: int = 0, wg_per_cu_f32: int = 0) -> None:
+        check(lib().erlbm_batch_set_launch(self._h, wg_per_cu_f64, wg_per_cu_f32), "erlbm_batch_set_launch")
+
+    def run(self, jobs: np.ndarray) -> np.ndarray:
+        """jobs: structured array of JOB_DTYPE; returns RESULT_DTYPE results."""
+        jobs = np.ascontiguousarray(jobs, dtype=JOB_DTYPE)
+        res = np.zeros(jobs.size, dtype=RESULT_DTYPE)
+        check(lib().erlbm_batch_run(self._h, jobs.ctypes.data_as(C.c_void_p), jobs.size,
+                                    res.ctypes.data_as(C.c_void_p)), "erlbm_batch_run")
+        return res
+
+
+def tg_map_jobs(n_ma=400, n_nu=400, collision=L.COL_EXT_REG, tau_nh=0.65, precision=L.FP64, i0=0, i1=None, j0=0,
+                j1=None) -> np.ndarray:
+    i1 = n_ma if i1 is None else i1
+    j1 = n_nu if j1 is None else j1
+    jobs = np.zeros((j1 - j0) * (i1 - i0), dtype=JOB_DTYPE)
+    check(lib().erlbm_tg_map_jobs(n_ma, n_nu, collision, tau_nh, precision, i0, i1, j0, j1,
+                                  jobs.ctypes.data_as(C.c_void_p)), "erlbm_tg_map_jobs")
+    return jobs
+
+
+def obstacle_map_jobs(n_tau=400, n_nu=400, ma=0.1, precision=L.FP64, i0=0, i1=None, j0=0, j1=None) -> np.ndarray:
+    i1 = n_tau if i1 is None else i1
+    j1 = n_nu if j1 is None else j1
+    jobs = np.zeros((j1 - j0) * (i1 - i0), dtype=JOB_DTYPE)
+    check(lib().erlbm_obstacle_map_jobs(n_tau, n_nu, ma, precision, i0, i1, j0, j1,
+                                        jobs.ctypes.data_as(C.c_void_p)), "erlbm_obstacle_map_jobs")
+    return jobs
+
+
+def tg_batch() -> Batch:
+    """The Taylor-Green map lattice: 16 x 16 x 1, periodic."""
+    return Batch(16, 16)
+
+
+def obstacle_batch() -> Batch:
+    """The obstacle map channel: 22 x 12 x 1, Zou-He inlet at x = 1, outlet at x = 20."""
+    from .lattice import obstacle_mask
+    return Batch(22, 12, obstacle_mask(), inlet_x=1, outlet_x=20)
+
+
+def run_tg_map(collision=L.COL_EXT_REG, tau_nh=0.65, precision=L.FP64, n=400, batch: Optional[Batch] = None,
+               **sub) -> np.ndarray:
+    """Stability map (1 = unstable) as an (n_nu, n_ma) array, i fastest."""
+    jobs = tg_map_jobs(n, n, collision, tau_nh, precision, **sub)
+    own = batch is None
+    b = tg_batch() if own else batch
+    try:
+        res = b.run(jobs)
+    finally:
+        if own:
+            b.close()
+    return res
+
+
+def run_obstacle_map(precision=L.FP64, n=400, ma=0.1, batch: Optional[Batch] = None, **sub) -> np.ndarray:
+    jobs = obstacle_map_jobs(n, n, ma, precision, **sub)
+    own = batch is None
+    b = obstacle_batch() if own else batch
+    try:
+        res = b.run(jobs)
+    finally:
+        if own:
+            b.close()
+    return res

@@ -149,6 +149,74 @@ int erlbm_tg_map_point_params(int i, int j, int n_ma, int n_nu, int L, int colli
 int erlbm_obstacle_map_point_params(int i, int j, int n_tau, int n_nu, double Ma, double *u_imp, double *tau,
                                     double *tau_nh, int *n_steps);
 
+/* ---- batched tiny lattices (stability maps, small requests) ------------------
+ * Replaces the map drivers' `omp parallel for` over map points, each calling
+ * roda_ponto (src/main_stability-map_taylor-green.cpp:41-104, 150-177;
+ * src/main_stability-map_obstacle.cpp:75-123, 161-186): thousands of
+ * independent D3Q19 lattices of at most 256 fluid sites (nz = 1) sharing one
+ * geometry run in one persistent kernel, one lattice per workgroup resident
+ * in LDS, jobs taken from a device-side queue.  Per job: initial state,
+ * tau / tau_nh, collision, step budget and the map driver's stop test.    */
+enum { ERLBM_INIT_REST = 0,          /* dist_eq(u = 0, rho0) everywhere (obstacle map :81-82) */
+       ERLBM_INIT_TAYLOR_GREEN = 1   /* equilibrium Taylor-Green field of amplitude u0 (:49-61) */ };
+/* precision codes for jobs: ERLBM_FP64, ERLBM_FP32_SHIFTED, or ERLBM_FP_AUTO =
+ * FP32-shifted unless tau - 1/2 < 1e-3 (near-singular viscosity: FP64).   */
+enum { ERLBM_FP_AUTO = 3 };
+
+typedef struct {
+    double tau, tau_nh;
+    double u0;             /* Taylor-Green amplitude, and the Zou-He inlet velocity */
+    double rho0;           /* initial / outlet density */
+    double energy_factor;  /* energy test: stop if E > energy_factor * E0 or E non-finite */
+    double dens_limit;     /* > 0: stop after a step with |rho| > dens_limit or non-finite rho */
+    long long n_steps;     /* step budget */
+    int check_every;       /* energy test before steps k = check_every, 2 check_every, ... (0: none) */
+    int init;              /* ERLBM_INIT_* */
+    int collision;         /* ERLBM_COL_BGK / REG / EXT_REG */
+    int precision;         /* ERLBM_FP64 / ERLBM_FP32_SHIFTED / ERLBM_FP_AUTO */
+} erlbm_job;
+
+typedef struct {
+    int unstable;          /* 1 if a stop test fired (the map's 1 = instavel) */
+    int steps;             /* steps run (the step index the run stopped at) */
+    int done;              /* 1 when retired (stopped or budget used) */
+    int precision;         /* precision the job ran in */
+    double e0, energy;     /* initial and last measured kinetic energy (energy test only) */
+} erlbm_job_result;
+
+typedef struct erlbm_batch erlbm_batch;
+
+/* Geometry shared by the batch: nx * ny voxels (nz = 1), mask as in
+ * erlbm_lattice_create (NULL = all fluid, periodic), at most 256 fluid
+ * sites.  inlet_x >= 0: ZouHeD3Q19_vx_x at that column with vx = job.u0;
+ * outlet_x >= 0: devnull_rho_x at that column with rho = job.rho0.       */
+int erlbm_batch_create(int nx, int ny, const int *mask, int inlet_x, int outlet_x, void *stream,
+                       erlbm_batch **out);
+int erlbm_batch_destroy(erlbm_batch *b);
+/* Runs jobs[0..n_jobs) to retirement; results[k] for jobs[k].  Synchronous. */
+int erlbm_batch_run(erlbm_batch *b, const erlbm_job *jobs, int n_jobs, erlbm_job_result *results);
+/* Launch shape of the persistent kernels: workgroups per CU for the FP64 and
+ * FP32 kernels (0 = occupancy maximum).  Chosen by the autotuner.       */
+int erlbm_batch_set_launch(erlbm_batch *b, int wg_per_cu_f64, int wg_per_cu_f32);
+
+/* Job lists of the two published maps, with the drivers' axis expressions:
+ * Taylor-Green map (src/main_stability-map_taylor-green.cpp:134-164; L = 16,
+ * 100 T, energy test every 100 steps) and obstacle map
+ * (src/main_stability-map_obstacle.cpp:147-172; 200 ny / U0 steps, density
+ * test |rho| > 10 rho0).  Points (i, j) for i in [i0, i1), j in [j0, j1),
+ * i fastest; jobs must hold (i1-i0)*(j1-j0) entries.                      */
+int erlbm_tg_map_jobs(int n_ma, int n_nu, int collision, double tau_nh, int precision, int i0, int i1, int j0,
+                      int j1, erlbm_job *jobs);
+int erlbm_obstacle_map_jobs(int n_tau, int n_nu, double Ma, int precision, int i0, int i1, int j0, int j1,
+                            erlbm_job *jobs);
+
+/* ---- measurement -------------------------------------------------------------
+ * No counterpart in the reference: a plain device-to-device copy of `bytes`
+ * (nontemporal double2 loads/stores), timed with HIP events over `iters`
+ * launches.  Calibrates the HBM traffic counters and gives the achievable
+ * bandwidth beside the collide-stream kernel.                              */
+int erlbm_hbm_copy_bench(long long bytes, int iters, float *ms_per_iter);
+
 #ifdef __cplusplus
 }
 #endif

@@ -146,3 +146,38 @@ def test_fma_reference_tolerance_1000_steps(gpu, golden):
         lat.step(1000)
         got = lat.download()
     assert _rel(got, want) <= TOL_FMA
+
+
+@pytest.mark.parametrize("npt,ntl,pad,remap,pair,layout", [(1, 0, 320, 1, 0, 0), (2, 0, 0, 1, 0, 1), (2, 1, 64, 0, 0, 0),
+                                                          (4, 1, 4160, 1, 0, 1), (1, 1, 0, 0, 1, 0), (1, 0, 64, 1, 1, 1)])
+def test_periodic_launch_variants_bitwise(gpu, orc, monkeypatch, npt, ntl, pad, remap, pair, layout):
+    """Every launch shape and layout the tuner may pick gives the same populations."""
+    for k, v in (("ERLBM_NPT", npt), ("ERLBM_NT_LOADS", ntl), ("ERLBM_PLANE_PAD", pad), ("ERLBM_XCD_REMAP", remap),
+                 ("ERLBM_PAIR", pair), ("ERLBM_LAYOUT", layout)):
+        monkeypatch.setenv(k, str(v))
+    L, n = 96, 30
+    tau = 0.5 + 3. * (1. / 32.) * L / 80000.
+    f0 = gpu.init_double_shear(L)
+    ini, st, nf = orc.stream_map(9, L, L, 1)
+    want = orc.run_steps(9, f0, st, n, O.COL_EXT_REG, tau, 0.65)
+    with _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0) as lat:
+        lat.step(n)
+        assert np.array_equal(lat.download(), want)
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_layouts_obstacle_and_monitors(gpu, orc, golden, monkeypatch, layout):
+    """Both population layouts through the general kernel, transfers and monitors."""
+    monkeypatch.setenv("ERLBM_LAYOUT", str(layout))
+    g = golden("ref_trajectories.npz")
+    tau, tnh, nst = g["obst_stable_params"]
+    with gpu.Lattice(19, 22, 12, 1, mask=gpu.obstacle_mask()) as lat:
+        lat.set_params(tau, tnh, 1.0, O.COL_EXT_REG)
+        lat.set_zouhe_inlet(1, 0.1 / np.sqrt(3.0))
+        lat.set_outlet_rescale(20, 1.0)
+        lat.set_density_check(10.0)
+        lat.upload(g["obst_in"])
+        lat.step(int(nst))
+        got = lat.download()
+        assert np.array_equal(got, g["obst_stable_out_strict"])
+        assert lat.kinetic_energy(gpu.STATE, ordered=True) == orc.kinetic_energy(19, got)

@@ -1,0 +1,125 @@
+"""Batched stability-map path against the reference's own map runs.
+
+FP64 jobs are bitwise faithful, so their classification must equal the
+strict reference build point for point (tests/golden/ref_map_subsample.npz:
+roda_ponto of both map drivers on the README's 20 x 20 subsample positions).
+FP32-shifted jobs are an approximation: their agreement is measured and
+bounded below.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _subsample_jobs(gpu, kind, g, precision, col=O.COL_EXT_REG, tau_nh=0.65):
+    rows = []
+    for i, j in zip(g["i"], g["j"]):
+        if kind == "tg":
+            rows.append(gpu.tg_map_jobs(400, 400, col, tau_nh, precision, int(i), int(i) + 1, int(j), int(j) + 1))
+        else:
+            rows.append(gpu.obstacle_map_jobs(400, 400, 0.1, precision, int(i), int(i) + 1, int(j), int(j) + 1))
+    return np.concatenate(rows)
+
+
+@pytest.mark.parametrize("key,col,tnh", [("tg_erlbm065", O.COL_EXT_REG, 0.65), ("tg_reg", O.COL_REG, 1.0)])
+def test_tg_map_subsample_fp64_equals_reference(gpu, golden, key, col, tnh):
+    g = golden("ref_map_subsample.npz")
+    jobs = _subsample_jobs(gpu, "tg", g, gpu.FP64, col, tnh)
+    print(f"tg map subsample {key}: {jobs.size} points", flush=True)
+    with gpu.tg_batch() as b:
+        res = b.run(jobs)
+    assert np.all(res["done"] == 1)
+    agree = int(np.sum(res["unstable"] == g[f"{key}_strict"]))
+    assert agree == jobs.size, f"{agree}/{jobs.size}"
+
+
+def test_obstacle_map_subsample_fp64_equals_reference(gpu, golden):
+    g = golden("ref_map_subsample.npz")
+    jobs = _subsample_jobs(gpu, "obstacle", g, gpu.FP64)
+    print(f"obstacle map subsample: {jobs.size} points", flush=True)
+    with gpu.obstacle_batch() as b:
+        res = b.run(jobs)
+    agree = int(np.sum(res["unstable"] == g["obstacle_strict"]))
+    assert agree == jobs.size, f"{agree}/{jobs.size}"
+
+
+def test_stop_steps_and_energies_match_oracle(gpu, orc):
+    """Stop step and energies of single points, bitwise, against the oracle's
+    restatement of roda_ponto."""
+    pts = [(380, 5), (150, 30), (20, 390), (399, 0)]
+    jobs = np.concatenate([gpu.tg_map_jobs(400, 400, O.COL_EXT_REG, 0.65, gpu.FP64, i, i + 1, j, j + 1)
+                           for i, j in pts])
+    # cap the long stable runs to keep the CPU side short
+    jobs["n_steps"] = np.minimum(jobs["n_steps"], 3000)
+    with gpu.tg_batch() as b:
+        res = b.run(jobs)
+    for k, (i, j) in enumerate(pts):
+        u0, tau, tnh = orc.tg_map_params(i, j)
+        f0 = orc.init_tg_map_d3q19(16, u0)
+        ini, st, nf = orc.stream_map(19, 16, 16, 1)
+        e0 = orc.kinetic_energy(19, f0)
+        assert res["e0"][k] == e0
+        # replay the driver loop on the oracle up to the GPU's stop step
+        a = f0
+        stop = int(jobs["n_steps"][k])
+        unstable = 0
+        e_last = e0
+        for s in range(int(jobs["n_steps"][k])):
+            if s % 100 == 0 and s > 0:
+                e_last = orc.kinetic_energy(19, a)
+                if e_last > 2 * e0 or not np.isfinite(e_last):
+                    stop, unstable = s, 1
+                    break
+            a = orc.run_steps(19, a, st, 1, O.COL_EXT_REG, tau, tnh)
+        assert (res["unstable"][k], res["steps"][k]) == (unstable, stop), (i, j)
+        if unstable:
+            assert res["energy"][k] == e_last or (np.isnan(res["energy"][k]) and np.isnan(e_last))
+
+
+def test_obstacle_points_match_oracle_steps(gpu, orc):
+    pts = [(0, 0), (10, 20), (200, 3), (5, 399)]
+    jobs = np.concatenate([gpu.obstacle_map_jobs(400, 400, 0.1, gpu.FP64, i, i + 1, j, j + 1) for i, j in pts])
+    jobs["n_steps"] = np.minimum(jobs["n_steps"], 4000)
+    with gpu.obstacle_batch() as b:
+        res = b.run(jobs)
+    m = orc.obstacle_mask()
+    ini, st, nf = orc.stream_map(19, 22, 12, 1, m)
+    for k, (i, j) in enumerate(pts):
+        u, tau, tnh = orc.obstacle_map_params(i, j)
+        out, n, v = orc.run_bc_steps(19, orc.init_rest_d3q19(nf), 22, 12, 1, ini, st, int(jobs["n_steps"][k]),
+                                     O.COL_EXT_REG, tau, tnh, inlet_x=1, u_in=u, outlet_x=20, rho_out=1.0,
+                                     dens_limit=10.0)
+        assert (res["unstable"][k], res["steps"][k]) == (v, n), (i, j)
+
+
+def test_fp32_shifted_agreement(gpu, golden):
+    """FP32-shifted storage (with the FP64 fallback below tau - 1/2 < 1e-3)
+    against the strict reference classification on the subsample."""
+    g = golden("ref_map_subsample.npz")
+    for key, kind, col, tnh in (("tg_erlbm065", "tg", O.COL_EXT_REG, 0.65), ("tg_reg", "tg", O.COL_REG, 1.0),
+                                ("obstacle", "obstacle", O.COL_EXT_REG, 0.65)):
+        jobs = _subsample_jobs(gpu, kind, g, gpu.FP_AUTO, col, tnh)
+        with (gpu.tg_batch() if kind == "tg" else gpu.obstacle_batch()) as b:
+            res = b.run(jobs)
+        agree = float(np.mean(res["unstable"] == g[f"{key}_strict"]))
+        n32 = int(np.sum(res["precision"] == gpu.FP32_SHIFTED))
+        print(f"{key}: FP_AUTO agreement {agree:.4f} ({n32} points in FP32)", flush=True)
+        assert n32 > 0
+        assert agree >= 0.98, (key, agree)
+
+
+def test_result_independent_of_batch_mix(gpu):
+    """A job's result is the same alone and inside any batch (per-lattice
+    arithmetic and reduction order do not depend on the batch)."""
+    mix = gpu.tg_map_jobs(400, 400, O.COL_EXT_REG, 0.65, gpu.FP_AUTO, 100, 140, 150, 160)
+    mix["n_steps"] = np.minimum(mix["n_steps"], 5000)
+    with gpu.tg_batch() as b:
+        together = b.run(mix)
+        for k in (0, 7, 123, 399):
+            alone = b.run(mix[k:k + 1])
+            assert alone.tobytes() == together[k:k + 1].tobytes()
+        rev = b.run(mix[::-1].copy())[::-1]
+        assert rev.tobytes() == together.tobytes()
