@@ -75,6 +75,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "fp16"],
+                    help="population storage of the timed run (fp32/fp16: shifted deviations)")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the calibration copy kernel (302 MB read + 302 MB written per launch)")
     a = ap.parse_args()
@@ -95,8 +97,10 @@ def main() -> int:
     if a.copy_only:
         print(json.dumps({"copy_bytes_each_way": copy_bytes, "ms": copy_ms, "GBs": copy_gbs}))
         return 0
+    prec = {"fp64": E.FP64, "fp32": E.FP32_SHIFTED, "fp16": E.FP16_SHIFTED}[a.precision]
+    bytes_per_node = {"fp64": 144, "fp32": 72, "fp16": 36}[a.precision]
     f0 = E.init_double_shear(L)
-    lat = E.Lattice(9, L, L, 1)
+    lat = E.Lattice(9, L, L, 1, precision=prec)
     lat.set_params(TAU, TAU_NH, 1.0, E.COL_EXT_REG)
     lat.upload(f0)
     mass0 = float(f0.sum())
@@ -107,19 +111,19 @@ def main() -> int:
     steps_done, reason = lat.status()
     ms_per_step = ms / a.steps
     mlups = L * L / (ms_per_step * 1e-3) / 1e6
-    achieved = BYTES_PER_NODE * L * L / (ms_per_step * 1e-3) / 1e9
+    achieved = bytes_per_node * L * L / (ms_per_step * 1e-3) / 1e9
     post = lat.download(E.POST_COLLISION)
     mass_err = abs(float(post.sum()) - mass0) / mass0
     finite = bool(np.isfinite(post).all())
     del post
     lat.close()
-    if steps_done != a.warmup + a.steps or reason != 0 or not finite or mass_err > 1e-9:
+    if steps_done != a.warmup + a.steps or reason != 0 or not finite or mass_err > (1e-9 if prec == E.FP64 else 1e-3):
         print(f"bench.py: run check failed (steps {steps_done}, stop {reason}, finite {finite}, "
               f"mass error {mass_err:.2e})", file=sys.stderr)
         return 1
 
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
+    if a.precision == "fp64" and os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
@@ -134,11 +138,12 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"fp64": "f64", "fp32": "f32 (shifted storage, f32 arithmetic)",
+                  "fp16": "f16 storage / f32 arithmetic (shifted)"}[a.precision],
         "data": "synthetic: double-shear-layer initial field (deterministic, no dataset)",
         "config": {"workload": "D2Q9 E-RLBM double shear layer, periodic L=2048 (4,194,304 nodes), "
-                               "tau=0.5024, tau_nh=0.65, FP64 SoA pull-stream+collide",
-                   "L": L, "tau": TAU, "tau_nh": TAU_NH, "collision": "E-RLBM", "precision": "fp64"},
+                               f"tau=0.5024, tau_nh=0.65, {a.precision} row-blocked pull-stream+collide",
+                   "L": L, "tau": TAU, "tau_nh": TAU_NH, "collision": "E-RLBM", "precision": a.precision},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "liberlbm.so")
 
 ERLBM_OK, ERLBM_ERR_ARG, ERLBM_ERR_HIP, ERLBM_ERR_STATE, ERLBM_ERR_NOMEM, ERLBM_ERR_UNSUPPORTED = range(6)
 COL_BGK, COL_MRT, COL_REG, COL_EXT_REG = 1, 2, 3, 4
-FP64, FP32_SHIFTED, FP16_SHIFTED = 0, 1, 2
+FP64, FP32_SHIFTED, FP16_SHIFTED, FP32_SHIFTED_F32 = 0, 1, 2, 4
 STATE, POST_COLLISION = 0, 1
 STOP_DENSITY, STOP_ENERGY = 1, 2
 
@@ -40,14 +40,14 @@ class Job(C.Structure):
 class JobResult(C.Structure):
     """erlbm_job_result (include/erlbm.h)."""
     _fields_ = [("unstable", C.c_int), ("steps", C.c_int), ("done", C.c_int), ("precision", C.c_int),
-                ("e0", C.c_double), ("energy", C.c_double)]
+                ("e0", C.c_double), ("energy", C.c_double), ("margin", C.c_double)]
 
 
 JOB_DTYPE = np.dtype([("tau", "f8"), ("tau_nh", "f8"), ("u0", "f8"), ("rho0", "f8"), ("energy_factor", "f8"),
                       ("dens_limit", "f8"), ("n_steps", "i8"), ("check_every", "i4"), ("init", "i4"),
                       ("collision", "i4"), ("precision", "i4")])
 RESULT_DTYPE = np.dtype([("unstable", "i4"), ("steps", "i4"), ("done", "i4"), ("precision", "i4"), ("e0", "f8"),
-                         ("energy", "f8")])
+                         ("energy", "f8"), ("margin", "f8")])
 assert JOB_DTYPE.itemsize == C.sizeof(Job) and RESULT_DTYPE.itemsize == C.sizeof(JobResult)
 
 

@@ -12,6 +12,7 @@
 // The moment sums use Neumaier compensation; FMA is used explicitly.
 #pragma once
 
+#include "collide.hpp"
 #include "vset.hpp"
 
 namespace erlbm {

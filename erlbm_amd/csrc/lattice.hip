@@ -28,8 +28,10 @@
 
 #include "collide.hpp"
 #include "common.hpp"
+#include "lattice_impl.hpp"
 
 using namespace erlbm;
+using namespace erlbm::lat;
 
 namespace erlbm {
 
@@ -55,107 +57,6 @@ extern "C" int erlbm_device_count(int *count)
 }
 
 namespace {
-
-constexpr int BLOCK = 256;
-constexpr uint32_t FLUID_BIT = 1u << 31;
-
-struct Geometry {
-    int nx, ny, nz;
-    long long si;   // population stride
-    long long sr;   // row stride
-};
-
-__host__ __device__ __forceinline__ long long at(const Geometry &g, int i, long long row, int x)
-{
-    return i * g.si + row * g.sr + x;
-}
-
-struct BCArgs {
-    int inlet_x;    // < 0: none
-    int inlet_west; // pos_x < nx/2 branch of ZouHeD3Q19_vx_x
-    double u_in;
-    int outlet_x;   // < 0: none
-    int outlet_dx;  // +1 west half, -1 east half (devnull_rho_x :90-92)
-    double rho_out;
-};
-
-struct StepArgs {
-    const double *__restrict__ src;
-    double *__restrict__ dst;
-    Geometry g;
-    const uint32_t *__restrict__ nbr;  // general kernel only
-    CollideConsts k;
-    BCArgs bc;
-    double dens_limit;                 // <= 0: no test
-    unsigned long long *status;        // written (atomics) by this launch
-    const unsigned long long *__restrict__ stop_ro;  // the same word, read as of the launch
-    unsigned long long step;
-};
-
-__device__ __forceinline__ bool stopped(const unsigned long long *status, unsigned long long step)
-{
-    const unsigned long long s =
-        __hip_atomic_load(&status[ST_STOP_STEP], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return step >= s;
-}
-
-__device__ __forceinline__ void flag_density(unsigned long long *status, unsigned long long step, bool bad)
-{
-    // one atomic per wavefront
-    if (__any(bad)) {
-        if ((threadIdx.x & 63) == 0) {
-            atomicMin(&status[ST_STOP_STEP], step + 1);
-            atomicOr(&status[ST_REASON], (unsigned long long)ERLBM_STOP_DENSITY);
-        }
-    }
-}
-
-__device__ __forceinline__ int wrap(int a, int n)
-{
-    return a < 0 ? a + n : (a >= n ? a - n : a);
-}
-
-// ZouHeD3Q19_vx_x (Boundary_conditions.cpp:350-442) on one site, in registers.
-__device__ __forceinline__ void zouhe_d3q19(double (&f)[19], double vx, bool west)
-{
-    if (west) {
-        double sum_f_xminus = f[1] + f[7] + f[9] + f[11] + f[13];
-        double sum_f_xzero = f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18];
-        double rho = (2.0 * sum_f_xminus + sum_f_xzero) / (1.0 - vx);
-        double vy = 0.0, vz = 0.0;
-        f[1] = f[1] + rho * vx / 3.0;
-        f[7] = f[7] + rho * vx / 6.0 + 0.5 * rho * vy - 0.5 * (f[18] + f[3] + f[16] - f[15] - f[4] - f[17]);
-        f[9] = f[9] + rho * vx / 6.0 - 0.5 * rho * vy + 0.5 * (f[18] + f[3] + f[16] - f[15] - f[4] - f[17]);
-        f[11] = f[11] + rho * vx / 6.0 + 0.5 * rho * vz - 0.5 * (f[17] + f[5] + f[16] - f[15] - f[6] - f[18]);
-        f[13] = f[13] + rho * vx / 6.0 - 0.5 * rho * vz + 0.5 * (f[17] + f[5] + f[16] - f[15] - f[6] - f[18]);
-    } else {
-        double sum_f_xplus = f[2] + f[8] + f[10] + f[12] + f[14];
-        double sum_f_xzero = f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18];
-        double rho = (2.0 * sum_f_xplus + sum_f_xzero) / (1.0 - vx);
-        double vy = 0.0, vz = 0.0;
-        f[2] = f[2] - rho * vx / 3.0;
-        f[10] = f[10] - rho * vx / 6.0 + 0.5 * rho * vy - 0.5 * (f[18] + f[3] + f[16] - f[15] - f[4] - f[17]);
-        f[8] = f[8] - rho * vx / 6.0 - 0.5 * rho * vy + 0.5 * (f[18] + f[3] + f[16] - f[15] - f[4] - f[17]);
-        f[14] = f[14] - rho * vx / 6.0 + 0.5 * rho * vz - 0.5 * (f[17] + f[5] + f[16] - f[15] - f[6] - f[18]);
-        f[12] = f[12] - rho * vx / 6.0 - 0.5 * rho * vz + 0.5 * (f[17] + f[5] + f[16] - f[15] - f[6] - f[18]);
-    }
-}
-
-// ZouHeD2Q9_vx_x (Boundary_conditions.cpp:710-735) on one site.
-__device__ __forceinline__ void zouhe_d2q9(double (&f)[9], double vx)
-{
-    double rho = (f[0] + f[3] + f[4] + 2 * (f[2] + f[8] + f[6])) / (1. - vx);
-    f[1] = (rho / 9.0) * (1.0 + 3.0 * vx + 3.0 * vx * vx);
-    f[5] = (rho / 36.0) * (1.0 + 3.0 * vx + 3.0 * vx * vx);
-    f[7] = (rho / 36.0) * (1.0 + 3.0 * vx + 3.0 * vx * vx);
-}
-
-template <int Q>
-__device__ __forceinline__ void zouhe(double (&f)[Q], double vx, bool west)
-{
-    if constexpr (Q == 19) zouhe_d3q19(f, vx, west);
-    else zouhe_d2q9(f, vx);
-}
 
 // Gather the pre-collision populations of voxel (x,y,z): identity (first step
 // after an upload) or pull-stream with periodic wrap and bounce-back.
@@ -484,54 +385,8 @@ __global__ void k_reset_status(unsigned long long *status)
 
 }  // namespace
 
-// ---------------------------------------------------------------------------
-struct erlbm_lattice {
-    int q = 0, nx = 0, ny = 0, nz = 0;
-    long long nvox = 0, si = 0, sr = 0, extent = 0;
-    int layout = 1;
-    int n_fluid = 0;
-    bool all_fluid = true;
-    int precision = ERLBM_FP64;
-    hipStream_t stream = nullptr;
-    int device = 0;
-
-    double *buf[2] = {nullptr, nullptr};
-    int base = 0;             // buffer holding the uploaded state
-    bool uploaded = false;
-    long long launched = 0;   // steps enqueued since upload
-
-    uint32_t *nbr = nullptr;          // per voxel (always allocated)
-    int *vox_of_fluid = nullptr;      // null when all fluid
-    double *ke_node = nullptr, *partial_ke = nullptr, *partial_rho = nullptr;
-    double *scalars = nullptr;        // [0] E, [1] max|rho|, [2] E0
-    int *nonfinite = nullptr;
-    unsigned long long *status = nullptr;
-
-    double tau = 1.0, tau_nh = 1.0, rho0 = 1.0;
-    int col = ERLBM_COL_EXT_REG;
-    int inlet_x = -1;
-    double u_in = 0.0;
-    int outlet_x = -1;
-    double rho_out = 1.0;
-    double dens_limit = 0.0;
-    bool e0_valid = false;
-
-    // launch tuning (environment overrides read at creation):
-    //   ERLBM_PLANE_PAD  doubles added to each population plane's stride
-    //   ERLBM_NPT        nodes per thread of the periodic D2Q9 kernel (1, 2, 4)
-    //   ERLBM_NT_LOADS   1: nontemporal loads in that kernel
-    //   ERLBM_XCD_REMAP  1: renumber blocks so each XCD walks a band of rows
-    //   ERLBM_PAIR       1: D2Q9 kernel with two adjacent nodes per thread
-    long long tune_pad = 0;
-    //   ERLBM_BLOCK      threads per block of that kernel (128, 256, 512)
-    int tune_npt = 1, tune_ntl = 1, tune_remap = 0, tune_pair = 0, tune_block = 256;
-};
-
 namespace {
 
-int mon_blocks(const erlbm_lattice *h) { return (h->n_fluid + BLOCK - 1) / BLOCK; }
-
-Geometry geom(const erlbm_lattice *h) { return Geometry{h->nx, h->ny, h->nz, h->si, h->sr}; }
 
 // Steps actually run since the upload (launched, capped by a stop); syncs.
 int steps_run(erlbm_lattice *h, long long *out, int *reason)
@@ -546,9 +401,17 @@ int steps_run(erlbm_lattice *h, long long *out, int *reason)
     return ERLBM_OK;
 }
 
+int launch_finish(erlbm_lattice *h, bool ordered, double *out_dev, unsigned long long check_step, double factor)
+{
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, h->stream, h->ke_node, h->partial_ke, h->partial_rho,
+                       h->n_fluid, mon_blocks(h), ordered ? 1 : 0, out_dev, h->scalars + 2, factor, check_step,
+                       h->status);
+    ERLBM_HIP_TRY(hipGetLastError());
+    return ERLBM_OK;
+}
+
 template <int Q>
-int launch_monitor(erlbm_lattice *h, const double *src, bool stream, bool ordered, double *out_dev,
-                   unsigned long long check_step, double factor)
+int launch_monitor(erlbm_lattice *h, const double *src, bool stream, bool ordered)
 {
     MonArgs m;
     m.src = src;
@@ -561,21 +424,25 @@ int launch_monitor(erlbm_lattice *h, const double *src, bool stream, bool ordere
     m.partial_rho = h->partial_rho;
     m.nonfinite = h->nonfinite;
     const int nb = mon_blocks(h);
-    ERLBM_HIP_TRY(hipMemsetAsync(h->nonfinite, 0, sizeof(int), h->stream));
     if (stream) hipLaunchKernelGGL((k_monitor<Q, true>), dim3(nb), dim3(BLOCK), 0, h->stream, m);
     else hipLaunchKernelGGL((k_monitor<Q, false>), dim3(nb), dim3(BLOCK), 0, h->stream, m);
-    ERLBM_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, h->stream, h->ke_node, h->partial_ke, h->partial_rho,
-                       h->n_fluid, nb, ordered ? 1 : 0, out_dev, h->scalars + 2, factor, check_step, h->status);
     ERLBM_HIP_TRY(hipGetLastError());
     return ERLBM_OK;
 }
 
-int monitor(erlbm_lattice *h, const double *src, bool stream, bool ordered, double *out_dev,
+// Energy / density reduction of the state in `src` (gathered through the
+// stream when `stream`), result in out_dev[0..1]; with check_step set, the
+// Taylor-Green map test stops the run before that step.
+int monitor(erlbm_lattice *h, const void *src, bool stream, bool ordered, double *out_dev,
             unsigned long long check_step = NO_STOP, double factor = 2.0)
 {
-    return h->q == 9 ? launch_monitor<9>(h, src, stream, ordered, out_dev, check_step, factor)
-                     : launch_monitor<19>(h, src, stream, ordered, out_dev, check_step, factor);
+    ERLBM_HIP_TRY(hipMemsetAsync(h->nonfinite, 0, sizeof(int), h->stream));
+    int rc;
+    if (h->elem != 8) rc = lowp_monitor(h, src, stream, ordered);
+    else if (h->q == 9) rc = launch_monitor<9>(h, static_cast<const double *>(src), stream, ordered);
+    else rc = launch_monitor<19>(h, static_cast<const double *>(src), stream, ordered);
+    if (rc) return rc;
+    return launch_finish(h, ordered, out_dev, check_step, factor);
 }
 
 template <int Q, int COL, bool STREAM, int NPT, bool NTL, int BS>
@@ -659,10 +526,11 @@ extern "C" int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *ma
         set_last_error_msg("erlbm_lattice_create: D2Q9 needs nz == 1");
         return ERLBM_ERR_ARG;
     }
-    if (precision != ERLBM_FP64) {
-        set_last_error_msg("erlbm_lattice_create: the plain lattice path stores FP64 (shifted precisions are "
-                           "served by the batched map path)");
-        return ERLBM_ERR_UNSUPPORTED;
+    if (precision != ERLBM_FP64 && precision != ERLBM_FP32_SHIFTED && precision != ERLBM_FP16_SHIFTED &&
+        precision != ERLBM_FP32_SHIFTED_F32) {
+        set_last_error_msg("erlbm_lattice_create: precision must be FP64, FP32_SHIFTED, FP16_SHIFTED or "
+                           "FP32_SHIFTED_F32");
+        return ERLBM_ERR_ARG;
     }
     const long long nvox = (long long)nx * ny * nz;
     if (nvox > (1ll << 31) - 1) {
@@ -685,6 +553,7 @@ extern "C" int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *ma
     h->tune_remap = (int)env_ll("ERLBM_XCD_REMAP", h->tune_remap);
     h->tune_pair = (int)env_ll("ERLBM_PAIR", h->tune_pair);
     h->tune_block = (int)env_ll("ERLBM_BLOCK", h->tune_block);
+    h->tune_vec = (int)env_ll("ERLBM_VEC", h->tune_vec);
     if (h->tune_pad < 0 || h->tune_pad > (1 << 24)) h->tune_pad = 0;
     h->layout = (int)env_ll("ERLBM_LAYOUT", h->layout);
     const long long pad = (h->tune_pad + 63) / 64 * 64;
@@ -698,6 +567,8 @@ extern "C" int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *ma
     }
     h->extent = (q - 1) * h->si + (nrows - 1) * h->sr + nx;
     h->precision = precision;
+    h->elem = precision == ERLBM_FP64 ? 8 : (precision == ERLBM_FP16_SHIFTED ? 2 : 4);
+    h->f32_arith = precision == ERLBM_FP32_SHIFTED_F32;
     h->stream = (hipStream_t)stream;
     hipGetDevice(&h->device);
 
@@ -740,7 +611,7 @@ extern "C" int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *ma
         return ERLBM_ERR_NOMEM;
     };
     hipError_t e;
-    const size_t fbytes = sizeof(double) * (size_t)h->extent;
+    const size_t fbytes = (size_t)h->elem * (size_t)h->extent;
     if ((e = hipMalloc(&h->buf[0], fbytes)) != hipSuccess) return fail(e, "hipMalloc populations");
     if ((e = hipMalloc(&h->buf[1], fbytes)) != hipSuccess) return fail(e, "hipMalloc populations");
     if ((e = hipMalloc(&h->nbr, sizeof(uint32_t) * nvox)) != hipSuccess) return fail(e, "hipMalloc mask");
@@ -845,17 +716,34 @@ extern "C" int erlbm_set_density_check(erlbm_lattice *h, double limit)
 extern "C" int erlbm_upload_aos(erlbm_lattice *h, const double *inif)
 {
     if (!h || !inif) return ERLBM_ERR_ARG;
-    // stage in buffer 1, transpose into buffer 0
     const size_t bytes = sizeof(double) * (size_t)h->n_fluid * h->q;
-    ERLBM_HIP_TRY(hipMemcpyAsync(h->buf[1], inif, bytes, hipMemcpyHostToDevice, h->stream));
     const int nb = mon_blocks(h);
-    if (h->q == 9)
-        hipLaunchKernelGGL(k_from_aos<9>, dim3(nb), dim3(BLOCK), 0, h->stream, h->buf[1], geom(h), h->vox_of_fluid,
-                           h->n_fluid, h->buf[0]);
-    else
-        hipLaunchKernelGGL(k_from_aos<19>, dim3(nb), dim3(BLOCK), 0, h->stream, h->buf[1], geom(h),
-                           h->vox_of_fluid, h->n_fluid, h->buf[0]);
-    ERLBM_HIP_TRY(hipGetLastError());
+    if (h->elem == 8) {
+        // stage in buffer 1, transpose into buffer 0
+        double *stage = static_cast<double *>(h->buf[1]);
+        double *dst = static_cast<double *>(h->buf[0]);
+        ERLBM_HIP_TRY(hipMemcpyAsync(stage, inif, bytes, hipMemcpyHostToDevice, h->stream));
+        if (h->q == 9)
+            hipLaunchKernelGGL(k_from_aos<9>, dim3(nb), dim3(BLOCK), 0, h->stream, stage, geom(h), h->vox_of_fluid,
+                               h->n_fluid, dst);
+        else
+            hipLaunchKernelGGL(k_from_aos<19>, dim3(nb), dim3(BLOCK), 0, h->stream, stage, geom(h), h->vox_of_fluid,
+                               h->n_fluid, dst);
+        ERLBM_HIP_TRY(hipGetLastError());
+    } else {
+        double *stage = nullptr;
+        ERLBM_HIP_TRY(hipMalloc(&stage, bytes));
+        h->shift = h->rho0;
+        int rc = ERLBM_OK;
+        if (hipMemcpyAsync(stage, inif, bytes, hipMemcpyHostToDevice, h->stream) != hipSuccess) rc = ERLBM_ERR_HIP;
+        if (rc == ERLBM_OK) rc = lowp_from_aos(h, stage, h->buf[0]);
+        if (hipStreamSynchronize(h->stream) != hipSuccess && rc == ERLBM_OK) rc = ERLBM_ERR_HIP;
+        (void)hipFree(stage);
+        if (rc) {
+            set_last_error_msg("erlbm_upload_aos: shifted upload failed");
+            return rc;
+        }
+    }
     hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, h->stream, h->status);
     ERLBM_HIP_TRY(hipGetLastError());
     h->base = 0;
@@ -900,10 +788,13 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
             int rc = monitor(h, h->buf[cur], true, ordered, h->scalars, (unsigned long long)k, energy_factor);
             if (rc) return rc;
         }
-        a.src = h->buf[cur];
-        a.dst = h->buf[cur ^ 1];
+        a.src = static_cast<const double *>(h->buf[cur]);
+        a.dst = static_cast<double *>(h->buf[cur ^ 1]);
         a.step = (unsigned long long)k;
-        int rc = h->q == 9 ? launch_step<9>(h, a, k > 0) : launch_step<19>(h, a, k > 0);
+        int rc;
+        if (vec_step_applies(h)) rc = vec_step(h, a, k > 0);
+        else if (h->elem != 8) rc = lowp_step(h, a, k > 0);
+        else rc = h->q == 9 ? launch_step<9>(h, a, k > 0) : launch_step<19>(h, a, k > 0);
         if (rc) return rc;
         h->launched++;
     }
@@ -974,27 +865,39 @@ extern "C" int erlbm_download_aos(erlbm_lattice *h, int which, double *inif)
         return ERLBM_ERR_STATE;
     }
     const bool stream = (which == ERLBM_STATE) && run > 0;
-    double *stage = h->buf[cur ^ 1];
+    const size_t bytes = sizeof(double) * (size_t)h->n_fluid * h->q;
+    if (h->elem != 8) {
+        double *stage = nullptr;
+        ERLBM_HIP_TRY(hipMalloc(&stage, bytes));
+        rc = lowp_to_aos(h, h->buf[cur], stream, stage);
+        if (rc == ERLBM_OK && hipMemcpyAsync(inif, stage, bytes, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+            rc = ERLBM_ERR_HIP;
+        if (hipStreamSynchronize(h->stream) != hipSuccess && rc == ERLBM_OK) rc = ERLBM_ERR_HIP;
+        (void)hipFree(stage);
+        if (rc) set_last_error_msg("erlbm_download_aos: shifted download failed");
+        return rc;
+    }
+    double *stage = static_cast<double *>(h->buf[cur ^ 1]);
+    const double *src = static_cast<const double *>(h->buf[cur]);
     const int nb = mon_blocks(h);
     const uint32_t *nbr = h->all_fluid ? nullptr : h->nbr;
     if (h->q == 9) {
         if (stream)
-            hipLaunchKernelGGL((k_to_aos<9, true>), dim3(nb), dim3(BLOCK), 0, h->stream, h->buf[cur], geom(h), nbr,
+            hipLaunchKernelGGL((k_to_aos<9, true>), dim3(nb), dim3(BLOCK), 0, h->stream, src, geom(h), nbr,
                                h->vox_of_fluid, h->n_fluid, stage);
         else
-            hipLaunchKernelGGL((k_to_aos<9, false>), dim3(nb), dim3(BLOCK), 0, h->stream, h->buf[cur], geom(h), nbr,
+            hipLaunchKernelGGL((k_to_aos<9, false>), dim3(nb), dim3(BLOCK), 0, h->stream, src, geom(h), nbr,
                                h->vox_of_fluid, h->n_fluid, stage);
     } else {
         if (stream)
-            hipLaunchKernelGGL((k_to_aos<19, true>), dim3(nb), dim3(BLOCK), 0, h->stream, h->buf[cur], geom(h), nbr,
+            hipLaunchKernelGGL((k_to_aos<19, true>), dim3(nb), dim3(BLOCK), 0, h->stream, src, geom(h), nbr,
                                h->vox_of_fluid, h->n_fluid, stage);
         else
-            hipLaunchKernelGGL((k_to_aos<19, false>), dim3(nb), dim3(BLOCK), 0, h->stream, h->buf[cur], geom(h),
-                               nbr, h->vox_of_fluid, h->n_fluid, stage);
+            hipLaunchKernelGGL((k_to_aos<19, false>), dim3(nb), dim3(BLOCK), 0, h->stream, src, geom(h), nbr,
+                               h->vox_of_fluid, h->n_fluid, stage);
     }
     ERLBM_HIP_TRY(hipGetLastError());
-    ERLBM_HIP_TRY(hipMemcpyAsync(inif, stage, sizeof(double) * (size_t)h->n_fluid * h->q, hipMemcpyDeviceToHost,
-                                 h->stream));
+    ERLBM_HIP_TRY(hipMemcpyAsync(inif, stage, bytes, hipMemcpyDeviceToHost, h->stream));
     ERLBM_HIP_TRY(hipStreamSynchronize(h->stream));
     return ERLBM_OK;
 }

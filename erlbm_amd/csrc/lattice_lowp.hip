@@ -1,0 +1,388 @@
+// lattice_lowp.hip -- the plain lattice path with shifted low-precision
+// population storage.
+//
+// Populations are stored as deviations d_i = f_i - w_i rho0 (rho0 = the
+// lattice's reference density at upload) in FP32 or FP16.  The collision
+// runs in FP64 on f_i = d_i + w_i rho0 (collide.hpp), or, with F32A, in
+// FP32 on the deviations (collide_f32.hpp: compensated moment sums, FMA).  The stream, the
+// boundary columns (Zou-He inlet and outlet rescale, evaluated on absolute
+// FP64 populations), the stop tests and the monitors follow the FP64 path
+// (lattice.hip) step for step; energies and densities are reduced in FP64.
+// Storage traffic per D2Q9 node-step: 72 B (FP32) or 36 B (FP16) instead of
+// 144 B.
+#include <hip/hip_runtime.h>
+
+#include "collide_f32.hpp"
+#include "lattice_impl.hpp"
+
+namespace erlbm {
+namespace lat {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ float ld(const T *p, bool nt)
+{
+    if (nt) return (float)__builtin_nontemporal_load(p);
+    return (float)*p;
+}
+
+template <typename T>
+__device__ __forceinline__ void st(T *p, float v)
+{
+    __builtin_nontemporal_store((T)v, p);
+}
+
+struct LPArgs {
+    const void *src;
+    void *dst;
+    Geometry g;
+    const uint32_t *__restrict__ nbr;
+    CollideConstsF32 k;
+    CollideConsts k64;
+    BCArgs bc;
+    double shift;                      // rho0 of the storage
+    double dens_limit;
+    unsigned long long *status;
+    const unsigned long long *__restrict__ stop_ro;
+    unsigned long long step;
+};
+
+template <int Q, typename T, bool STREAM, bool SOLID>
+__device__ __forceinline__ void gather_lp(const T *__restrict__ src, const Geometry &g, uint32_t nb, int x, int y,
+                                          int z, float (&d)[Q])
+{
+    using V = VSet<Q>;
+    const long long row = y + (long long)g.ny * z;
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        if constexpr (!STREAM || i == 0) {
+            d[i] = ld(&src[at(g, i, row, x)], true);
+        } else {
+            if (SOLID && (nb & (1u << i))) {
+                d[i] = ld(&src[at(g, V::opp[i], row, x)], true);
+            } else {
+                const int xs = V::cx[i] == 0 ? x : wrap(x - V::cx[i], g.nx);
+                const int ys = V::cy[i] == 0 ? y : wrap(y - V::cy[i], g.ny);
+                const int zs = V::cz[i] == 0 ? z : wrap(z - V::cz[i], g.nz);
+                d[i] = ld(&src[at(g, i, ys + (long long)g.ny * zs, xs)], true);
+            }
+        }
+    });
+}
+
+template <int Q>
+__device__ __forceinline__ void to_abs(const float (&d)[Q], double (&f)[Q], double shift)
+{
+    static_for<Q>([&](auto I) { f[(int)I] = (double)d[(int)I] + VSet<Q>::w[(int)I] * shift; });
+}
+
+template <int Q>
+__device__ __forceinline__ void from_abs(const double (&f)[Q], float (&d)[Q], double shift)
+{
+    static_for<Q>([&](auto I) { d[(int)I] = (float)(f[(int)I] - VSet<Q>::w[(int)I] * shift); });
+}
+
+// Periodic all-fluid lattice.
+// Collision of shifted populations d (and the density for the stop test).
+template <int Q, int COL, bool F32A>
+__device__ __forceinline__ double collide_lp(float (&d)[Q], const LPArgs &a)
+{
+    if constexpr (F32A) {
+        float drho, vx, vy, vz;
+        collide_shifted_f32<Q, COL>(d, a.k, drho, vx, vy, vz);
+        return density_shifted_f64<Q>(d, a.shift);
+    } else {
+        double f[Q];
+        to_abs<Q>(d, f, a.shift);
+        double vx, vy, vz;
+        collide_exact<Q, COL>(f, a.k64, vx, vy, vz);
+        from_abs<Q>(f, d, a.shift);
+        return density_exact<Q>(f);
+    }
+}
+
+template <int Q, int COL, typename T, bool STREAM, bool F32A>
+__global__ __launch_bounds__(BLOCK) void k_lp_periodic(LPArgs a)
+{
+    const T *__restrict__ src = static_cast<const T *>(a.src);
+    T *__restrict__ dst = static_cast<T *>(a.dst);
+    const int row = blockIdx.y;
+    const int x = blockIdx.x * BLOCK + (int)threadIdx.x;
+    const int y = row % a.g.ny, z = row / a.g.ny;
+    float d[Q];
+    if (x < a.g.nx) gather_lp<Q, T, STREAM, false>(src, a.g, 0u, x, y, z, d);
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+    bool bad = false;
+    if (x < a.g.nx) {
+        const double r = collide_lp<Q, COL, F32A>(d, a);
+        if (a.dens_limit > 0.0) bad = !(fabs(r) <= a.dens_limit);
+        static_for<Q>([&](auto I) { st(&dst[at(a.g, (int)I, row, x)], d[(int)I]); });
+    }
+    flag_density(a.status, a.step, bad);
+}
+
+// Any geometry, boundary columns on absolute FP64 populations.
+template <int Q, int COL, typename T, bool STREAM, bool F32A>
+__global__ __launch_bounds__(BLOCK) void k_lp_general(LPArgs a)
+{
+    using V = VSet<Q>;
+    const T *__restrict__ src = static_cast<const T *>(a.src);
+    T *__restrict__ dst = static_cast<T *>(a.dst);
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+    const long long nvox = (long long)a.g.nx * a.g.ny * a.g.nz;
+    const long long v = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    bool bad = false;
+    if (v < nvox) {
+        const uint32_t nb = a.nbr[v];
+        if (nb & FLUID_BIT) {
+            const int x = (int)(v % a.g.nx);
+            const long long yz = v / a.g.nx;
+            const int y = (int)(yz % a.g.ny);
+            const int z = (int)(yz / a.g.ny);
+            float d[Q];
+            gather_lp<Q, T, STREAM, true>(src, a.g, nb, x, y, z, d);
+            if (x == a.bc.inlet_x || x == a.bc.outlet_x) {
+                double f[Q];
+                to_abs<Q>(d, f, a.shift);
+                if (x == a.bc.inlet_x) zouhe<Q>(f, a.bc.u_in, a.bc.inlet_west);
+                if (x == a.bc.outlet_x) {
+                    const int xn = x + a.bc.outlet_dx;
+                    const long long vn = xn + (long long)a.g.nx * yz;
+                    const uint32_t nbn = a.nbr[vn];
+                    if (nbn & FLUID_BIT) {
+                        float dn[Q];
+                        gather_lp<Q, T, STREAM, true>(src, a.g, nbn, xn, y, z, dn);
+                        double fa[Q];
+                        to_abs<Q>(dn, fa, a.shift);
+                        if (xn == a.bc.inlet_x) zouhe<Q>(fa, a.bc.u_in, a.bc.inlet_west);
+                        const double fator = a.bc.rho_out / density_exact<Q>(fa);
+                        static_for<Q>([&](auto I) { f[(int)I] = fa[(int)I] * fator; });
+                    } else {
+                        static_for<Q>([&](auto I) { f[(int)I] = V::w[(int)I] * a.bc.rho_out; });
+                    }
+                }
+                from_abs<Q>(f, d, a.shift);
+            }
+            const double r = collide_lp<Q, COL, F32A>(d, a);
+            if (a.dens_limit > 0.0) bad = !(fabs(r) <= a.dens_limit);
+            static_for<Q>([&](auto I) { st(&dst[at(a.g, (int)I, yz, x)], d[(int)I]); });
+        }
+    }
+    flag_density(a.status, a.step, bad);
+}
+
+template <int Q, typename T>
+__global__ __launch_bounds__(BLOCK) void k_lp_from_aos(const double *__restrict__ aos, Geometry g,
+                                                       const int *__restrict__ vof, int n_fluid, double shift,
+                                                       T *__restrict__ dst)
+{
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= n_fluid) return;
+    const long long v = vof ? vof[p] : p;
+    const int x = (int)(v % g.nx);
+    const long long row = v / g.nx;
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        dst[at(g, i, row, x)] = (T)(float)(aos[(long long)p * Q + i] - VSet<Q>::w[i] * shift);
+    });
+}
+
+template <int Q, typename T, bool STREAM>
+__global__ __launch_bounds__(BLOCK) void k_lp_to_aos(const T *__restrict__ src, Geometry g,
+                                                     const uint32_t *__restrict__ nbr, const int *__restrict__ vof,
+                                                     int n_fluid, double shift, double *__restrict__ aos)
+{
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= n_fluid) return;
+    const long long v = vof ? vof[p] : p;
+    const int x = (int)(v % g.nx);
+    const long long yz = v / g.nx;
+    const int y = (int)(yz % g.ny), z = (int)(yz / g.ny);
+    float d[Q];
+    if (nbr) gather_lp<Q, T, STREAM, true>(src, g, nbr[v], x, y, z, d);
+    else gather_lp<Q, T, STREAM, false>(src, g, 0u, x, y, z, d);
+    double f[Q];
+    to_abs<Q>(d, f, shift);
+    static_for<Q>([&](auto I) { aos[(long long)p * Q + (int)I] = f[(int)I]; });
+}
+
+// Per-site FP64 energy and density of the absolute populations.
+template <int Q, typename T, bool STREAM>
+__global__ __launch_bounds__(BLOCK) void k_lp_monitor(const T *__restrict__ src, Geometry g,
+                                                      const uint32_t *__restrict__ nbr, const int *__restrict__ vof,
+                                                      int n_fluid, double shift, double *ke_node,
+                                                      double *partial_ke, double *partial_rho, int *nonfinite)
+{
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    double e = 0.0, r_abs = 0.0;
+    bool nf = false;
+    if (p < n_fluid) {
+        const long long v = vof ? vof[p] : p;
+        const int x = (int)(v % g.nx);
+        const long long yz = v / g.nx;
+        const int y = (int)(yz % g.ny), z = (int)(yz / g.ny);
+        float d[Q];
+        if (nbr) gather_lp<Q, T, STREAM, true>(src, g, nbr[v], x, y, z, d);
+        else gather_lp<Q, T, STREAM, false>(src, g, 0u, x, y, z, d);
+        double f[Q];
+        to_abs<Q>(d, f, shift);
+        double rho, vx, vy, vz;
+        moments_exact<Q>(f, rho, vx, vy, vz);
+        e = ke_term<Q>(vx, vy, vz);
+        r_abs = fabs(rho);
+        nf = !isfinite(rho);
+        if (ke_node) ke_node[p] = e;
+    }
+    if (__any(nf) && (threadIdx.x & 63) == 0) atomicOr(nonfinite, 1);
+    __shared__ double s_e[BLOCK / 64], s_r[BLOCK / 64];
+    e = wave_sum(e);
+    r_abs = wave_max(r_abs);
+    if ((threadIdx.x & 63) == 0) {
+        s_e[threadIdx.x >> 6] = e;
+        s_r[threadIdx.x >> 6] = r_abs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double te = 0.0, tr = 0.0;
+        for (int w = 0; w < BLOCK / 64; w++) {
+            te += s_e[w];
+            tr = fmax(tr, s_r[w]);
+        }
+        partial_ke[blockIdx.x] = te;
+        partial_rho[blockIdx.x] = tr;
+    }
+}
+
+LPArgs make_args(erlbm_lattice *h, const StepArgs &a)
+{
+    LPArgs l;
+    l.src = a.src;
+    l.dst = a.dst;
+    l.g = a.g;
+    l.nbr = a.nbr;
+    l.k.omega1 = (float)(1.0 - 1.0 / h->tau);
+    l.k.omega2 = (float)(1.0 - 1.0 / h->tau_nh);
+    l.k.oot = (float)(1.0 / h->tau);
+    l.k.rho0 = (float)h->shift;
+    l.k64.omega1 = 1.0 - 1.0 / h->tau;
+    l.k64.omega2 = 1.0 - 1.0 / h->tau_nh;
+    l.k64.oot = 1.0 / h->tau;
+    l.bc = a.bc;
+    l.shift = h->shift;
+    l.dens_limit = a.dens_limit;
+    l.status = a.status;
+    l.stop_ro = a.stop_ro;
+    l.step = a.step;
+    return l;
+}
+
+template <int Q, int COL, typename T, bool F32A>
+void step_qcta(erlbm_lattice *h, const LPArgs &l, bool stream)
+{
+    if (h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0) {
+        const dim3 grid((h->nx + BLOCK - 1) / BLOCK, h->ny * h->nz);
+        if (stream) hipLaunchKernelGGL((k_lp_periodic<Q, COL, T, true, F32A>), grid, dim3(BLOCK), 0, h->stream, l);
+        else hipLaunchKernelGGL((k_lp_periodic<Q, COL, T, false, F32A>), grid, dim3(BLOCK), 0, h->stream, l);
+    } else {
+        const unsigned nb = (unsigned)((h->nvox + BLOCK - 1) / BLOCK);
+        if (stream) hipLaunchKernelGGL((k_lp_general<Q, COL, T, true, F32A>), dim3(nb), dim3(BLOCK), 0, h->stream, l);
+        else hipLaunchKernelGGL((k_lp_general<Q, COL, T, false, F32A>), dim3(nb), dim3(BLOCK), 0, h->stream, l);
+    }
+}
+
+template <int Q, int COL, typename T>
+int step_qct(erlbm_lattice *h, const LPArgs &l, bool stream)
+{
+    if constexpr (std::is_same<T, float>::value) {
+        if (h->f32_arith) step_qcta<Q, COL, T, true>(h, l, stream);
+        else step_qcta<Q, COL, T, false>(h, l, stream);
+    } else {
+        step_qcta<Q, COL, T, false>(h, l, stream);
+    }
+    ERLBM_HIP_TRY(hipGetLastError());
+    return ERLBM_OK;
+}
+
+template <int Q, typename T>
+int step_qt(erlbm_lattice *h, const LPArgs &l, bool stream)
+{
+    switch (h->col) {
+    case ERLBM_COL_BGK: return step_qct<Q, COL_BGK, T>(h, l, stream);
+    case ERLBM_COL_REG: return step_qct<Q, COL_REG, T>(h, l, stream);
+    case ERLBM_COL_EXT_REG: return step_qct<Q, COL_EXT_REG, T>(h, l, stream);
+    default: return ERLBM_ERR_UNSUPPORTED;
+    }
+}
+
+template <int Q, typename T>
+int to_aos_qt(erlbm_lattice *h, const void *src, bool stream, double *aos)
+{
+    const uint32_t *nbr = h->all_fluid ? nullptr : h->nbr;
+    const T *s = static_cast<const T *>(src);
+    if (stream)
+        hipLaunchKernelGGL((k_lp_to_aos<Q, T, true>), dim3(mon_blocks(h)), dim3(BLOCK), 0, h->stream, s, geom(h), nbr,
+                           h->vox_of_fluid, h->n_fluid, h->shift, aos);
+    else
+        hipLaunchKernelGGL((k_lp_to_aos<Q, T, false>), dim3(mon_blocks(h)), dim3(BLOCK), 0, h->stream, s, geom(h),
+                           nbr, h->vox_of_fluid, h->n_fluid, h->shift, aos);
+    ERLBM_HIP_TRY(hipGetLastError());
+    return ERLBM_OK;
+}
+
+template <int Q, typename T>
+int monitor_qt(erlbm_lattice *h, const void *src, bool stream, bool ordered)
+{
+    const uint32_t *nbr = h->all_fluid ? nullptr : h->nbr;
+    const T *s = static_cast<const T *>(src);
+    double *ke = ordered ? h->ke_node : nullptr;
+    if (stream)
+        hipLaunchKernelGGL((k_lp_monitor<Q, T, true>), dim3(mon_blocks(h)), dim3(BLOCK), 0, h->stream, s, geom(h),
+                           nbr, h->vox_of_fluid, h->n_fluid, h->shift, ke, h->partial_ke, h->partial_rho,
+                           h->nonfinite);
+    else
+        hipLaunchKernelGGL((k_lp_monitor<Q, T, false>), dim3(mon_blocks(h)), dim3(BLOCK), 0, h->stream, s, geom(h),
+                           nbr, h->vox_of_fluid, h->n_fluid, h->shift, ke, h->partial_ke, h->partial_rho,
+                           h->nonfinite);
+    ERLBM_HIP_TRY(hipGetLastError());
+    return ERLBM_OK;
+}
+
+template <int Q, typename T>
+int from_aos_qt(erlbm_lattice *h, const double *aos, void *dst)
+{
+    hipLaunchKernelGGL((k_lp_from_aos<Q, T>), dim3(mon_blocks(h)), dim3(BLOCK), 0, h->stream, aos, geom(h),
+                       h->vox_of_fluid, h->n_fluid, h->shift, static_cast<T *>(dst));
+    ERLBM_HIP_TRY(hipGetLastError());
+    return ERLBM_OK;
+}
+
+}  // namespace
+
+#define ERLBM_LP_DISPATCH(fn, ...)                                                          \
+    do {                                                                                    \
+        if (h->q == 9)                                                                      \
+            return h->elem == 4 ? fn<9, float>(h, __VA_ARGS__) : fn<9, _Float16>(h, __VA_ARGS__);   \
+        return h->elem == 4 ? fn<19, float>(h, __VA_ARGS__) : fn<19, _Float16>(h, __VA_ARGS__);     \
+    } while (0)
+
+int lowp_step(erlbm_lattice *h, const StepArgs &a, bool stream)
+{
+    const LPArgs l = make_args(h, a);
+    ERLBM_LP_DISPATCH(step_qt, l, stream);
+}
+
+int lowp_from_aos(erlbm_lattice *h, const double *aos, void *dst) { ERLBM_LP_DISPATCH(from_aos_qt, aos, dst); }
+
+int lowp_to_aos(erlbm_lattice *h, const void *src, bool stream, double *aos)
+{
+    ERLBM_LP_DISPATCH(to_aos_qt, src, stream, aos);
+}
+
+int lowp_monitor(erlbm_lattice *h, const void *src, bool stream, bool ordered)
+{
+    ERLBM_LP_DISPATCH(monitor_qt, src, stream, ordered);
+}
+
+}  // namespace lat
+}  // namespace erlbm

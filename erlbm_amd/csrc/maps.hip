@@ -16,7 +16,8 @@
 // (ordered FP64 sum over the sites, main_stability-map_taylor-green.cpp:71-81)
 // or the post-collision density test every step
 // (main_stability-map_obstacle.cpp:104-106).  FP64 jobs are bitwise those of
-// the reference; FP32 jobs store shifted deviations (collide_f32.hpp).
+// the reference; FP32 jobs store shifted deviations f_i - w_i rho0 in LDS
+// (half the footprint) and collide in FP64 or, on request, in FP32.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -64,6 +65,7 @@ struct RunArgs {
 // ---- storage policies -------------------------------------------------------
 struct F64 {
     using T = double;
+    static constexpr int code = ERLBM_FP64;
     struct K {
         CollideConsts c;
     };
@@ -86,7 +88,7 @@ struct F64 {
         return ke_term<Q>(vx, vy, vz);
     }
     template <int COL>
-    __device__ static void collide(double (&f)[Q], const K &k)
+    __device__ static void collide(double (&f)[Q], const K &k, double)
     {
         double vx, vy, vz;
         collide_exact<Q, COL>(f, k.c, vx, vy, vz);
@@ -102,8 +104,56 @@ struct F64 {
     }
 };
 
+// FP32-shifted storage d_i = f_i - w_i rho0; FP64 arithmetic on the
+// absolute populations (the exact collision of collide.hpp).
 struct F32 {
     using T = float;
+    static constexpr int code = ERLBM_FP32_SHIFTED;
+    struct K {
+        CollideConsts c;
+    };
+    __device__ static K consts(const MapJob &J) { return K{F64::consts(J).c}; }
+    __device__ static void to_abs(const float (&d)[Q], double (&a)[Q], double rho0)
+    {
+        for (int i = 0; i < Q; i++) a[i] = (double)d[i] + VSet<Q>::w[i] * rho0;
+    }
+    __device__ static void from_abs(const double (&a)[Q], float (&d)[Q], double rho0)
+    {
+        for (int i = 0; i < Q; i++) d[i] = (float)(a[i] - VSet<Q>::w[i] * rho0);
+    }
+    __device__ static void init(float (&d)[Q], double rho0, double ux, double uy)
+    {
+        double f[Q];
+        feq_exact<Q>(rho0, ux, uy, 0.0, f);
+        from_abs(f, d, rho0);
+    }
+    __device__ static double ke(const float (&d)[Q], double rho0)
+    {
+        double f[Q];
+        to_abs(d, f, rho0);
+        return F64::ke(f, rho0);
+    }
+    template <int COL>
+    __device__ static void collide(float (&d)[Q], const K &k, double rho0)
+    {
+        double f[Q];
+        to_abs(d, f, rho0);
+        double vx, vy, vz;
+        collide_exact<Q, COL>(f, k.c, vx, vy, vz);
+        from_abs(f, d, rho0);
+    }
+    __device__ static double density(const float (&d)[Q], double rho0)
+    {
+        double f[Q];
+        to_abs(d, f, rho0);
+        return density_exact<Q>(f);
+    }
+};
+
+// FP32-shifted storage with FP32 arithmetic (collide_f32.hpp).
+struct F32A {
+    using T = float;
+    static constexpr int code = ERLBM_FP32_SHIFTED_F32;
     struct K {
         CollideConstsF32 c;
     };
@@ -116,12 +166,7 @@ struct F32 {
         k.c.rho0 = (float)J.rho0;
         return k;
     }
-    __device__ static void init(float (&d)[Q], double rho0, double ux, double uy)
-    {
-        double f[Q];
-        feq_exact<Q>(rho0, ux, uy, 0.0, f);
-        for (int i = 0; i < Q; i++) d[i] = (float)(f[i] - VSet<Q>::w[i] * rho0);
-    }
+    __device__ static void init(float (&d)[Q], double rho0, double ux, double uy) { F32::init(d, rho0, ux, uy); }
     __device__ static double ke(const float (&d)[Q], double rho0)
     {
         float drho, rho, vx, vy, vz;
@@ -130,20 +175,14 @@ struct F32 {
         return 0.5 * (x * x + y * y + z * z);
     }
     template <int COL>
-    __device__ static void collide(float (&d)[Q], const K &k)
+    __device__ static void collide(float (&d)[Q], const K &k, double)
     {
         float drho, vx, vy, vz;
         collide_shifted_f32<Q, COL>(d, k.c, drho, vx, vy, vz);
     }
     __device__ static double density(const float (&d)[Q], double rho0) { return density_shifted_f64<Q>(d, rho0); }
-    __device__ static void to_abs(const float (&d)[Q], double (&a)[Q], double rho0)
-    {
-        for (int i = 0; i < Q; i++) a[i] = (double)d[i] + VSet<Q>::w[i] * rho0;
-    }
-    __device__ static void from_abs(const double (&a)[Q], float (&d)[Q], double rho0)
-    {
-        for (int i = 0; i < Q; i++) d[i] = (float)(a[i] - VSet<Q>::w[i] * rho0);
-    }
+    __device__ static void to_abs(const float (&d)[Q], double (&a)[Q], double rho0) { F32::to_abs(d, a, rho0); }
+    __device__ static void from_abs(const double (&a)[Q], float (&d)[Q], double rho0) { F32::from_abs(a, d, rho0); }
 };
 
 // ZouHeD3Q19_vx_x (Boundary_conditions.cpp:350-442), west branch, on
@@ -276,6 +315,8 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
     long long s = 0;
     int unstable = 0;
     double e_last = e0;
+    double e_max = e0;      // largest energy seen by the energy test
+    double r_max = 0.0;     // largest |rho| seen by this thread's density test
     for (; s < J.n_steps; s++) {
         const bool stream = s > 0;
         double fator = 0.0;   // outlet column: rho0 / rho(upstream neighbour)
@@ -294,6 +335,7 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
             // (energy tests run on lattices without an outlet column; checked on the host)
             const double e = ordered_energy<P>(f, active, rho0, scratch, nf, bcast);
             e_last = e;
+            e_max = (e > e_max || !isfinite(e)) ? e : e_max;
             if (e > J.energy_factor * e0 || !isfinite(e)) {
                 unstable = 1;
                 break;
@@ -319,10 +361,11 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
                     P::from_abs(fd, f, rho0);
                 }
             }
-            P::template collide<COL>(f, k);
+            P::template collide<COL>(f, k, rho0);
             if (J.dens_limit > 0.0) {
                 const double d = P::density(f, rho0);
                 bad = !(fabs(d) <= J.dens_limit);
+                r_max = (fabs(d) > r_max || !isfinite(d)) ? fabs(d) : r_max;
             }
         }
         __syncthreads();  // every gather of this step is done
@@ -335,12 +378,29 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
             break;
         }
     }
+    double margin = 0.0;
+    if (J.dens_limit > 0.0) {
+        // block max of the per-thread |rho| maxima (NaN propagates)
+        __syncthreads();
+        scratch[tid] = r_max;
+        __syncthreads();
+        if (tid == 0) {
+            double m = 0.0;
+            for (int p = 0; p < TPB; p++) m = (scratch[p] > m || !isfinite(scratch[p])) ? scratch[p] : m;
+            bcast[1] = m / J.dens_limit;
+        }
+        __syncthreads();
+        margin = bcast[1];
+    } else if (J.check_every > 0) {
+        margin = e_max / e0;
+    }
     if (tid == 0) {
         MapResult r;
+        r.margin = margin;
         r.unstable = unstable;
         r.steps = (int)s;
         r.done = 1;
-        r.precision = std::is_same<T, double>::value ? ERLBM_FP64 : ERLBM_FP32_SHIFTED;
+        r.precision = P::code;
         r.e0 = e0;
         r.energy = e_last;
         *res = r;
@@ -397,7 +457,7 @@ int launch_group(erlbm_batch *b, const RunArgs &a, unsigned int *counter)
     int per_cu = 0;
     const size_t lds = sizeof(typename P::T) * Q * (size_t)a.g.nvp;
     ERLBM_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, GENERAL>, TPB, lds));
-    const int want = b->wg_per_cu[std::is_same<typename P::T, double>::value ? 0 : 1];
+    const int want = b->wg_per_cu[P::code == ERLBM_FP64 ? 0 : 1];
     if (want > 0 && want < per_cu) per_cu = want;
     if (per_cu < 1) per_cu = 1;
     int grid = b->n_cu * per_cu;
@@ -536,7 +596,8 @@ extern "C" int erlbm_batch_run(erlbm_batch *b, const erlbm_job *jobs, int n_jobs
         if (!(j.tau > 0) || !(j.tau_nh > 0) || j.n_steps < 0 || j.n_steps > 2000000000LL ||
             (j.collision != ERLBM_COL_BGK && j.collision != ERLBM_COL_REG && j.collision != ERLBM_COL_EXT_REG) ||
             (j.init != ERLBM_INIT_REST && j.init != ERLBM_INIT_TAYLOR_GREEN) ||
-            (j.precision != ERLBM_FP64 && j.precision != ERLBM_FP32_SHIFTED && j.precision != ERLBM_FP_AUTO)) {
+            (j.precision != ERLBM_FP64 && j.precision != ERLBM_FP32_SHIFTED && j.precision != ERLBM_FP_AUTO &&
+             j.precision != ERLBM_FP32_SHIFTED_F32)) {
             set_last_error_msg("erlbm_batch_run: invalid job");
             return ERLBM_ERR_ARG;
         }
@@ -603,8 +664,9 @@ extern "C" int erlbm_batch_run(erlbm_batch *b, const erlbm_job *jobs, int n_jobs
         a.n_jobs = end - start;
         const int p = prec[idx[start]];
         const int col = jobs[idx[start]].collision;
-        int rc = (p == ERLBM_FP64) ? launch_col<F64>(b, col, a, b->counters + group)
-                                   : launch_col<F32>(b, col, a, b->counters + group);
+        int rc = (p == ERLBM_FP64)            ? launch_col<F64>(b, col, a, b->counters + group)
+                 : (p == ERLBM_FP32_SHIFTED) ? launch_col<F32>(b, col, a, b->counters + group)
+                                             : launch_col<F32A>(b, col, a, b->counters + group);
         if (rc) return rc;
         start = end;
         group++;

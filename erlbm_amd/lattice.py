@@ -19,7 +19,7 @@ from ._lib import check, lib
 
 
 class Lattice:
-    """One lattice resident in HBM (FP64, SoA over voxels).
+    """One lattice resident in HBM (FP64, or FP32/FP16 shifted deviations).
 
     q: 9 (D2Q9) or 19 (D3Q19).  mask[x + y nx + z nx ny] = 1 fluid / 0 solid
     (None: all fluid, periodic in every direction).  Populations cross in the
@@ -27,14 +27,15 @@ class Lattice:
     """
 
     def __init__(self, q: int, nx: int, ny: int, nz: int = 1, mask: Optional[np.ndarray] = None,
-                 stream: int = 0):
+                 stream: int = 0, precision: int = L.FP64):
         self.q, self.nx, self.ny, self.nz = q, nx, ny, nz
         h = C.c_void_p()
         m = None if mask is None else np.ascontiguousarray(mask, dtype=np.int32).reshape(-1)
         if m is not None and m.size != nx * ny * nz:
             raise ValueError("mask size does not match nx*ny*nz")
         mp = None if m is None else m.ctypes.data_as(C.c_void_p)
-        check(lib().erlbm_lattice_create(q, nx, ny, nz, mp, L.FP64, C.c_void_p(stream or None),
+        self.precision = precision
+        check(lib().erlbm_lattice_create(q, nx, ny, nz, mp, precision, C.c_void_p(stream or None),
                                          C.byref(h)), "erlbm_lattice_create")
         self._h = h
         self._mask = m

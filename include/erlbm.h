@@ -39,8 +39,11 @@ enum {
 /* Collision operators: the COLISAO enum of src/erlbm_common.h:42. */
 enum { ERLBM_COL_BGK = 1, ERLBM_COL_MRT = 2, ERLBM_COL_REG = 3, ERLBM_COL_EXT_REG = 4 };
 
-/* Population storage precision. */
-enum { ERLBM_FP64 = 0, ERLBM_FP32_SHIFTED = 1, ERLBM_FP16_SHIFTED = 2 };
+/* Population storage precision.  The shifted formats store f_i - w_i rho0
+ * and compute in FP64 (non-packed FP32 issues at the FP64 rate on MI355X,
+ * and FP32 arithmetic biases the equilibrium); ERLBM_FP32_SHIFTED_F32
+ * computes in FP32 with compensated moment sums.                          */
+enum { ERLBM_FP64 = 0, ERLBM_FP32_SHIFTED = 1, ERLBM_FP16_SHIFTED = 2, ERLBM_FP32_SHIFTED_F32 = 4 };
 
 /* Which populations a monitor reads. */
 enum {
@@ -173,7 +176,7 @@ typedef struct {
     int check_every;       /* energy test before steps k = check_every, 2 check_every, ... (0: none) */
     int init;              /* ERLBM_INIT_* */
     int collision;         /* ERLBM_COL_BGK / REG / EXT_REG */
-    int precision;         /* ERLBM_FP64 / ERLBM_FP32_SHIFTED / ERLBM_FP_AUTO */
+    int precision;         /* ERLBM_FP64 / ERLBM_FP32_SHIFTED / ERLBM_FP32_SHIFTED_F32 / ERLBM_FP_AUTO */
 } erlbm_job;
 
 typedef struct {
@@ -182,6 +185,8 @@ typedef struct {
     int done;              /* 1 when retired (stopped or budget used) */
     int precision;         /* precision the job ran in */
     double e0, energy;     /* initial and last measured kinetic energy (energy test only) */
+    double margin;         /* closeness to the stop test: max E/E0 over the energy tests, or
+                              max |rho| / dens_limit over all steps and sites */
 } erlbm_job_result;
 
 typedef struct erlbm_batch erlbm_batch;

@@ -153,7 +153,7 @@ def test_fma_reference_tolerance_1000_steps(gpu, golden):
 def test_periodic_launch_variants_bitwise(gpu, orc, monkeypatch, npt, ntl, pad, remap, pair, layout):
     """Every launch shape and layout the tuner may pick gives the same populations."""
     for k, v in (("ERLBM_NPT", npt), ("ERLBM_NT_LOADS", ntl), ("ERLBM_PLANE_PAD", pad), ("ERLBM_XCD_REMAP", remap),
-                 ("ERLBM_PAIR", pair), ("ERLBM_LAYOUT", layout)):
+                 ("ERLBM_PAIR", pair), ("ERLBM_LAYOUT", layout), ("ERLBM_VEC", 0)):
         monkeypatch.setenv(k, str(v))
     L, n = 96, 30
     tau = 0.5 + 3. * (1. / 32.) * L / 80000.
@@ -181,3 +181,25 @@ def test_layouts_obstacle_and_monitors(gpu, orc, golden, monkeypatch, layout):
         got = lat.download()
         assert np.array_equal(got, g["obst_stable_out_strict"])
         assert lat.kinetic_energy(gpu.STATE, ordered=True) == orc.kinetic_energy(19, got)
+
+
+@pytest.mark.parametrize("vec,layout,q,L", [(1, 1, 9, 96), (2, 1, 9, 96), (1, 0, 9, 64), (1, 1, 19, 16), (2, 1, 19, 16),
+                                            (1, 1, 9, 2048)])
+def test_vectorised_kernel_bitwise(gpu, orc, monkeypatch, vec, layout, q, L):
+    """The 16-byte vectorised periodic kernel (shuffle or edge-load variant)
+    gives the reference's populations bit for bit, D2Q9 and D3Q19."""
+    monkeypatch.setenv("ERLBM_VEC", str(vec))
+    monkeypatch.setenv("ERLBM_LAYOUT", str(layout))
+    n = 40 if L < 1000 else 3
+    if q == 9:
+        tau = 0.5 + 3. * (1. / 32.) * L / 80000.
+        f0 = gpu.init_double_shear(L)
+        tnh = 0.65
+    else:
+        tau, tnh = 0.5003, 0.65
+        f0 = gpu.init_tg_map(L, 0.3)
+    ini, st, nf = orc.stream_map(q, L, L, 1)
+    want = orc.run_steps(q, f0, st, n, O.COL_EXT_REG, tau, tnh, nthreads=8)
+    with _periodic(gpu, q, L, tau, tnh, O.COL_EXT_REG, f0) as lat:
+        lat.step(n)
+        assert np.array_equal(lat.download(), want)

@@ -1,0 +1,105 @@
+"""Shifted low-precision storage of the plain lattice path (FP32 / FP16
+populations f_i - w_i rho0, FP32 arithmetic, FP64 reductions) against the
+FP64 oracle.  Tolerances are stated per observable, as the error budget of
+the storage format, not bitwise."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _vel(f, q=9):
+    c = np.array([[0, 0], [1, 0], [-1, 0], [0, 1], [0, -1], [1, 1], [-1, -1], [1, -1], [-1, 1]], float)
+    rho = f.sum(1)
+    return (f @ c) / rho[:, None], rho
+
+
+def _l2(u, ref):
+    return float(np.sqrt(np.sum((u - ref) ** 2) / np.sum(ref ** 2)))
+
+
+@pytest.mark.parametrize("prec,tol", [("FP32_SHIFTED", 1e-4), ("FP32_SHIFTED_F32", 1e-4), ("FP16_SHIFTED", 5e-2)])
+def test_double_shear_shifted_vs_fp64(gpu, orc, prec, tol):
+    """D2Q9 double shear layer L=256, 1000 steps: relative L2 velocity error
+    against the FP64 oracle below tol (FP32 storage: 1e-4, FP16: 5e-2; FP32
+    storage with FP32 arithmetic measured 3.2e-5)."""
+    L, n = 256, 1000
+    tau = 0.5 + 3. * (1. / 32.) * L / 80000.
+    f0 = gpu.init_double_shear(L)
+    ini, st, nf = orc.stream_map(9, L, L, 1)
+    want = orc.run_steps(9, f0, st, n, O.COL_EXT_REG, tau, 0.65, nthreads=8)
+    lat = gpu.Lattice(9, L, L, 1, precision=getattr(gpu, prec))
+    with lat:
+        lat.set_params(tau, 0.65, 1.0, O.COL_EXT_REG)
+        lat.upload(f0)
+        lat.step(n)
+        got = lat.download()
+        assert lat.status() == (n, 0)
+    u, rho = _vel(got)
+    uw, rhow = _vel(want)
+    err = _l2(u, uw)
+    print(f"{prec}: DSL L={L} {n} steps, L2 velocity error vs FP64 {err:.3e}, "
+          f"mass drift {abs(rho.sum() - rhow.sum()) / rhow.sum():.2e}", flush=True)
+    assert np.isfinite(got).all()
+    assert err < tol
+    assert abs(rho.sum() - rhow.sum()) / rhow.sum() < (1e-7 if prec.startswith("FP32") else 1e-3)
+
+
+@pytest.mark.parametrize("prec,tol", [("FP32_SHIFTED", 0.01), ("FP16_SHIFTED", None), ("FP32_SHIFTED_F32", None)])
+def test_taylor_green_shifted_error(gpu, orc, prec, tol):
+    """Config: TG D2Q9 L=64, E-RLBM 0.65, t=T.  FP32-shifted storage (FP64
+    arithmetic) keeps E_vector within 1% of the FP64 value 0.000298896; the
+    FP16 and FP32-arithmetic variants are measured and reported."""
+    L = 64
+    u0 = 2.0 / L
+    visc = u0 * L / 50.0
+    tau = 0.5 + 3.0 * visc
+    nT = int(round(L / u0))
+    f0 = gpu.init_taylor_green(9, L, u0, tau)
+    lat = gpu.Lattice(9, L, L, 1, precision=getattr(gpu, prec))
+    with lat:
+        lat.set_params(tau, 0.65, 1.0, O.COL_EXT_REG)
+        lat.upload(f0)
+        lat.step(nT + 1)
+        post = lat.download(gpu.POST_COLLISION)
+    ev, em, _ = orc.tg_error_d2q9(post, L, u0, visc, nT)
+    rel = abs(ev - 0.000298896) / 0.000298896
+    print(f"{prec} TG L=64 E_vector {ev:.6g} (FP64 0.000298896, {100 * rel:.2f}% off)", flush=True)
+    assert np.isfinite(ev)
+    if tol is not None:
+        assert rel < tol
+
+
+def test_obstacle_fp32_runs_like_fp64(gpu, golden):
+    g = golden("ref_trajectories.npz")
+    tau, tnh, nst = g["obst_stable_params"]
+    lat = gpu.Lattice(19, 22, 12, 1, mask=gpu.obstacle_mask(), precision=gpu.FP32_SHIFTED)
+    with lat:
+        lat.set_params(tau, tnh, 1.0, O.COL_EXT_REG)
+        lat.set_zouhe_inlet(1, 0.1 / np.sqrt(3.0))
+        lat.set_outlet_rescale(20, 1.0)
+        lat.set_density_check(10.0)
+        lat.upload(g["obst_in"])
+        lat.step(int(nst))
+        got = lat.download()
+        assert lat.status() == (int(nst), 0)
+    ref = g["obst_stable_out_strict"]
+    rel = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print(f"obstacle FP32 vs FP64 after {int(nst)} steps: max abs diff / max {rel:.2e}", flush=True)
+    assert rel < 1e-4
+
+
+def test_shifted_energy_monitor_fp64(gpu, orc):
+    """Monitors of shifted storage reduce in FP64: the energy of an uploaded
+    state equals the FP64 energy of its rounded populations."""
+    f0 = gpu.init_tg_map(16, 0.3)
+    lat = gpu.Lattice(19, 16, 16, 1, precision=gpu.FP32_SHIFTED)
+    with lat:
+        lat.set_params(0.6, 0.65, 1.0, O.COL_EXT_REG)
+        lat.upload(f0)
+        back = lat.download()
+        e = lat.kinetic_energy(gpu.STATE, ordered=True)
+    assert np.max(np.abs(back - f0)) < 1e-8
+    assert e == orc.kinetic_energy(19, back)
