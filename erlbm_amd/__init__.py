@@ -11,12 +11,12 @@ from ._lib import (COL_BGK, COL_EXT_REG, COL_MRT, COL_REG, FP16_SHIFTED, FP32_SH
 from ._lib import FP_AUTO, INIT_REST, INIT_TAYLOR_GREEN, JOB_DTYPE, RESULT_DTYPE
 from .maps import Batch, obstacle_batch, obstacle_map_jobs, run_obstacle_map, run_tg_map, tg_batch, tg_map_jobs
 from .lattice import (Lattice, dist_eq, init_double_shear, init_taylor_green, init_tg_map, obstacle_map_point_params,
-                      obstacle_mask, tg_map_point_params)
+                      obstacle_mask, taylor_green_error, tg_map_point_params)
 
 __all__ = [
     "COL_BGK", "COL_MRT", "COL_REG", "COL_EXT_REG", "FP64", "FP32_SHIFTED", "FP16_SHIFTED", "FP32_SHIFTED_F32", "STATE",
     "POST_COLLISION", "STOP_DENSITY", "STOP_ENERGY", "ErlbmError", "device_count", "hbm_copy_bench", "lib", "Lattice", "dist_eq",
     "init_double_shear", "init_taylor_green", "init_tg_map", "obstacle_mask", "tg_map_point_params",
-    "obstacle_map_point_params", "FP_AUTO", "INIT_REST", "INIT_TAYLOR_GREEN", "JOB_DTYPE", "RESULT_DTYPE", "Batch",
+    "obstacle_map_point_params", "FP_AUTO", "INIT_REST", "INIT_TAYLOR_GREEN", "JOB_DTYPE", "RESULT_DTYPE", "Batch", "taylor_green_error",
     "obstacle_batch", "obstacle_map_jobs", "run_obstacle_map", "run_tg_map", "tg_batch", "tg_map_jobs",
 ]

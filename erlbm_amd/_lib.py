@@ -43,6 +43,20 @@ class JobResult(C.Structure):
                 ("e0", C.c_double), ("energy", C.c_double), ("margin", C.c_double)]
 
 
+class Request(C.Structure):
+    """erlbm_request (include/erlbm.h)."""
+    _fields_ = [("kind", C.c_int), ("i", C.c_int), ("j", C.c_int), ("n_i", C.c_int), ("n_j", C.c_int),
+                ("L", C.c_int), ("collision", C.c_int), ("precision", C.c_int), ("tau_nh", C.c_double),
+                ("t_final", C.c_double)]
+
+
+class Response(C.Structure):
+    """erlbm_response (include/erlbm.h)."""
+    _fields_ = [("id", C.c_longlong), ("kind", C.c_int), ("status", C.c_int), ("unstable", C.c_int),
+                ("precision", C.c_int), ("steps", C.c_longlong), ("value", C.c_double * 3),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
 JOB_DTYPE = np.dtype([("tau", "f8"), ("tau_nh", "f8"), ("u0", "f8"), ("rho0", "f8"), ("energy_factor", "f8"),
                       ("dens_limit", "f8"), ("n_steps", "i8"), ("check_every", "i4"), ("init", "i4"),
                       ("collision", "i4"), ("precision", "i4")])
@@ -77,6 +91,14 @@ SIGNATURES = {
     "erlbm_kinetic_energy": (_i, [_vp, _i, _i, C.POINTER(_d)]),
     "erlbm_density_monitor": (_i, [_vp, _i, _d, C.POINTER(_d), C.POINTER(_i)]),
     "erlbm_hbm_copy_bench": (_i, [_ll, _i, C.POINTER(C.c_float)]),
+    "erlbm_taylor_green_error": (_i, [_i, _i, _d, _d, _ll, _dp, _dp]),
+    "erlbm_server_create": (_i, [_ll, _i, _i, _vp, C.POINTER(_vp)]),
+    "erlbm_server_destroy": (_i, [_vp]),
+    "erlbm_server_submit": (_i, [_vp, C.POINTER(Request), C.POINTER(_ll)]),
+    "erlbm_server_pump": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
+    "erlbm_server_poll": (_i, [_vp, C.POINTER(Response), _i, C.POINTER(_i)]),
+    "erlbm_server_clock": (_i, [_vp, C.POINTER(_d)]),
+    "erlbm_server_memory": (_i, [_vp, C.POINTER(_ll), C.POINTER(_ll)]),
     "erlbm_batch_create": (_i, [_i, _i, _vp, _i, _i, _vp, C.POINTER(_vp)]),
     "erlbm_batch_destroy": (_i, [_vp]),
     "erlbm_batch_run": (_i, [_vp, _vp, _i, _vp]),

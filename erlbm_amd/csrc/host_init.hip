@@ -167,3 +167,55 @@ extern "C" int erlbm_obstacle_map_point_params(int i, int j, int n_tau, int n_nu
     if (n_steps) *n_steps = (int)((double)(200 * 12) / *u_imp);
     return ERLBM_OK;
 }
+
+// Analytic-decay error of a Taylor-Green run, src/main_taylor-green.cpp:178-213:
+// moments of the post-collision populations at step `step` against
+// u_ex = u_0 e^{-2 nu k^2 t} (sin kx cos ky, -cos kx sin ky).
+// out[0] = E_vector, out[1] = E_magnitude, out[2] = (nu_eff - nu) / nu.
+extern "C" int erlbm_taylor_green_error(int q, int Li, double u_0, double visc, long long step, const double *inif,
+                                        double *out)
+{
+    if (!inif || !out || Li < 1 || (q != 9 && q != 19)) return ERLBM_ERR_ARG;
+    const double L = Li;
+    const double k = 2. * M_PI / L;
+    double s_mag = 0, s_vec = 0, s_r = 0, s_proj = 0, s_phi = 0;
+    const double e = exp(-2. * visc * k * k * step);
+    const int *cx = q == 9 ? VSet<9>::cx : VSet<19>::cx;
+    const int *cy = q == 9 ? VSet<9>::cy : VSet<19>::cy;
+    const int *cz = q == 9 ? VSet<9>::cz : VSet<19>::cz;
+    for (int y = 0; y < Li; y++) {
+        for (int x = 0; x < Li; x++) {
+            // calcula (LBM_functions.cpp:3517-3589)
+            const double *f = inif + (size_t)(x + y * Li) * q;
+            double rho = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+            for (int i = 0; i < q; i++) {
+                rho = rho + f[i];
+                mx = mx + cx[i] * f[i];
+                my = my + cy[i] * f[i];
+                mz = mz + cz[i] * f[i];
+            }
+            double vx = 0., vy = 0., vz = 0.;
+            if (rho) {
+                const double oor = 1.0 / (rho);
+                vx = mx * oor;
+                vy = my * oor;
+                vz = q == 19 ? mz * oor : 0.0;
+            }
+            const double rx = u_0 * e * sin(k * x) * cos(k * y);
+            const double ry = -u_0 * e * cos(k * x) * sin(k * y);
+            const double us = sqrt(vx * vx + vy * vy + vz * vz), ur = sqrt(rx * rx + ry * ry);
+            s_mag += (us - ur) * (us - ur);
+            s_vec += (vx - rx) * (vx - rx) + (vy - ry) * (vy - ry) + vz * vz;
+            s_r += ur * ur;
+            const double px = sin(k * x) * cos(k * y), py = -cos(k * x) * sin(k * y);
+            s_proj += vx * px + vy * py;
+            s_phi += px * px + py * py;
+        }
+    }
+    const double A = s_proj / s_phi / u_0;
+    const double nu_ef = -log(A) / (2. * k * k * step);
+    out[0] = sqrt(s_vec / s_r);
+    out[1] = sqrt(s_mag / s_r);
+    out[2] = nu_ef / visc - 1.;
+    return ERLBM_OK;
+}

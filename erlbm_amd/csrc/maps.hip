@@ -56,10 +56,13 @@ struct TinyGeom {
 struct RunArgs {
     TinyGeom g;
     const MapJob *jobs;
-    MapResult *results;
+    MapResult *results;          // in/out: a job with 0 < steps and !done resumes
     const int *job_idx;          // jobs of this launch
     int n_jobs;
+    const int *n_jobs_dev;       // if set, the job count is read from device memory
     unsigned int *next_job;      // queue head
+    int max_steps;               // steps per job in this launch (<= 0: to retirement)
+    void *saved;                 // per job state between launches, Q * nvp values each
 };
 
 // ---- storage policies -------------------------------------------------------
@@ -295,29 +298,40 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
     // used between a completed gather and the next write)
     double *scratch = reinterpret_cast<double *>(sf);
 
-    // ---- initial state and E0
+    // ---- initial state and E0, or the state saved by the previous launch
     T f[Q];
-    if (active) {
-        double ux = 0.0, uy = 0.0;
-        if (J.init == INIT_TAYLOR_GREEN) {
-            ux = J.u0 * a.g.sin_tab[my_x] * a.g.cos_tab[my_y];
-            uy = -J.u0 * a.g.cos_tab[my_x] * a.g.sin_tab[my_y];
-        }
-        P::init(f, rho0, ux, uy);
-    }
+    const MapResult prev = *res;
+    const bool resume = a.saved != nullptr && prev.steps > 0 && !prev.done;
+    T *slot = a.saved ? static_cast<T *>(a.saved) + (size_t)(res - a.results) * Q * nvp : nullptr;
     double e0 = 0.0;
-    if (J.check_every > 0) e0 = ordered_energy<P>(f, active, rho0, scratch, nf, bcast);
-    __syncthreads();
-    if (active)
-        for (int i = 0; i < Q; i++) sf[i * nvp + my_v] = f[i];
+    if (resume) {
+        e0 = prev.e0;
+        if (active)
+            for (int i = 0; i < Q; i++) sf[i * nvp + my_v] = slot[i * nvp + my_v];
+    } else {
+        if (active) {
+            double ux = 0.0, uy = 0.0;
+            if (J.init == INIT_TAYLOR_GREEN) {
+                ux = J.u0 * a.g.sin_tab[my_x] * a.g.cos_tab[my_y];
+                uy = -J.u0 * a.g.cos_tab[my_x] * a.g.sin_tab[my_y];
+            }
+            P::init(f, rho0, ux, uy);
+        }
+        if (J.check_every > 0) e0 = ordered_energy<P>(f, active, rho0, scratch, nf, bcast);
+        __syncthreads();
+        if (active)
+            for (int i = 0; i < Q; i++) sf[i * nvp + my_v] = f[i];
+    }
     __syncthreads();
 
-    long long s = 0;
+    long long s = resume ? prev.steps : 0;
+    long long end = J.n_steps;
+    if (a.max_steps > 0 && s + a.max_steps < end) end = s + a.max_steps;
     int unstable = 0;
-    double e_last = e0;
+    double e_last = resume ? prev.energy : e0;
     double e_max = e0;      // largest energy seen by the energy test
     double r_max = 0.0;     // largest |rho| seen by this thread's density test
-    for (; s < J.n_steps; s++) {
+    for (; s < end; s++) {
         const bool stream = s > 0;
         double fator = 0.0;   // outlet column: rho0 / rho(upstream neighbour)
         if (active) {
@@ -378,6 +392,9 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
             break;
         }
     }
+    const bool finished = unstable || s >= J.n_steps;
+    if (!finished && slot && active)
+        for (int i = 0; i < Q; i++) slot[i * nvp + my_v] = sf[i * nvp + my_v];
     double margin = 0.0;
     if (J.dens_limit > 0.0) {
         // block max of the per-thread |rho| maxima (NaN propagates)
@@ -394,12 +411,16 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
     } else if (J.check_every > 0) {
         margin = e_max / e0;
     }
+    if (resume) {   // running maximum over launches; a non-finite value sticks
+        if (!isfinite(prev.margin)) margin = prev.margin;
+        else if (isfinite(margin) && prev.margin > margin) margin = prev.margin;
+    }
     if (tid == 0) {
         MapResult r;
         r.margin = margin;
         r.unstable = unstable;
         r.steps = (int)s;
-        r.done = 1;
+        r.done = finished ? 1 : 0;
         r.precision = P::code;
         r.e0 = e0;
         r.energy = e_last;
@@ -415,12 +436,13 @@ __global__ __launch_bounds__(TPB) void k_tiny(RunArgs a)
     __shared__ double bcast[2];
     __shared__ int sjob;
     typename P::T *sf = reinterpret_cast<typename P::T *>(lds_raw);
+    const int n_jobs = a.n_jobs_dev ? *a.n_jobs_dev : a.n_jobs;
     for (;;) {
         if (threadIdx.x == 0) sjob = (int)atomicAdd(a.next_job, 1u);
         __syncthreads();
         const int k = sjob;
         __syncthreads();
-        if (k >= a.n_jobs) return;
+        if (k >= n_jobs) return;
         const int j = a.job_idx[k];
         const MapJob J = a.jobs[j];
         run_job<P, COL, GENERAL>(a, J, &a.results[j], sf, bcast);
@@ -462,6 +484,7 @@ int launch_group(erlbm_batch *b, const RunArgs &a, unsigned int *counter)
     if (per_cu < 1) per_cu = 1;
     int grid = b->n_cu * per_cu;
     if (grid > a.n_jobs) grid = a.n_jobs;
+    if (grid < 1) return ERLBM_OK;
     RunArgs ra = a;
     ra.next_job = counter;
     hipLaunchKernelGGL((k_tiny<P, COL, GENERAL>), dim3(grid), dim3(TPB), lds, b->stream, ra);
@@ -489,6 +512,70 @@ int resolve_precision(const MapJob &j)
 }
 
 }  // namespace
+
+namespace {
+
+void fill_geom(const erlbm_batch *b, TinyGeom &g)
+{
+    g.nx = b->nx;
+    g.ny = b->ny;
+    g.n_fluid = b->n_fluid;
+    g.nvp = (b->nx * b->ny + 7) / 8 * 8;
+    g.general = b->general ? 1 : 0;
+    g.inlet_x = b->inlet_x;
+    g.outlet_x = b->outlet_x;
+    g.outlet_dx = b->outlet_dx;
+    g.vox_of_fluid = b->vox_of_fluid;
+    g.fluid_of_vox = b->fluid_of_vox;
+    g.nbr = b->nbr;
+    g.sin_tab = b->tabs;
+    g.cos_tab = b->tabs + b->tab_n;
+}
+
+__global__ void k_zero_counter(unsigned int *c)
+{
+    if (threadIdx.x == 0) *c = 0u;
+}
+
+}  // namespace
+
+namespace erlbm {
+namespace tiny {
+
+size_t state_bytes(const erlbm_batch *b, int precision)
+{
+    const size_t nvp = (size_t)(b->nx * b->ny + 7) / 8 * 8;
+    return (size_t)Q * nvp * (precision == ERLBM_FP64 ? sizeof(double) : sizeof(float));
+}
+
+int fluid_sites(const erlbm_batch *b) { return b->n_fluid; }
+
+int launch(erlbm_batch *b, int precision, int collision, const MapJob *jobs, MapResult *results, const int *job_idx,
+           const int *n_jobs_dev, int n_jobs_max, unsigned int *counter, int max_steps, void *saved,
+           hipStream_t stream)
+{
+    RunArgs a;
+    fill_geom(b, a.g);
+    a.jobs = jobs;
+    a.results = results;
+    a.job_idx = job_idx;
+    a.n_jobs = n_jobs_max;
+    a.n_jobs_dev = n_jobs_dev;
+    a.next_job = counter;
+    a.max_steps = max_steps;
+    a.saved = saved;
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, stream, counter);
+    const hipStream_t keep = b->stream;
+    b->stream = stream;
+    int rc = (precision == ERLBM_FP64)            ? launch_col<F64>(b, collision, a, counter)
+             : (precision == ERLBM_FP32_SHIFTED) ? launch_col<F32>(b, collision, a, counter)
+                                                 : launch_col<F32A>(b, collision, a, counter);
+    b->stream = keep;
+    return rc;
+}
+
+}  // namespace tiny
+}  // namespace erlbm
 
 extern "C" int erlbm_batch_create(int nx, int ny, const int *mask, int inlet_x, int outlet_x, void *stream,
                                   erlbm_batch **out)
@@ -639,22 +726,14 @@ extern "C" int erlbm_batch_run(erlbm_batch *b, const erlbm_job *jobs, int n_jobs
     ERLBM_HIP_TRY(hipMemcpyAsync(b->d_jobs, jobs, sizeof(MapJob) * n_jobs, hipMemcpyHostToDevice, b->stream));
     ERLBM_HIP_TRY(hipMemcpyAsync(b->d_idx, idx.data(), sizeof(int) * n_jobs, hipMemcpyHostToDevice, b->stream));
     ERLBM_HIP_TRY(hipMemsetAsync(b->counters, 0, sizeof(unsigned int) * 16, b->stream));
+    ERLBM_HIP_TRY(hipMemsetAsync(b->d_results, 0, sizeof(MapResult) * n_jobs, b->stream));
     RunArgs a;
-    a.g.nx = b->nx;
-    a.g.ny = b->ny;
-    a.g.n_fluid = b->n_fluid;
-    a.g.nvp = (b->nx * b->ny + 7) / 8 * 8;
-    a.g.general = b->general ? 1 : 0;
-    a.g.inlet_x = b->inlet_x;
-    a.g.outlet_x = b->outlet_x;
-    a.g.outlet_dx = b->outlet_dx;
-    a.g.vox_of_fluid = b->vox_of_fluid;
-    a.g.fluid_of_vox = b->fluid_of_vox;
-    a.g.nbr = b->nbr;
-    a.g.sin_tab = b->tabs;
-    a.g.cos_tab = b->tabs + b->tab_n;
+    fill_geom(b, a.g);
     a.jobs = b->d_jobs;
     a.results = b->d_results;
+    a.n_jobs_dev = nullptr;
+    a.max_steps = 0;
+    a.saved = nullptr;
     int start = 0, group = 0;
     while (start < n_jobs) {
         int end = start;

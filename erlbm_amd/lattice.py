@@ -159,3 +159,12 @@ def obstacle_map_point_params(i, j, n_tau=400, n_nu=400, ma=0.1):
     check(lib().erlbm_obstacle_map_point_params(i, j, n_tau, n_nu, ma, C.byref(u), C.byref(t), C.byref(tn),
                                                 C.byref(n)), "erlbm_obstacle_map_point_params")
     return u.value, t.value, tn.value, n.value
+
+
+def taylor_green_error(f_post: np.ndarray, L_: int, u0: float, visc: float, step: int) -> np.ndarray:
+    """[E_vector, E_magnitude, (nu_eff - nu)/nu] of post-collision populations."""
+    q = f_post.shape[-1]
+    out = np.zeros(3)
+    check(lib().erlbm_taylor_green_error(q, L_, u0, visc, step, np.ascontiguousarray(f_post, np.float64).reshape(-1),
+                                         out), "erlbm_taylor_green_error")
+    return out

@@ -151,6 +151,11 @@ int erlbm_tg_map_point_params(int i, int j, int n_ma, int n_nu, int L, int colli
                               double *u_0, double *tau, double *tau_nh_out, int *n_steps);
 int erlbm_obstacle_map_point_params(int i, int j, int n_tau, int n_nu, double Ma, double *u_imp, double *tau,
                                     double *tau_nh, int *n_steps);
+/* Taylor-Green analytic-decay error (src/main_taylor-green.cpp:178-213) of
+ * post-collision populations at `step`: out = {E_vector, E_magnitude,
+ * (nu_eff - nu) / nu}.                                                     */
+int erlbm_taylor_green_error(int q, int L, double u_0, double visc, long long step, const double *inif,
+                             double *out);
 
 /* ---- batched tiny lattices (stability maps, small requests) ------------------
  * Replaces the map drivers' `omp parallel for` over map points, each calling
@@ -214,6 +219,61 @@ int erlbm_tg_map_jobs(int n_ma, int n_nu, int collision, double tau_nh, int prec
                       int j1, erlbm_job *jobs);
 int erlbm_obstacle_map_jobs(int n_tau, int n_nu, double Ma, int precision, int i0, int i1, int j0, int j1,
                             erlbm_job *jobs);
+
+/* ---- serving: continuous batching of uneven solves ----------------------------
+ * No single counterpart in the reference, whose drivers run one solve per
+ * process (scripts/run_*.sh): the four drivers' solves as requests to one
+ * device-resident loop.  Map points (src/main_stability-map_*.cpp:41-123)
+ * share persistent LDS-resident kernels in slot pools, one graph-replayed
+ * chunk of `chunk_steps` steps per round; D2Q9 runs
+ * (src/main_taylor-green.cpp, src/main_double-shear-layer.cpp) are plain
+ * lattices on their own streams, one time slice per round.  Admission by
+ * HBM budget (bytes of resident request state), first fit in arrival order;
+ * each request stays resident until its stop test or budget retires it.    */
+enum {
+    ERLBM_REQ_TG_MAP_POINT = 1,        /* roda_ponto of the Taylor-Green map at (i, j) of n_i x n_j */
+    ERLBM_REQ_OBSTACLE_MAP_POINT = 2,  /* roda_ponto of the obstacle map at (i, j) of n_i x n_j (Ma 0.1) */
+    ERLBM_REQ_TAYLOR_GREEN = 3,        /* D2Q9 Taylor-Green, Fig. 3 mode (Re 50, U0 = 2/L) at L, to t = T */
+    ERLBM_REQ_DOUBLE_SHEAR = 4         /* D2Q9 double shear layer (Re 80000) at L, to t* = t_final */
+};
+
+typedef struct {
+    int kind;              /* ERLBM_REQ_* */
+    int i, j, n_i, n_j;    /* map point and grid (map requests) */
+    int L;                 /* lattice size (D2Q9 runs) */
+    int collision;         /* ERLBM_COL_* (the obstacle map is E-RLBM) */
+    int precision;         /* ERLBM_FP64 / ERLBM_FP32_SHIFTED / ERLBM_FP_AUTO / ... */
+    double tau_nh;         /* E-RLBM tau_nh (Taylor-Green map, D2Q9 runs) */
+    double t_final;        /* double shear layer: t* = step U0 / L at the end */
+} erlbm_request;
+
+typedef struct {
+    long long id;
+    int kind, status;      /* status: ERLBM_OK, or the error that rejected the request */
+    int unstable;          /* map points: 1 = unstable; D2Q9 runs: 1 = stopped by a stop test */
+    int precision;         /* storage precision it ran in */
+    long long steps;       /* steps run */
+    double value[3];       /* map: {margin, E0, last E}; Taylor-Green: {E_vector, E_magnitude,
+                              (nu_eff - nu)/nu}; double shear: {kinetic energy, 0, 0} */
+    double t_submit, t_admit, t_done;   /* seconds on the server clock */
+} erlbm_response;
+
+typedef struct erlbm_server erlbm_server;
+
+/* stream: the pools' HIP stream (NULL: the server makes one); tiny_slots:
+ * slots per pool (a pool reserves them from the budget on first use).    */
+int erlbm_server_create(long long hbm_budget_bytes, int chunk_steps, int tiny_slots, void *stream,
+                        erlbm_server **out);
+int erlbm_server_destroy(erlbm_server *s);
+/* Queue a request (never blocks); *id identifies its response. */
+int erlbm_server_submit(erlbm_server *s, const erlbm_request *req, long long *id);
+/* One scheduling round: admit what fits, run one chunk / slice of every
+ * resident request, retire finished ones.  Synchronous.                  */
+int erlbm_server_pump(erlbm_server *s, int *in_flight, int *queued);
+/* Move up to max_out finished responses to out (completion order). */
+int erlbm_server_poll(erlbm_server *s, erlbm_response *out, int max_out, int *n_out);
+int erlbm_server_clock(erlbm_server *s, double *t);
+int erlbm_server_memory(erlbm_server *s, long long *used, long long *budget);
 
 /* ---- measurement -------------------------------------------------------------
  * No counterpart in the reference: a plain device-to-device copy of `bytes`
