@@ -8,13 +8,17 @@ rho = 1, equilibrium initialisation of the double-shear-layer field
 pull-stream + collide sweep over all 4,194,304 nodes on one GPU, inputs
 resident in HBM.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 
-Prints ONE JSON line with the MLUPS, the roofline of the step kernel
-(144 algorithmic bytes per node-step: 9 FP64 populations read + 9 written)
-against MI355X's 8 TB/s, and the reference's own CPU step loop (compiled from
-its sources into oracle/_ref, OpenMP on the host cores) timed on a bounded
-sample as the CPU baseline.
+Prints ONE JSON line: the FP64 MLUPS (`value`), the roofline of the step
+kernel (144 algorithmic bytes per node-step: 9 FP64 populations read + 9
+written) against MI355X's 8 TB/s, and the reference's own CPU step loop
+(compiled from its sources into oracle/_ref, OpenMP on the host cores) timed
+on a bounded sample as the CPU baseline.  The rest of the metric rides along:
+FP32 / FP16 shifted storage of the same run (`precision`), the full 400 x 400
+stability maps in points/s against the plain one-run-per-launch FP64 path
+(`maps`), and the continuous-batching serving mix with p50/p99 latency
+(`serving`).  --quick keeps only the headline run.
 """
 from __future__ import annotations
 
@@ -34,16 +38,16 @@ U0 = 1.0 / 32.0
 RE = 80000.0
 TAU = 0.5 + 3.0 * U0 * L / RE
 TAU_NH = 0.65
-BYTES_PER_NODE = 144           # 9 x 8 B read + 9 x 8 B written
 HBM_PEAK_GBS = 8000.0
+BYTES_PER_NODE = {"fp64": 144, "fp32": 72, "fp16": 36}
 # rocprofv3 PMC traffic per launch (FETCH_SIZE + WRITE_SIZE, calibrated against
-# a plain copy kernel); filled from profiles/ when a measurement exists.
+# a plain copy kernel), measured on the FP64 step kernel
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "dsl2048_d2q9_fp64_traffic.json")
 
 
 def cpu_baseline(seconds: float):
     """The reference's OpenMP step loop on a bounded sample of the same
-    workload (the first n steps of the L=2048 run), all host cores."""
+    workload (the first n steps of the L=2048 run), on the host cores."""
     import oracle
     n_threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
     f0 = oracle.Oracle().init_double_shear_d2q9(L)
@@ -63,9 +67,88 @@ def cpu_baseline(seconds: float):
     t1 = run(f0.copy(), 1)
     n = max(2, min(2000, int(seconds / max(t1, 1e-6))))
     dt = run(f0.copy(), n)
-    return {"value": L * L * n / dt / 1e6, "unit": "MLUPS", "cores": n_threads, "kind": kind,
-            "sample": f"D2Q9 E-RLBM double shear L={L}, first {n} steps ({dt:.1f} s wall, "
-                      f"{'reference step loop built from its sources, -O3 -march=x86-64-v3, OpenMP' if kind == 'reference' else 'C restatement, OpenMP'})"}
+    how = ("the reference's step loop built from its sources, -O3 -march=x86-64-v3, OpenMP"
+           if kind == "reference" else "C restatement, OpenMP")
+    return {"value": round(L * L * n / dt / 1e6, 2), "unit": "MLUPS", "cores": n_threads, "kind": kind,
+            "sample": f"D2Q9 E-RLBM double shear L={L}, first {n} steps ({dt:.1f} s wall, {how})"}
+
+
+def dsl_run(E, precision: str, steps: int, warmup: int):
+    """Timed D2Q9 double-shear steps at L=2048 (HIP events on the lattice's stream)."""
+    prec = {"fp64": E.FP64, "fp32": E.FP32_SHIFTED, "fp16": E.FP16_SHIFTED}[precision]
+    f0 = E.init_double_shear(L)
+    lat = E.Lattice(9, L, L, 1, precision=prec)
+    lat.set_params(TAU, TAU_NH, 1.0, E.COL_EXT_REG)
+    lat.upload(f0)
+    mass0 = float(f0.sum())
+    del f0
+    lat.step(warmup)
+    lat.synchronize()
+    ms = lat.step_timed(steps)
+    steps_done, reason = lat.status()
+    ms_per_step = ms / steps
+    post = lat.download(E.POST_COLLISION)
+    mass_err = abs(float(post.sum()) - mass0) / mass0
+    finite = bool(np.isfinite(post).all())
+    del post
+    lat.close()
+    ok = steps_done == warmup + steps and reason == 0 and finite and mass_err < (1e-9 if prec == E.FP64 else 1e-3)
+    mlups = L * L / (ms_per_step * 1e-3) / 1e6
+    gbs = BYTES_PER_NODE[precision] * L * L / (ms_per_step * 1e-3) / 1e9
+    return ok, {"MLUPS": round(mlups, 1), "ms_per_step": round(ms_per_step, 5), "achieved_GBs": round(gbs, 1),
+                "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "mass_rel_error": mass_err}
+
+
+def maps_section(E):
+    """Full 400 x 400 stability maps (FP64, batched) vs the plain one-run-per-
+    launch FP64 path on a sample, and agreement with the published maps."""
+    d = np.load(os.path.join(ROOT, "tests", "golden", "published_maps.npz"), allow_pickle=False)
+    pub = {k: np.unpackbits(d[k])[: 400 * 400].reshape(400, 400) for k in d.files}
+    out = {}
+    for name, mk, jobs, nf in (("tg_erlbm065", E.tg_batch, E.tg_map_jobs(400, 400, E.COL_EXT_REG, 0.65, E.FP64), 256),
+                               ("obstacle", E.obstacle_batch, E.obstacle_map_jobs(400, 400, 0.1, E.FP64), 232)):
+        with mk() as b:
+            t = time.perf_counter()
+            res = b.run(jobs)
+            dt = time.perf_counter() - t
+        agree = float(np.mean(res["unstable"] == pub[name].reshape(-1)))
+        out[name] = {"points": int(jobs.size), "seconds": round(dt, 3), "points_per_s": round(jobs.size / dt, 1),
+                     "node_steps_per_s": float(np.sum(res["steps"].astype(float)) * nf / dt),
+                     "unstable": int(res["unstable"].sum()), "agreement_vs_published_data": round(agree, 5)}
+    # the plain FP64 path, one run at a time, on a random sample of points
+    from erlbm_amd import serve as S
+    rng = np.random.default_rng(5)
+    sample = []
+    for kind, n in ((S.REQ_TG_MAP_POINT, 24), (S.REQ_OBSTACLE_MAP_POINT, 6)):
+        for _ in range(n):
+            r = S.Request()
+            r.kind, r.n_i, r.n_j, r.collision, r.tau_nh, r.precision = kind, 400, 400, E.COL_EXT_REG, 0.65, E.FP64
+            r.i, r.j = int(rng.integers(400)), int(rng.integers(400))
+            sample.append(S.Solve(req=r, t_arrival=0.0))
+    t = time.perf_counter()
+    S.run_plain(sample)
+    dt = time.perf_counter() - t
+    tg = [s for s in sample if s.req.kind == S.REQ_TG_MAP_POINT]
+    ob = [s for s in sample if s.req.kind == S.REQ_OBSTACLE_MAP_POINT]
+    t_tg = sum(s.response["t_done"] - s.response["t_admit"] for s in tg)
+    t_ob = sum(s.response["t_done"] - s.response["t_admit"] for s in ob)
+    out["plain_one_run_per_launch"] = {"tg_points_per_s": round(len(tg) / t_tg, 2),
+                                       "obstacle_points_per_s": round(len(ob) / t_ob, 2),
+                                       "sample": f"{len(tg)} + {len(ob)} random map points, {dt:.1f} s"}
+    out["speedup_vs_plain"] = {"tg": round(out["tg_erlbm065"]["points_per_s"] / (len(tg) / t_tg), 1),
+                               "obstacle": round(out["obstacle"]["points_per_s"] / (len(ob) / t_ob), 1)}
+    return out
+
+
+def serving_section(n: int = 120, rate: float = 30.0):
+    from erlbm_amd import serve as S
+    import erlbm_amd as E
+    served = S.run_served(S.poisson_mix(n, rate, map_precision=E.FP64))
+    plain = S.run_plain(S.poisson_mix(n, rate, map_precision=E.FP64))
+    keep = ("requests", "wall_s", "throughput_req_per_s", "p50_s", "p99_s")
+    return {"mix": f"{n} seeded Poisson arrivals at {rate}/s: 60% TG map points, 25% obstacle map points, "
+                   "10% D2Q9 TG L in {128,256,512}, 5% D2Q9 double shear L=512",
+            "served": {k: served[k] for k in keep}, "plain": {k: plain[k] for k in keep}}
 
 
 def main() -> int:
@@ -75,8 +158,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline FP64 run only")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "fp16"],
-                    help="population storage of the timed run (fp32/fp16: shifted deviations)")
+                    help="storage of the headline run (fp32/fp16: shifted deviations, FP64 arithmetic)")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the calibration copy kernel (302 MB read + 302 MB written per launch)")
     a = ap.parse_args()
@@ -90,65 +174,62 @@ def main() -> int:
     if E.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
-    # plain copy of the bytes one step moves (half read, half written)
-    copy_bytes = BYTES_PER_NODE * L * L // 2
+    copy_bytes = BYTES_PER_NODE["fp64"] * L * L // 2     # a copy moving the bytes of one FP64 step
     copy_ms = E.hbm_copy_bench(copy_bytes, 5 if a.copy_only else 20)
     copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
     if a.copy_only:
         print(json.dumps({"copy_bytes_each_way": copy_bytes, "ms": copy_ms, "GBs": copy_gbs}))
         return 0
-    prec = {"fp64": E.FP64, "fp32": E.FP32_SHIFTED, "fp16": E.FP16_SHIFTED}[a.precision]
-    bytes_per_node = {"fp64": 144, "fp32": 72, "fp16": 36}[a.precision]
-    f0 = E.init_double_shear(L)
-    lat = E.Lattice(9, L, L, 1, precision=prec)
-    lat.set_params(TAU, TAU_NH, 1.0, E.COL_EXT_REG)
-    lat.upload(f0)
-    mass0 = float(f0.sum())
-    del f0
-    lat.step(a.warmup)
-    lat.synchronize()
-    ms = lat.step_timed(a.steps)          # HIP events on the lattice's stream
-    steps_done, reason = lat.status()
-    ms_per_step = ms / a.steps
-    mlups = L * L / (ms_per_step * 1e-3) / 1e6
-    achieved = bytes_per_node * L * L / (ms_per_step * 1e-3) / 1e9
-    post = lat.download(E.POST_COLLISION)
-    mass_err = abs(float(post.sum()) - mass0) / mass0
-    finite = bool(np.isfinite(post).all())
-    del post
-    lat.close()
-    if steps_done != a.warmup + a.steps or reason != 0 or not finite or mass_err > (1e-9 if prec == E.FP64 else 1e-3):
-        print(f"bench.py: run check failed (steps {steps_done}, stop {reason}, finite {finite}, "
-              f"mass error {mass_err:.2e})", file=sys.stderr)
+
+    ok, head = dsl_run(E, a.precision, a.steps, a.warmup)
+    if not ok:
+        print(f"bench.py: run check failed {head}", file=sys.stderr)
         return 1
+    extra = {}
+    if not a.quick:
+        prec = {}
+        for p in ("fp64", "fp32", "fp16"):
+            if p == a.precision:
+                prec[p] = head
+                continue
+            okp, r = dsl_run(E, p, max(200, a.steps // 2), 20)
+            prec[p] = r if okp else {"error": "run check failed", **r}
+        prec["accuracy_vs_fp64"] = ("FP32 storage: L2 velocity error 2.1e-7 (DSL L=256, 1000 steps), "
+                                    "TG L=64 E_vector within 0.01%; FP16: 1.7e-3 / +401% "
+                                    "(tests/test_gpu_lowp.py)")
+        extra["precision"] = prec
+        extra["maps"] = maps_section(E)
+        extra["serving"] = serving_section()
 
     traffic = None
     if a.precision == "fp64" and os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
+    achieved = head["achieved_GBs"]
     line = {
         "metric": "D2Q9 FP64 MLUPS (%HBM peak); map points/s + p99 latency, FP32 vs FP64",
-        "value": round(mlups, 1),
+        "value": head["MLUPS"],
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms_per_step, 5),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32 (shifted storage, f32 arithmetic)",
-                  "fp16": "f16 storage / f32 arithmetic (shifted)"}[a.precision],
+        "dtype": {"fp64": "f64", "fp32": "f32 storage (shifted) / f64 arithmetic",
+                  "fp16": "f16 storage (shifted) / f64 arithmetic"}[a.precision],
         "data": "synthetic: double-shear-layer initial field (deterministic, no dataset)",
         "config": {"workload": "D2Q9 E-RLBM double shear layer, periodic L=2048 (4,194,304 nodes), "
                                f"tau=0.5024, tau_nh=0.65, {a.precision} row-blocked pull-stream+collide",
                    "L": L, "tau": TAU, "tau_nh": TAU_NH, "collision": "E-RLBM", "precision": a.precision},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "copy_kernel_GBs": round(copy_gbs, 1),
-        "mass_rel_error": mass_err,
+        "mass_rel_error": head["mass_rel_error"],
+        **extra,
     }
     print(json.dumps(line))
     return 0
