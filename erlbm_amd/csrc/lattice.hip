@@ -142,10 +142,13 @@ __global__ __launch_bounds__(BLOCK) void k_step_general(StepArgs a)
 
 // Periodic all-fluid kernel: 2-D grid (x chunks of BS*NPT nodes, rows).
 // NPT nodes per thread (stride BS, so every load stays coalesced) raise the
-// loads in flight per wave; NTL selects nontemporal loads.  The stop test
+// loads in flight per wave.  NTL 1 makes every load nontemporal; NTL 2 only
+// the populations with c_x = 0, whose cache lines no other workgroup reads,
+// so that the lines an x-shifted population shares with the neighbouring
+// x chunk stay cached for that chunk's workgroup.  The stop test
 // reads the launch-time value of the stop word after the loads are issued,
 // so it costs no extra memory round trip.
-template <int Q, int COL, bool STREAM, int NPT, bool NTL, int BS>
+template <int Q, int COL, bool STREAM, int NPT, int NTL, int BS>
 __global__ __launch_bounds__(BS) void k_step_periodic(StepArgs a, int remap)
 {
     using V = VSet<Q>;
@@ -175,7 +178,9 @@ __global__ __launch_bounds__(BS) void k_step_periodic(StepArgs a, int remap)
                     src = at(a.g, i, ys + (long long)a.g.ny * zs, xs);
                 }
                 const double *p = &a.src[src];
-                if constexpr (NTL) f[k][i] = __builtin_nontemporal_load(p);
+                // NTL 1: every load nontemporal; 2: only the populations
+                // whose cache lines no x-neighbouring workgroup reads again
+                if constexpr (NTL == 1 || (NTL == 2 && V::cx[i] == 0)) f[k][i] = __builtin_nontemporal_load(p);
                 else f[k][i] = *p;
             });
         }
@@ -445,7 +450,7 @@ int monitor(erlbm_lattice *h, const void *src, bool stream, bool ordered, double
     return launch_finish(h, ordered, out_dev, check_step, factor);
 }
 
-template <int Q, int COL, bool STREAM, int NPT, bool NTL, int BS>
+template <int Q, int COL, bool STREAM, int NPT, int NTL, int BS>
 void launch_periodic_bs(erlbm_lattice *h, const StepArgs &a)
 {
     const int gx = (h->nx + BS * NPT - 1) / (BS * NPT);
@@ -454,7 +459,7 @@ void launch_periodic_bs(erlbm_lattice *h, const StepArgs &a)
                        h->tune_remap);
 }
 
-template <int Q, int COL, bool STREAM, int NPT, bool NTL>
+template <int Q, int COL, bool STREAM, int NPT, int NTL>
 void launch_periodic(erlbm_lattice *h, const StepArgs &a)
 {
     if constexpr (Q == 9 && NPT == 1) {
@@ -469,6 +474,10 @@ void launch_periodic_tuned(erlbm_lattice *h, const StepArgs &a)
 {
     if constexpr (Q == 9) {
         const bool ntl = h->tune_ntl != 0;
+        if (h->tune_ntl == 2 && h->tune_npt == 1 && !h->tune_pair) {
+            launch_periodic<Q, COL, STREAM, 1, 2>(h, a);
+            return;
+        }
         if (h->tune_pair && (h->nx % 2) == 0 && h->nz == 1) {
             const int gx = (h->nx / 2 + BLOCK - 1) / BLOCK;
             const dim3 grid(gx, h->ny);
@@ -477,12 +486,12 @@ void launch_periodic_tuned(erlbm_lattice *h, const StepArgs &a)
             return;
         }
         switch (h->tune_npt) {
-        case 2: ntl ? launch_periodic<Q, COL, STREAM, 2, true>(h, a) : launch_periodic<Q, COL, STREAM, 2, false>(h, a); return;
-        case 4: ntl ? launch_periodic<Q, COL, STREAM, 4, true>(h, a) : launch_periodic<Q, COL, STREAM, 4, false>(h, a); return;
-        default: ntl ? launch_periodic<Q, COL, STREAM, 1, true>(h, a) : launch_periodic<Q, COL, STREAM, 1, false>(h, a); return;
+        case 2: ntl ? launch_periodic<Q, COL, STREAM, 2, 1>(h, a) : launch_periodic<Q, COL, STREAM, 2, 0>(h, a); return;
+        case 4: ntl ? launch_periodic<Q, COL, STREAM, 4, 1>(h, a) : launch_periodic<Q, COL, STREAM, 4, 0>(h, a); return;
+        default: ntl ? launch_periodic<Q, COL, STREAM, 1, 1>(h, a) : launch_periodic<Q, COL, STREAM, 1, 0>(h, a); return;
         }
     } else {
-        launch_periodic<Q, COL, STREAM, 1, false>(h, a);
+        launch_periodic<Q, COL, STREAM, 1, 0>(h, a);
     }
 }
 

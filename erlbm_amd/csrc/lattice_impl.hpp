@@ -163,7 +163,7 @@ struct erlbm_lattice {
     //   ERLBM_BLOCK      threads per block of that kernel (128, 256, 512)
     //   ERLBM_VEC        vectorised 16-byte kernel for periodic lattices (lattice_vec.hip):
     //                    -1 auto, 0 off, 1 edge elements by wavefront shuffle, 2 edge elements loaded
-    int tune_npt = 1, tune_ntl = 1, tune_remap = 0, tune_pair = 0, tune_block = 256, tune_vec = -1;
+    int tune_npt = 1, tune_ntl = 2, tune_remap = 0, tune_pair = 0, tune_block = 256, tune_vec = -1;
 };
 
 

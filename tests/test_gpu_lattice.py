@@ -149,7 +149,8 @@ def test_fma_reference_tolerance_1000_steps(gpu, golden):
 
 
 @pytest.mark.parametrize("npt,ntl,pad,remap,pair,layout", [(1, 0, 320, 1, 0, 0), (2, 0, 0, 1, 0, 1), (2, 1, 64, 0, 0, 0),
-                                                          (4, 1, 4160, 1, 0, 1), (1, 1, 0, 0, 1, 0), (1, 0, 64, 1, 1, 1)])
+                                                          (4, 1, 4160, 1, 0, 1), (1, 1, 0, 0, 1, 0), (1, 0, 64, 1, 1, 1),
+                                                          (1, 1, 0, 0, 0, 1), (1, 2, 64, 1, 0, 0)])
 def test_periodic_launch_variants_bitwise(gpu, orc, monkeypatch, npt, ntl, pad, remap, pair, layout):
     """Every launch shape and layout the tuner may pick gives the same populations."""
     for k, v in (("ERLBM_NPT", npt), ("ERLBM_NT_LOADS", ntl), ("ERLBM_PLANE_PAD", pad), ("ERLBM_XCD_REMAP", remap),
