@@ -103,6 +103,9 @@ SIGNATURES = {
     "erlbm_batch_destroy": (_i, [_vp]),
     "erlbm_batch_run": (_i, [_vp, _vp, _i, _vp]),
     "erlbm_batch_set_launch": (_i, [_vp, _i, _i]),
+    "erlbm_batch_set_chunk": (_i, [_vp, _i]),
+    "erlbm_lattice_autotune": (_i, [_vp, _i, C.c_char_p, _i, C.POINTER(C.c_float)]),
+    "erlbm_batch_autotune": (_i, [_vp, _vp, _i, C.c_longlong, _vp, _vp, _vp, _vp]),
     "erlbm_tg_map_jobs": (_i, [_i, _i, _i, _d, _i, _i, _i, _i, _i, _vp]),
     "erlbm_obstacle_map_jobs": (_i, [_i, _i, _d, _i, _i, _i, _i, _i, _vp]),
     "erlbm_dist_eq": (_i, [_i, _d, _d, _d, _d, _dp]),

@@ -491,7 +491,9 @@ void launch_periodic_tuned(erlbm_lattice *h, const StepArgs &a)
         default: ntl ? launch_periodic<Q, COL, STREAM, 1, 1>(h, a) : launch_periodic<Q, COL, STREAM, 1, 0>(h, a); return;
         }
     } else {
-        launch_periodic<Q, COL, STREAM, 1, 0>(h, a);
+        if (h->tune_ntl == 2) launch_periodic<Q, COL, STREAM, 1, 2>(h, a);
+        else if (h->tune_ntl == 1) launch_periodic<Q, COL, STREAM, 1, 1>(h, a);
+        else launch_periodic<Q, COL, STREAM, 1, 0>(h, a);
     }
 }
 
@@ -767,13 +769,10 @@ extern "C" int erlbm_upload_aos(erlbm_lattice *h, const double *inif)
     return ERLBM_OK;
 }
 
-extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, double energy_factor)
+namespace {
+
+StepArgs step_args(const erlbm_lattice *h)
 {
-    if (!h || n_steps < 0) return ERLBM_ERR_ARG;
-    if (!h->uploaded) {
-        set_last_error_msg("erlbm_step: upload populations first");
-        return ERLBM_ERR_STATE;
-    }
     StepArgs a;
     a.g = geom(h);
     a.nbr = h->nbr;
@@ -789,6 +788,26 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
     a.dens_limit = h->dens_limit;
     a.status = h->status;
     a.stop_ro = h->status;
+    return a;
+}
+
+int dispatch_step(erlbm_lattice *h, const StepArgs &a, bool stream)
+{
+    if (vec_step_applies(h)) return vec_step(h, a, stream);
+    if (h->elem != 8) return lowp_step(h, a, stream);
+    return h->q == 9 ? launch_step<9>(h, a, stream) : launch_step<19>(h, a, stream);
+}
+
+}  // namespace
+
+extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, double energy_factor)
+{
+    if (!h || n_steps < 0) return ERLBM_ERR_ARG;
+    if (!h->uploaded) {
+        set_last_error_msg("erlbm_step: upload populations first");
+        return ERLBM_ERR_STATE;
+    }
+    StepArgs a = step_args(h);
     for (int s = 0; s < n_steps; s++) {
         const long long k = h->launched;
         const int cur = (h->base + (int)(k & 1)) & 1;
@@ -800,13 +819,132 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
         a.src = static_cast<const double *>(h->buf[cur]);
         a.dst = static_cast<double *>(h->buf[cur ^ 1]);
         a.step = (unsigned long long)k;
-        int rc;
-        if (vec_step_applies(h)) rc = vec_step(h, a, k > 0);
-        else if (h->elem != 8) rc = lowp_step(h, a, k > 0);
-        else rc = h->q == 9 ? launch_step<9>(h, a, k > 0) : launch_step<19>(h, a, k > 0);
+        int rc = dispatch_step(h, a, k > 0);
         if (rc) return rc;
         h->launched++;
     }
+    return ERLBM_OK;
+}
+
+namespace {
+
+// Launch shapes the autotuner tries for a lattice (all give identical
+// populations; only speed differs).
+struct LaunchShape {
+    int ntl, block, npt, pair, vec;
+    const char *name;
+};
+
+std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
+{
+    std::vector<LaunchShape> c;
+    const bool periodic = h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0;
+    if (!periodic) return c;
+    if (h->elem == 8 && h->q == 9) {
+        c.push_back({2, 256, 1, 0, 0, "nt-loads=c_x0 block=256"});
+        c.push_back({2, 512, 1, 0, 0, "nt-loads=c_x0 block=512"});
+        c.push_back({2, 128, 1, 0, 0, "nt-loads=c_x0 block=128"});
+        c.push_back({1, 256, 1, 0, 0, "nt-loads=all block=256"});
+        c.push_back({0, 256, 1, 0, 0, "nt-loads=none block=256"});
+        c.push_back({1, 256, 2, 0, 0, "nt-loads=all nodes/thread=2"});
+        if (h->nx % 2 == 0) c.push_back({1, 256, 1, 1, 0, "nt-loads=all paired 16B"});
+    } else if (h->elem == 8) {
+        c.push_back({2, 256, 1, 0, 0, "nt-loads=c_x0"});
+        c.push_back({1, 256, 1, 0, 0, "nt-loads=all"});
+        c.push_back({0, 256, 1, 0, 0, "nt-loads=none"});
+    } else {
+        c.push_back({h->tune_ntl, h->tune_block, 1, 0, 0, "one node/thread"});
+    }
+    if (!h->f32_arith) {
+        c.push_back({h->tune_ntl, h->tune_block, 1, 0, 1, "16B vectors, shuffled edges"});
+        c.push_back({h->tune_ntl, h->tune_block, 1, 0, 2, "16B vectors, loaded edges"});
+    }
+    return c;
+}
+
+void apply_shape(erlbm_lattice *h, const LaunchShape &s)
+{
+    h->tune_ntl = s.ntl;
+    h->tune_block = s.block;
+    h->tune_npt = s.npt;
+    h->tune_pair = s.pair;
+    h->tune_vec = s.vec;
+}
+
+}  // namespace
+
+extern "C" int erlbm_lattice_autotune(erlbm_lattice *h, int n_steps, char *choice, int choice_len, float *us_per_step)
+{
+    if (!h || n_steps < 1) return ERLBM_ERR_ARG;
+    if (!h->uploaded) {
+        set_last_error_msg("erlbm_lattice_autotune: upload populations first");
+        return ERLBM_ERR_STATE;
+    }
+    const std::vector<LaunchShape> shapes = candidate_shapes(h);
+    const LaunchShape keep{h->tune_ntl, h->tune_block, h->tune_npt, h->tune_pair, h->tune_vec, "unchanged"};
+    // scratch copy of the current state, a private stop word: the handle's
+    // run is not touched
+    const size_t bytes = (size_t)h->extent * h->elem;
+    void *tmp[2] = {nullptr, nullptr};
+    unsigned long long *st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = ERLBM_OK;
+    auto hip = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && rc == ERLBM_OK) {
+            set_last_error(what, e);
+            rc = e == hipErrorOutOfMemory ? ERLBM_ERR_NOMEM : ERLBM_ERR_HIP;
+        }
+        return rc == ERLBM_OK;
+    };
+    int best = -1;
+    float best_ms = 0.f;
+    if (!shapes.empty() && hip(hipMalloc(&tmp[0], bytes), "erlbm_lattice_autotune: hipMalloc") &&
+        hip(hipMalloc(&tmp[1], bytes), "erlbm_lattice_autotune: hipMalloc") &&
+        hip(hipMalloc(&st, 4 * sizeof(unsigned long long)), "erlbm_lattice_autotune: hipMalloc") &&
+        hip(hipEventCreate(&e0), "hipEventCreate") && hip(hipEventCreate(&e1), "hipEventCreate")) {
+        const int cur = (h->base + (int)(h->launched & 1)) & 1;
+        hip(hipMemcpyAsync(tmp[0], h->buf[cur], bytes, hipMemcpyDeviceToDevice, h->stream), "hipMemcpyAsync");
+        hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, h->stream, st);
+        StepArgs a = step_args(h);
+        a.status = st;
+        a.stop_ro = st;
+        unsigned long long k = 0;
+        for (size_t c = 0; c < shapes.size() && rc == ERLBM_OK; c++) {
+            apply_shape(h, shapes[c]);
+            float ms = 0.f;
+            for (int s = -2; s < n_steps && rc == ERLBM_OK; s++) {
+                if (s == 0) hip(hipEventRecord(e0, h->stream), "hipEventRecord");
+                a.src = static_cast<const double *>(tmp[k & 1]);
+                a.dst = static_cast<double *>(tmp[(k & 1) ^ 1]);
+                a.step = k++;
+                const int r = dispatch_step(h, a, true);
+                if (r && rc == ERLBM_OK) rc = r;
+            }
+            if (hip(hipEventRecord(e1, h->stream), "hipEventRecord") &&
+                hip(hipEventSynchronize(e1), "hipEventSynchronize") &&
+                hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) {
+                if (best < 0 || ms < best_ms) {
+                    best = (int)c;
+                    best_ms = ms;
+                }
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(tmp[0]);
+    (void)hipFree(tmp[1]);
+    (void)hipFree(st);
+    const LaunchShape &pick = (rc == ERLBM_OK && best >= 0) ? shapes[best] : keep;
+    apply_shape(h, pick);
+    if (rc) return rc;
+    if (choice && choice_len > 0) {
+        const char *name = best >= 0 ? pick.name : "fixed (no launch choice for this geometry)";
+        std::strncpy(choice, name, (size_t)choice_len - 1);
+        choice[choice_len - 1] = '\0';
+    }
+    if (us_per_step) *us_per_step = best >= 0 ? best_ms * 1000.f / n_steps : 0.f;
     return ERLBM_OK;
 }
 

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -464,6 +465,11 @@ struct erlbm_batch {
     unsigned int *counters = nullptr;
     int wg_per_cu[2] = {0, 0};      // FP64, FP32 (0 = occupancy max)
     int n_cu = 0;
+    int chunk_steps = 0;            // > 0: graph-replayed chunks of this many steps
+    hipStream_t cstream = nullptr;  // capture stream of the chunked mode (created on first use)
+    void *d_saved = nullptr;        // per job state between chunks
+    size_t saved_cap = 0;
+    int *d_n = nullptr;             // live job count of the chunked mode
     // job buffers (grown on demand)
     MapJob *d_jobs = nullptr;
     MapResult *d_results = nullptr;
@@ -535,6 +541,41 @@ void fill_geom(const erlbm_batch *b, TinyGeom &g)
 __global__ void k_zero_counter(unsigned int *c)
 {
     if (threadIdx.x == 0) *c = 0u;
+}
+
+// Early-exit compaction between chunks: idx[0..*n) keeps, in order, the jobs
+// whose result is not done, and *n becomes their count.  One workgroup walks
+// the list in slices of CB; a kept entry moves to a position no later than
+// its own, inside the slice every thread has already read.
+constexpr int CB = 1024;
+__global__ __launch_bounds__(CB) void k_compact(int *idx, int *n, const MapResult *results)
+{
+    __shared__ int wsum[CB / 64];
+    const int n0 = *n;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int base = 0;
+    for (int start = 0; start < n0; start += CB) {
+        const int k = start + (int)threadIdx.x;
+        int j = -1;
+        bool keep = false;
+        if (k < n0) {
+            j = idx[k];
+            keep = results[j].done == 0;
+        }
+        const unsigned long long m = __ballot(keep);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[w] = __popcll(m);
+        __syncthreads();
+        int off = 0, total = 0;
+        for (int q = 0; q < CB / 64; q++) {
+            if (q < w) off += wsum[q];
+            total += wsum[q];
+        }
+        if (keep) idx[base + off + before] = j;
+        base += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *n = base;
 }
 
 }  // namespace
@@ -662,6 +703,9 @@ extern "C" int erlbm_batch_destroy(erlbm_batch *b)
     (void)hipFree(b->d_jobs);
     (void)hipFree(b->d_results);
     (void)hipFree(b->d_idx);
+    (void)hipFree(b->d_saved);
+    (void)hipFree(b->d_n);
+    if (b->cstream) (void)hipStreamDestroy(b->cstream);
     delete b;
     return ERLBM_OK;
 }
@@ -673,6 +717,83 @@ extern "C" int erlbm_batch_set_launch(erlbm_batch *b, int wg_per_cu_f64, int wg_
     b->wg_per_cu[1] = wg_per_cu_f32;
     return ERLBM_OK;
 }
+
+extern "C" int erlbm_batch_set_chunk(erlbm_batch *b, int chunk_steps)
+{
+    if (!b || chunk_steps < 0) return ERLBM_ERR_ARG;
+    b->chunk_steps = chunk_steps;
+    return ERLBM_OK;
+}
+
+namespace {
+
+// Chunked mode of erlbm_batch_run: per (precision, collision) group, one
+// captured graph = [queue reset, persistent kernel advancing every live job
+// by at most chunk_steps steps (state saved per job), compaction of the live
+// list]; the graph is replayed until no job is live.
+int run_chunked(erlbm_batch *b, const erlbm_job *jobs, int n_jobs, const std::vector<int> &idx,
+                const std::vector<int> &prec)
+{
+    ERLBM_HIP_TRY(hipStreamSynchronize(b->stream));
+    if (!b->cstream) ERLBM_HIP_TRY(hipStreamCreateWithFlags(&b->cstream, hipStreamNonBlocking));
+    if (!b->d_n) ERLBM_HIP_TRY(hipMalloc(&b->d_n, sizeof(int) * 16));
+    const size_t need = (size_t)n_jobs * tiny::state_bytes(b, ERLBM_FP64);
+    if (need > b->saved_cap) {
+        (void)hipFree(b->d_saved);
+        b->d_saved = nullptr;
+        b->saved_cap = 0;
+        ERLBM_HIP_TRY(hipMalloc(&b->d_saved, need));
+        b->saved_cap = need;
+    }
+    auto key = [&](int k) { return prec[k] * 8 + jobs[k].collision; };
+    int start = 0, group = 0;
+    std::vector<int> counts(16, 0);
+    while (start < n_jobs) {
+        int end = start;
+        while (end < n_jobs && key(idx[end]) == key(idx[start])) end++;
+        if (group >= 16) return ERLBM_ERR_UNSUPPORTED;
+        const int p = prec[idx[start]];
+        const int col = jobs[idx[start]].collision;
+        counts[group] = end - start;
+        int *d_idx = b->d_idx + start;
+        int *d_n = b->d_n + group;
+        ERLBM_HIP_TRY(hipMemcpyAsync(d_n, &counts[group], sizeof(int), hipMemcpyHostToDevice, b->cstream));
+        ERLBM_HIP_TRY(hipStreamSynchronize(b->cstream));
+        ERLBM_HIP_TRY(hipStreamBeginCapture(b->cstream, hipStreamCaptureModeThreadLocal));
+        int rc = tiny::launch(b, p, col, b->d_jobs, b->d_results, d_idx, d_n, end - start, b->counters + group,
+                              b->chunk_steps, b->d_saved, b->cstream);
+        hipLaunchKernelGGL(k_compact, dim3(1), dim3(CB), 0, b->cstream, d_idx, d_n, (const MapResult *)b->d_results);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->cstream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) {
+            set_last_error("erlbm_batch_run: graph capture", e);
+            return ERLBM_ERR_HIP;
+        }
+        hipGraphExec_t ge = nullptr;
+        hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        int live = end - start;
+        while (ei == hipSuccess && live > 0) {
+            ei = hipGraphLaunch(ge, b->cstream);
+            if (ei == hipSuccess) ei = hipMemcpyAsync(&live, d_n, sizeof(int), hipMemcpyDeviceToHost, b->cstream);
+            if (ei == hipSuccess) ei = hipStreamSynchronize(b->cstream);
+        }
+        if (ge) (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            set_last_error("erlbm_batch_run: chunk", ei);
+            return ERLBM_ERR_HIP;
+        }
+        start = end;
+        group++;
+    }
+    return ERLBM_OK;
+}
+
+}  // namespace
 
 extern "C" int erlbm_batch_run(erlbm_batch *b, const erlbm_job *jobs, int n_jobs, erlbm_job_result *results)
 {
@@ -727,32 +848,133 @@ extern "C" int erlbm_batch_run(erlbm_batch *b, const erlbm_job *jobs, int n_jobs
     ERLBM_HIP_TRY(hipMemcpyAsync(b->d_idx, idx.data(), sizeof(int) * n_jobs, hipMemcpyHostToDevice, b->stream));
     ERLBM_HIP_TRY(hipMemsetAsync(b->counters, 0, sizeof(unsigned int) * 16, b->stream));
     ERLBM_HIP_TRY(hipMemsetAsync(b->d_results, 0, sizeof(MapResult) * n_jobs, b->stream));
-    RunArgs a;
-    fill_geom(b, a.g);
-    a.jobs = b->d_jobs;
-    a.results = b->d_results;
-    a.n_jobs_dev = nullptr;
-    a.max_steps = 0;
-    a.saved = nullptr;
-    int start = 0, group = 0;
-    while (start < n_jobs) {
-        int end = start;
-        while (end < n_jobs && key(idx[end]) == key(idx[start])) end++;
-        if (group >= 16) return ERLBM_ERR_UNSUPPORTED;
-        a.job_idx = b->d_idx + start;
-        a.n_jobs = end - start;
-        const int p = prec[idx[start]];
-        const int col = jobs[idx[start]].collision;
-        int rc = (p == ERLBM_FP64)            ? launch_col<F64>(b, col, a, b->counters + group)
-                 : (p == ERLBM_FP32_SHIFTED) ? launch_col<F32>(b, col, a, b->counters + group)
-                                             : launch_col<F32A>(b, col, a, b->counters + group);
+    if (b->chunk_steps > 0) {
+        int rc = run_chunked(b, jobs, n_jobs, idx, prec);
         if (rc) return rc;
-        start = end;
-        group++;
+    } else {
+        RunArgs a;
+        fill_geom(b, a.g);
+        a.jobs = b->d_jobs;
+        a.results = b->d_results;
+        a.n_jobs_dev = nullptr;
+        a.max_steps = 0;
+        a.saved = nullptr;
+        int start = 0, group = 0;
+        while (start < n_jobs) {
+            int end = start;
+            while (end < n_jobs && key(idx[end]) == key(idx[start])) end++;
+            if (group >= 16) return ERLBM_ERR_UNSUPPORTED;
+            a.job_idx = b->d_idx + start;
+            a.n_jobs = end - start;
+            const int p = prec[idx[start]];
+            const int col = jobs[idx[start]].collision;
+            int rc = (p == ERLBM_FP64)            ? launch_col<F64>(b, col, a, b->counters + group)
+                     : (p == ERLBM_FP32_SHIFTED) ? launch_col<F32>(b, col, a, b->counters + group)
+                                                 : launch_col<F32A>(b, col, a, b->counters + group);
+            if (rc) return rc;
+            start = end;
+            group++;
+        }
     }
     ERLBM_HIP_TRY(hipMemcpyAsync(results, b->d_results, sizeof(MapResult) * n_jobs, hipMemcpyDeviceToHost,
                                  b->stream));
     ERLBM_HIP_TRY(hipStreamSynchronize(b->stream));
+    return ERLBM_OK;
+}
+
+// ---- autotuner --------------------------------------------------------------------
+namespace {
+
+template <class P, int COL>
+int occupancy_of(const erlbm_batch *b)
+{
+    int per_cu = 0;
+    const size_t nvp = (size_t)(b->nx * b->ny + 7) / 8 * 8;
+    const size_t lds = sizeof(typename P::T) * Q * nvp;
+    hipError_t e = b->general
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, true>, TPB, lds)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, false>, TPB, lds);
+    return e == hipSuccess && per_cu > 0 ? per_cu : 1;
+}
+
+template <class P>
+int occupancy_col(const erlbm_batch *b, int col)
+{
+    switch (col) {
+    case ERLBM_COL_BGK: return occupancy_of<P, COL_BGK>(b);
+    case ERLBM_COL_REG: return occupancy_of<P, COL_REG>(b);
+    default: return occupancy_of<P, COL_EXT_REG>(b);
+    }
+}
+
+int occupancy_max(const erlbm_batch *b, int precision, int col)
+{
+    if (precision == ERLBM_FP64) return occupancy_col<F64>(b, col);
+    if (precision == ERLBM_FP32_SHIFTED) return occupancy_col<F32>(b, col);
+    return occupancy_col<F32A>(b, col);
+}
+
+}  // namespace
+
+extern "C" int erlbm_batch_autotune(erlbm_batch *b, const erlbm_job *sample, int n_sample, long long max_steps,
+                                    int *wg_per_cu_f64, int *wg_per_cu_f32, int *chunk_steps, double *points_per_s)
+{
+    if (!b || !sample || n_sample < 1 || max_steps < 1) return ERLBM_ERR_ARG;
+    std::vector<erlbm_job> all(sample, sample + n_sample), cls[2];
+    for (auto &j : all) {
+        if (j.n_steps > max_steps) j.n_steps = max_steps;
+        cls[resolve_precision(j) == ERLBM_FP64 ? 0 : 1].push_back(j);
+    }
+    std::vector<MapResult> res(n_sample);
+    int rc = ERLBM_OK;
+    auto timed = [&](std::vector<erlbm_job> &v) -> double {
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = erlbm_batch_run(b, v.data(), (int)v.size(), res.data());
+        const auto t1 = std::chrono::steady_clock::now();
+        return rc ? -1.0 : std::chrono::duration<double>(t1 - t0).count();
+    };
+    const int chunk0 = b->chunk_steps;
+    b->chunk_steps = 0;
+    // workgroups per CU, per precision class: 1 .. the occupancy limit
+    for (int c = 0; c < 2; c++) {
+        if (cls[c].empty()) continue;
+        const int maxpc = occupancy_max(b, resolve_precision(cls[c][0]), cls[c][0].collision);
+        if (timed(cls[c]) < 0) break;   // warm-up (buffer growth)
+        double best = 1e300;
+        int best_v = 0;
+        for (int v = 1; v <= maxpc; v++) {
+            b->wg_per_cu[c] = v;
+            const double t = std::min(timed(cls[c]), timed(cls[c]));
+            if (rc) break;
+            if (t < best) {
+                best = t;
+                best_v = v;
+            }
+        }
+        b->wg_per_cu[c] = best_v;
+        if (rc) break;
+    }
+    // run to retirement in one launch, or graph-replayed chunks with compaction
+    double best = 1e300;
+    int best_chunk = 0;
+    if (rc == ERLBM_OK) {
+        for (int ch : {0, 100, 1000}) {
+            b->chunk_steps = ch;
+            timed(all);   // warm-up (saved-state buffer)
+            const double t = std::min(timed(all), timed(all));
+            if (rc) break;
+            if (t < best) {
+                best = t;
+                best_chunk = ch;
+            }
+        }
+    }
+    b->chunk_steps = rc ? chunk0 : best_chunk;
+    if (rc) return rc;
+    if (wg_per_cu_f64) *wg_per_cu_f64 = b->wg_per_cu[0];
+    if (wg_per_cu_f32) *wg_per_cu_f32 = b->wg_per_cu[1];
+    if (chunk_steps) *chunk_steps = best_chunk;
+    if (points_per_s) *points_per_s = n_sample / best;
     return ERLBM_OK;
 }
 

@@ -96,6 +96,14 @@ class Lattice:
         check(lib().erlbm_step_timed(self._h, n, C.byref(ms)), "erlbm_step_timed")
         return ms.value
 
+    def autotune(self, n_steps: int = 200):
+        """Pick the fastest launch shape for this lattice (the run is not
+        advanced); returns (description, microseconds per step)."""
+        buf = C.create_string_buffer(128)
+        us = C.c_float()
+        check(lib().erlbm_lattice_autotune(self._h, n_steps, buf, 128, C.byref(us)), "erlbm_lattice_autotune")
+        return buf.value.decode(), us.value
+
     def status(self):
         s, r = C.c_longlong(0), C.c_int(0)
         check(lib().erlbm_status(self._h, C.byref(s), C.byref(r)), "erlbm_status")

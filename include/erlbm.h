@@ -114,6 +114,14 @@ int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, double energy
 /* erlbm_step(h, n_steps, 0, 0) bracketed by HIP events recorded on the
  * handle's stream; *ms = elapsed milliseconds (synchronises).            */
 int erlbm_step_timed(erlbm_lattice *h, int n_steps, float *ms);
+/* Autotuner (no counterpart in the reference): times n_steps steps of every
+ * launch shape this lattice's kernels offer (nontemporal-load policy, block
+ * size, nodes per thread, 16-byte vector kernels) on a scratch copy of the
+ * current state, keeps the fastest for the handle and reports it
+ * (`choice`, may be NULL) with its time per step.  The run itself is not
+ * advanced.  Geometries with solids or boundary columns have one kernel
+ * and nothing to choose.  Needs uploaded populations; synchronises.      */
+int erlbm_lattice_autotune(erlbm_lattice *h, int n_steps, char *choice, int choice_len, float *us_per_step);
 /* Global steps run so far and the stop reason (0 if none); synchronises. */
 int erlbm_status(erlbm_lattice *h, long long *steps_done, int *stop_reason);
 int erlbm_synchronize(erlbm_lattice *h);
@@ -208,6 +216,20 @@ int erlbm_batch_run(erlbm_batch *b, const erlbm_job *jobs, int n_jobs, erlbm_job
 /* Launch shape of the persistent kernels: workgroups per CU for the FP64 and
  * FP32 kernels (0 = occupancy maximum).  Chosen by the autotuner.       */
 int erlbm_batch_set_launch(erlbm_batch *b, int wg_per_cu_f64, int wg_per_cu_f32);
+/* chunk_steps = 0 (default): each (precision, collision) group of a run is
+ * one persistent launch that takes jobs until every job has retired.
+ * chunk_steps > 0: each group runs as a captured HIP graph [queue reset,
+ * persistent kernel advancing every live job by <= chunk_steps steps with
+ * its state saved in HBM, compaction of the live-job list], replayed until
+ * no job is live.  Results are identical either way.                     */
+int erlbm_batch_set_chunk(erlbm_batch *b, int chunk_steps);
+/* Autotuner (no counterpart in the reference): times the sample, each job's
+ * step budget capped at max_steps, under every workgroups-per-CU value up
+ * to the occupancy limit (per precision class), then under chunk_steps in
+ * {0, 100, 1000}; keeps the fastest on the batch and reports it.  Any out
+ * pointer may be NULL; *points_per_s is the sample's rate at the choice.  */
+int erlbm_batch_autotune(erlbm_batch *b, const erlbm_job *sample, int n_sample, long long max_steps,
+                         int *wg_per_cu_f64, int *wg_per_cu_f32, int *chunk_steps, double *points_per_s);
 
 /* Job lists of the two published maps, with the drivers' axis expressions:
  * Taylor-Green map (src/main_stability-map_taylor-green.cpp:134-164; L = 16,

@@ -204,3 +204,30 @@ def test_vectorised_kernel_bitwise(gpu, orc, monkeypatch, vec, layout, q, L):
     with _periodic(gpu, q, L, tau, tnh, O.COL_EXT_REG, f0) as lat:
         lat.step(n)
         assert np.array_equal(lat.download(), want)
+
+
+@pytest.mark.parametrize("q,L,prec", [(9, 96, "FP64"), (19, 16, "FP64"), (9, 64, "FP16_SHIFTED")])
+def test_autotune_leaves_the_run_alone(gpu, orc, q, L, prec):
+    """The autotuner times candidate launch shapes on a scratch copy: the run
+    continues from where it was, bitwise as if no tuning happened."""
+    p = getattr(gpu, prec)
+    if q == 9:
+        tau = 0.5 + 3. * (1. / 32.) * L / 80000.
+        f0 = gpu.init_double_shear(L)
+    else:
+        tau = 0.503
+        f0 = gpu.init_tg_map(L, 0.05)
+    ini, st, nf = orc.stream_map(q, L, L, 1)
+    with gpu.Lattice(q, L, L, 1, precision=p) as a, gpu.Lattice(q, L, L, 1, precision=p) as b:
+        for lat in (a, b):
+            lat.set_params(tau, 0.65, 1.0, O.COL_EXT_REG)
+            lat.upload(f0)
+            lat.step(7)
+        name, us = a.autotune(20)
+        print(f"q={q} L={L} {prec}: picked '{name}' at {us:.2f} us/step", flush=True)
+        assert us > 0 and name
+        a.step(13)
+        b.step(13)
+        assert np.array_equal(a.download(), b.download())
+        if prec == "FP64":
+            assert np.array_equal(a.download(), orc.run_steps(q, f0, st, 20, O.COL_EXT_REG, tau, 0.65))

@@ -128,3 +128,43 @@ def test_result_independent_of_batch_mix(gpu):
             assert alone.tobytes() == together[k:k + 1].tobytes()
         rev = b.run(mix[::-1].copy())[::-1]
         assert rev.tobytes() == together.tobytes()
+
+
+def _mixed_jobs(gpu):
+    """Stable and unstable Taylor-Green points (RLBM and E-RLBM, FP64 and FP32
+    storage) with budgets cut so the test stays short."""
+    a = gpu.tg_map_jobs(400, 400, O.COL_EXT_REG, 0.65, gpu.FP64, 0, 400, 10, 11)
+    b = gpu.tg_map_jobs(400, 400, O.COL_REG, 1.0, gpu.FP_AUTO, 0, 400, 300, 301)
+    jobs = np.concatenate([a, b])
+    jobs["n_steps"] = np.minimum(jobs["n_steps"], 3000)
+    return jobs
+
+
+def test_chunked_graph_mode_equals_one_launch(gpu):
+    """Graph-replayed 100-step chunks with compaction of the live jobs give
+    exactly the results of one persistent launch per group."""
+    jobs = _mixed_jobs(gpu)
+    obst = gpu.obstacle_map_jobs(400, 400, 0.1, gpu.FP64, 0, 400, 200, 201)
+    obst["n_steps"] = np.minimum(obst["n_steps"], 4000)
+    with gpu.tg_batch() as b:
+        ref = b.run(jobs)
+        for chunk in (100, 1000, 7):
+            b.set_chunk(chunk)
+            got = b.run(jobs)
+            assert got.tobytes() == ref.tobytes(), chunk
+    with gpu.obstacle_batch() as b:
+        ref = b.run(obst)
+        b.set_chunk(100)
+        assert b.run(obst).tobytes() == ref.tobytes()
+    assert 0 < int(ref["unstable"].sum()) < len(ref)
+
+
+def test_batch_autotune_keeps_results(gpu):
+    jobs = _mixed_jobs(gpu)
+    with gpu.tg_batch() as b:
+        ref = b.run(jobs)
+        pick = b.autotune(jobs, max_steps=500)
+        print(f"batch autotune: {pick}", flush=True)
+        assert pick["wg_per_cu_f64"] >= 1 and pick["wg_per_cu_f32"] >= 1
+        assert pick["chunk_steps"] in (0, 100, 1000) and pick["sample_points_per_s"] > 0
+        assert b.run(jobs).tobytes() == ref.tobytes()
