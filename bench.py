@@ -73,21 +73,28 @@ def cpu_baseline(seconds: float):
             "sample": f"D2Q9 E-RLBM double shear L={L}, first {n} steps ({dt:.1f} s wall, {how})"}
 
 
-def dsl_run(E, precision: str, steps: int, warmup: int):
-    """Timed D2Q9 double-shear steps at L=2048 (HIP events on the lattice's stream)."""
+def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True):
+    """Timed D2Q9 double-shear steps at L=2048 (HIP events on the lattice's
+    stream), after the autotuner has picked the launch shape."""
     prec = {"fp64": E.FP64, "fp32": E.FP32_SHIFTED, "fp16": E.FP16_SHIFTED}[precision]
     f0 = E.init_double_shear(L)
     lat = E.Lattice(9, L, L, 1, precision=prec)
     lat.set_params(TAU, TAU_NH, 1.0, E.COL_EXT_REG)
+    t = time.perf_counter()
     lat.upload(f0)
+    lat.synchronize()
+    t_up = time.perf_counter() - t
     mass0 = float(f0.sum())
     del f0
     lat.step(warmup)
     lat.synchronize()
+    launch = lat.autotune(200)[0] if tune else "default"
     ms = lat.step_timed(steps)
     steps_done, reason = lat.status()
     ms_per_step = ms / steps
+    t = time.perf_counter()
     post = lat.download(E.POST_COLLISION)
+    t_down = time.perf_counter() - t
     mass_err = abs(float(post.sum()) - mass0) / mass0
     finite = bool(np.isfinite(post).all())
     del post
@@ -95,8 +102,12 @@ def dsl_run(E, precision: str, steps: int, warmup: int):
     ok = steps_done == warmup + steps and reason == 0 and finite and mass_err < (1e-9 if prec == E.FP64 else 1e-3)
     mlups = L * L / (ms_per_step * 1e-3) / 1e6
     gbs = BYTES_PER_NODE[precision] * L * L / (ms_per_step * 1e-3) / 1e9
+    aos_bytes = L * L * 9 * 8
     return ok, {"MLUPS": round(mlups, 1), "ms_per_step": round(ms_per_step, 5), "achieved_GBs": round(gbs, 1),
-                "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "mass_rel_error": mass_err}
+                "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "mass_rel_error": mass_err, "launch": launch,
+                "host_transfer": {"aos_bytes": aos_bytes, "upload_s": round(t_up, 4), "download_s": round(t_down, 4),
+                                  "upload_GBs": round(aos_bytes / t_up / 1e9, 2),
+                                  "download_GBs": round(aos_bytes / t_down / 1e9, 2)}}
 
 
 def maps_section(E):
@@ -108,13 +119,15 @@ def maps_section(E):
     for name, mk, jobs, nf in (("tg_erlbm065", E.tg_batch, E.tg_map_jobs(400, 400, E.COL_EXT_REG, 0.65, E.FP64), 256),
                                ("obstacle", E.obstacle_batch, E.obstacle_map_jobs(400, 400, 0.1, E.FP64), 232)):
         with mk() as b:
+            pick = b.autotune(np.random.default_rng(3).choice(jobs, 4096, replace=False), max_steps=2000)
             t = time.perf_counter()
             res = b.run(jobs)
             dt = time.perf_counter() - t
         agree = float(np.mean(res["unstable"] == pub[name].reshape(-1)))
         out[name] = {"points": int(jobs.size), "seconds": round(dt, 3), "points_per_s": round(jobs.size / dt, 1),
                      "node_steps_per_s": float(np.sum(res["steps"].astype(float)) * nf / dt),
-                     "unstable": int(res["unstable"].sum()), "agreement_vs_published_data": round(agree, 5)}
+                     "unstable": int(res["unstable"].sum()), "agreement_vs_published_data": round(agree, 5),
+                     "autotune": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in pick.items()}}
     # the plain FP64 path, one run at a time, on a random sample of points
     from erlbm_amd import serve as S
     rng = np.random.default_rng(5)
@@ -159,6 +172,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline FP64 run only")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "fp16"],
                     help="storage of the headline run (fp32/fp16: shifted deviations, FP64 arithmetic)")
     ap.add_argument("--copy-only", action="store_true",
@@ -181,7 +195,7 @@ def main() -> int:
         print(json.dumps({"copy_bytes_each_way": copy_bytes, "ms": copy_ms, "GBs": copy_gbs}))
         return 0
 
-    ok, head = dsl_run(E, a.precision, a.steps, a.warmup)
+    ok, head = dsl_run(E, a.precision, a.steps, a.warmup, tune=not a.no_tune)
     if not ok:
         print(f"bench.py: run check failed {head}", file=sys.stderr)
         return 1
@@ -192,7 +206,7 @@ def main() -> int:
             if p == a.precision:
                 prec[p] = head
                 continue
-            okp, r = dsl_run(E, p, max(200, a.steps // 2), 20)
+            okp, r = dsl_run(E, p, max(200, a.steps // 2), 20, tune=not a.no_tune)
             prec[p] = r if okp else {"error": "run check failed", **r}
         prec["accuracy_vs_fp64"] = ("FP32 storage: L2 velocity error 2.1e-7 (DSL L=256, 1000 steps), "
                                     "TG L=64 E_vector within 0.01%; FP16: 1.7e-3 / +401% "
@@ -223,12 +237,14 @@ def main() -> int:
         "data": "synthetic: double-shear-layer initial field (deterministic, no dataset)",
         "config": {"workload": "D2Q9 E-RLBM double shear layer, periodic L=2048 (4,194,304 nodes), "
                                f"tau=0.5024, tau_nh=0.65, {a.precision} row-blocked pull-stream+collide",
-                   "L": L, "tau": TAU, "tau_nh": TAU_NH, "collision": "E-RLBM", "precision": a.precision},
+                   "L": L, "tau": TAU, "tau_nh": TAU_NH, "collision": "E-RLBM", "precision": a.precision,
+                   "launch": head["launch"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "copy_kernel_GBs": round(copy_gbs, 1),
         "mass_rel_error": head["mass_rel_error"],
+        "host_transfer_not_in_value": head["host_transfer"],
         **extra,
     }
     print(json.dumps(line))

@@ -105,6 +105,16 @@ SIGNATURES = {
     "erlbm_batch_set_launch": (_i, [_vp, _i, _i]),
     "erlbm_batch_set_chunk": (_i, [_vp, _i]),
     "erlbm_lattice_autotune": (_i, [_vp, _i, C.c_char_p, _i, C.POINTER(C.c_float)]),
+    "erlbm_velocity_field": (_i, [_vp, _i, _vp]),
+    "erlbm_vorticity_field": (_i, [_vp, _i, _vp]),
+    "erlbm_enstrophy": (_i, [_vp, _i, C.POINTER(C.c_double)]),
+    "erlbm_write_velocity_vtk": (_i, [C.c_char_p, _i, _i, _i, _vp]),
+    "erlbm_write_vorticity_vtk": (_i, [C.c_char_p, _i, _i, _i, _vp]),
+    "erlbm_read_velocity_vtk": (_i, [C.c_char_p, _vp, C.c_longlong, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
+    "erlbm_vtk_round": (_i, [_vp, C.c_longlong]),
+    "erlbm_velocity_l2_error": (_i, [_vp, _i, _i, _vp, _i, _i, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "erlbm_write_tg_map_vtk": (_i, [C.c_char_p, _i, _i, _i, C.c_double, _vp]),
+    "erlbm_write_obstacle_map_vtk": (_i, [C.c_char_p, _i, _i, C.c_double, _vp]),
     "erlbm_batch_autotune": (_i, [_vp, _vp, _i, C.c_longlong, _vp, _vp, _vp, _vp]),
     "erlbm_tg_map_jobs": (_i, [_i, _i, _i, _d, _i, _i, _i, _i, _i, _vp]),
     "erlbm_obstacle_map_jobs": (_i, [_i, _i, _d, _i, _i, _i, _i, _i, _vp]),

@@ -72,7 +72,30 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     cmd = [cc, "-shared", f"--offload-arch={ARCH}", *objs, "-o", tmp]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_drivers(verbose)
     return LIB
+
+
+DRIVERS = os.path.join(HERE, "drivers")
+BIN = os.path.join(HERE, "bin")
+
+
+def build_drivers(verbose: bool = False) -> list:
+    """The C++ drivers (drivers/*.cpp): host programs over the C ABI, linked
+    against liberlbm.so (found next to bin/ at run time)."""
+    os.makedirs(BIN, exist_ok=True)
+    cxx = shutil.which("g++") or "g++"
+    out = []
+    for src in sorted(glob.glob(os.path.join(DRIVERS, "*.cpp"))):
+        exe = os.path.join(BIN, os.path.splitext(os.path.basename(src))[0])
+        cmd = [cxx, "-std=c++17", "-O2", "-ffp-contract=off", f"-I{INCLUDE}", f"-I{DRIVERS}", src, "-o", exe + ".tmp",
+               f"-L{HERE}", "-lerlbm", "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
 
 
 if __name__ == "__main__":

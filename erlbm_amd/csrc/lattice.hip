@@ -999,30 +999,37 @@ int current(erlbm_lattice *h, long long *run, int *cur)
 
 }  // namespace
 
-extern "C" int erlbm_download_aos(erlbm_lattice *h, int which, double *inif)
+namespace erlbm {
+namespace lat {
+
+// The populations `which` names, as device AoS in reference fluid order.
+// *owned is set when *aos was allocated here (shifted storage); otherwise
+// *aos is the handle's spare population buffer, valid until the next step.
+int to_aos_device(erlbm_lattice *h, int which, double **aos, void **owned)
 {
-    if (!h || !inif) return ERLBM_ERR_ARG;
+    *aos = nullptr;
+    *owned = nullptr;
     if (!h->uploaded) return ERLBM_ERR_STATE;
     long long run;
     int cur;
     int rc = current(h, &run, &cur);
     if (rc) return rc;
     if (which == ERLBM_POST_COLLISION && run == 0) {
-        set_last_error_msg("erlbm_download_aos: no step has run since the upload");
+        set_last_error_msg("no step has run since the upload");
         return ERLBM_ERR_STATE;
     }
     const bool stream = (which == ERLBM_STATE) && run > 0;
-    const size_t bytes = sizeof(double) * (size_t)h->n_fluid * h->q;
     if (h->elem != 8) {
         double *stage = nullptr;
-        ERLBM_HIP_TRY(hipMalloc(&stage, bytes));
+        ERLBM_HIP_TRY(hipMalloc(&stage, sizeof(double) * (size_t)h->n_fluid * h->q));
         rc = lowp_to_aos(h, h->buf[cur], stream, stage);
-        if (rc == ERLBM_OK && hipMemcpyAsync(inif, stage, bytes, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
-            rc = ERLBM_ERR_HIP;
-        if (hipStreamSynchronize(h->stream) != hipSuccess && rc == ERLBM_OK) rc = ERLBM_ERR_HIP;
-        (void)hipFree(stage);
-        if (rc) set_last_error_msg("erlbm_download_aos: shifted download failed");
-        return rc;
+        if (rc) {
+            (void)hipFree(stage);
+            return rc;
+        }
+        *aos = stage;
+        *owned = stage;
+        return ERLBM_OK;
     }
     double *stage = static_cast<double *>(h->buf[cur ^ 1]);
     const double *src = static_cast<const double *>(h->buf[cur]);
@@ -1044,9 +1051,27 @@ extern "C" int erlbm_download_aos(erlbm_lattice *h, int which, double *inif)
                                h->vox_of_fluid, h->n_fluid, stage);
     }
     ERLBM_HIP_TRY(hipGetLastError());
-    ERLBM_HIP_TRY(hipMemcpyAsync(inif, stage, bytes, hipMemcpyDeviceToHost, h->stream));
-    ERLBM_HIP_TRY(hipStreamSynchronize(h->stream));
+    *aos = stage;
     return ERLBM_OK;
+}
+
+}  // namespace lat
+}  // namespace erlbm
+
+extern "C" int erlbm_download_aos(erlbm_lattice *h, int which, double *inif)
+{
+    if (!h || !inif) return ERLBM_ERR_ARG;
+    if (!h->uploaded) return ERLBM_ERR_STATE;
+    double *aos = nullptr;
+    void *owned = nullptr;
+    int rc = to_aos_device(h, which, &aos, &owned);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * (size_t)h->n_fluid * h->q;
+    if (hipMemcpyAsync(inif, aos, bytes, hipMemcpyDeviceToHost, h->stream) != hipSuccess) rc = ERLBM_ERR_HIP;
+    if (hipStreamSynchronize(h->stream) != hipSuccess && rc == ERLBM_OK) rc = ERLBM_ERR_HIP;
+    (void)hipFree(owned);
+    if (rc) set_last_error_msg("erlbm_download_aos: copy failed");
+    return rc;
 }
 
 extern "C" int erlbm_kinetic_energy(erlbm_lattice *h, int which, int ordered, double *energy)

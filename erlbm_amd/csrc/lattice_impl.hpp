@@ -183,6 +183,10 @@ int lowp_to_aos(erlbm_lattice *h, const void *src, bool stream, double *aos);
 // Per-site 0.5 |u|^2 and |rho| partials (FP64) of the state in `src`.
 int lowp_monitor(erlbm_lattice *h, const void *src, bool stream, bool ordered);
 
+// Device AoS copy of the populations `which` names (lattice.hip); *owned
+// (to be hipFree'd) is set when the copy was allocated for the call.
+int to_aos_device(erlbm_lattice *h, int which, double **aos, void **owned);
+
 // Vectorised periodic kernel (lattice_vec.hip), any storage width.
 bool vec_step_applies(const erlbm_lattice *h);
 int vec_step(erlbm_lattice *h, const StepArgs &a, bool stream);

@@ -137,6 +137,42 @@ int erlbm_kinetic_energy(erlbm_lattice *h, int which, int ordered, double *energ
 int erlbm_density_monitor(erlbm_lattice *h, int which, double thresh, double *max_abs_rho,
                           int *violated);
 
+/* ---- output fields and files --------------------------------------------------
+ * The drivers' output, computed on the GPU from the populations `which`
+ * names (ERLBM_STATE / ERLBM_POST_COLLISION) and copied to the host once.
+ * Vector fields are 3 doubles per voxel, x fastest, as the VTK files hold
+ * them.                                                                     */
+/* calcula at every fluid site (LBM_functions.cpp:3517-3589) as rec_velocity
+ * (:4218-4288) writes it: u at fluid voxels, 0 at solid ones.              */
+int erlbm_velocity_field(erlbm_lattice *h, int which, double *vel);
+/* vorticity (LBM_functions.cpp:3761-3872) as rec_vorticity (:4302-4346)
+ * writes it: central differences with periodic wrap, a solid neighbour
+ * counting as u = 0, 0 at solid voxels.                                    */
+int erlbm_vorticity_field(erlbm_lattice *h, int which, double *vort);
+/* calc_enstrophy (Other_functions.cpp:119-142): sum of 0.5 |w|^2 over every
+ * voxel in the reference's order (x outermost), bitwise.                   */
+int erlbm_enstrophy(erlbm_lattice *h, int which, double *enstrophy);
+/* The files rec_velocity / rec_vorticity write (legacy ASCII VTK, six
+ * significant digits), byte for byte.  Names: "vel_" + std::to_string(t)
+ * + ".vtk" and "vor_" + step + ".vtk" in the reference.                    */
+int erlbm_write_velocity_vtk(const char *path, int nx, int ny, int nz, const double *vel);
+int erlbm_write_vorticity_vtk(const char *path, int nx, int ny, int nz, const double *vort);
+/* read_velocity of tools/error_calculator.cpp:125-168: dimensions, and with
+ * vel != NULL (cap >= 3 nx ny nz doubles) the vectors.                     */
+int erlbm_read_velocity_vtk(const char *path, double *vel, long long cap, int *nx, int *ny, int *nz);
+/* Each value replaced by what a write-then-read of the VTK text gives. */
+int erlbm_vtk_round(double *v, long long n);
+/* tools/error_calculator.cpp:35-114: relative L2 errors of a run's velocity
+ * (nx x ny) against a reference run on an integer refinement of its grid,
+ * sampled at coinciding nodes: E_vector (the paper's) and E_magnitude.     */
+int erlbm_velocity_l2_error(const double *sim, int nx, int ny, const double *ref, int nx_ref, int ny_ref,
+                            double *e_vector, double *e_magnitude);
+/* The map files of src/main_stability-map_taylor-green.cpp:181-200 and
+ * src/main_stability-map_obstacle.cpp:188-207 (map[i + j n_i], 1 unstable);
+ * tau_nh is the driver's --tau-nh value, as it prints it.                  */
+int erlbm_write_tg_map_vtk(const char *path, int n_ma, int n_nu, int collision, double tau_nh, const int *map);
+int erlbm_write_obstacle_map_vtk(const char *path, int n_tau, int n_nu, double Ma, const int *map);
+
 /* ---- host-side driver helpers ---------------------------------------------
  * Initial conditions, geometries and map axes of the reference drivers,
  * computed on the host with the reference's libm calls and operation order

@@ -258,6 +258,10 @@ class Reference(_Lib):
         _sig(L, f"ref_boundary_{s}", None, [_i, _i, _i, _i, _ip, _dp, _i, _d])
         _sig(L, f"ref_run_steps_{s}", _i, [_i, _i, _i, _ip, _dp, _i, _i, _d, _d, _i, _d, _i, _d, _d, C.POINTER(_i)])
         _sig(L, f"ref_run_steps_omp_{s}", _d, [_i, _i, _i, _dp, _i, _i, _d, _d, _i])
+        _sig(L, f"ref_rec_velocity_{s}", None, [_i, _i, _i, _ip, _dp, _d])
+        _sig(L, f"ref_rec_vorticity_{s}", None, [_i, _i, _i, _ip, _dp, C.c_uint])
+        _sig(L, f"ref_enstrophy_{s}", _d, [_i, _i, _i, _ip, _dp])
+        _sig(L, f"ref_driver_main_{s}", _i, [_i, _i, C.POINTER(C.c_char_p)])
         if q == 9:
             _sig(L, "ref_init_taylor_green_d2q9", None, [_i, _d, _d, _i, _i, _dp])
             _sig(L, "ref_init_double_shear_d2q9", None, [_i, _dp])
@@ -356,10 +360,83 @@ class Reference(_Lib):
         self.lib.ref_obstacle_map_points(ma, n_tau, n_nu, ii, jj, ii.size, out)
         return out
 
+    def rec_velocity(self, nx, ny, nz, mask, inif, tempo, cwd):
+        """rec_velocity into directory cwd; returns the file's bytes."""
+        m = np.ones(nx * ny * nz, np.int32) if mask is None else np.ascontiguousarray(mask, np.int32)
+        g = np.ascontiguousarray(inif, np.float64).copy()
+        with _InDir(cwd):
+            getattr(self.lib, f"ref_rec_velocity_{self.s}")(nx, ny, nz, m, g.reshape(-1), tempo)
+        with open(os.path.join(cwd, "vel_%f.vtk" % tempo), "rb") as fh:
+            return fh.read()
+
+    def rec_vorticity(self, nx, ny, nz, mask, inif, passo, cwd):
+        m = np.ones(nx * ny * nz, np.int32) if mask is None else np.ascontiguousarray(mask, np.int32)
+        g = np.ascontiguousarray(inif, np.float64).copy()
+        with _InDir(cwd):
+            getattr(self.lib, f"ref_rec_vorticity_{self.s}")(nx, ny, nz, m, g.reshape(-1), passo)
+        with open(os.path.join(cwd, f"vor_{passo}.vtk"), "rb") as fh:
+            return fh.read()
+
+    def enstrophy(self, nx, ny, nz, mask, inif):
+        m = np.ones(nx * ny * nz, np.int32) if mask is None else np.ascontiguousarray(mask, np.int32)
+        return getattr(self.lib, f"ref_enstrophy_{self.s}")(nx, ny, nz, m,
+                                                             np.ascontiguousarray(inif, np.float64).reshape(-1))
+
+    def driver_main(self, which, args, cwd):
+        """A reference driver's main() run in directory cwd; returns (status,
+        what it printed to stdout).  D2Q9: 0 taylor-green, 1 double shear
+        layer, 2 error calculator; D3Q19: 0 Taylor-Green map, 1 obstacle map."""
+        argv = (C.c_char_p * (len(args) + 1))(*[a.encode() for a in ["driver", *args]])
+        with _InDir(cwd), _Stdout(os.path.join(cwd, "_stdout.txt")):
+            rc = getattr(self.lib, f"ref_driver_main_{self.s}")(which, len(args) + 1, argv)
+        with open(os.path.join(cwd, "_stdout.txt"), "rb") as fh:
+            out = fh.read()
+        os.remove(os.path.join(cwd, "_stdout.txt"))
+        return rc, out
+
     def obstacle_geometry(self):
         ini = np.zeros(22 * 12, np.int32)
         n = self.lib.ref_obstacle_geometry(ini)
         return ini, n
+
+
+class _InDir:
+    def __init__(self, d):
+        self.d = d
+
+    def __enter__(self):
+        self.old = os.getcwd()
+        os.chdir(self.d)
+
+    def __exit__(self, *exc):
+        os.chdir(self.old)
+
+
+class _Stdout:
+    """Redirect file descriptor 1 (C printf / std::cout) to a file."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        import sys
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        self.fd = os.open(self.path, os.O_WRONLY | os.O_CREAT | os.O_TRUNC, 0o644)
+        os.dup2(self.fd, 1)
+
+    def __exit__(self, *exc):
+        _libc_fflush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        os.close(self.fd)
+
+
+def _libc_fflush():
+    try:
+        C.CDLL(None).fflush(None)
+    except Exception:
+        pass
 
 
 def _s(q: int) -> str:

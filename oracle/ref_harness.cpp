@@ -23,6 +23,9 @@
 #define main ref_main_double_shear
 #include "main_double-shear-layer.cpp"
 #undef main
+#define main ref_main_error_calculator
+#include "error_calculator.cpp"
+#undef main
 #else
 #define nvel 19
 #define dim 3
@@ -330,6 +333,58 @@ void ref_obstacle_map_points(double Ma, int n_tau, int n_nu, const int *ii,
     }
 }
 
+// The obstacle geometry as geometria_obstaculo builds it (numbered ini).
+// (declared below, D3Q19 only)
+#endif
+
+// rec_velocity / rec_vorticity of inif, written into the current directory
+// under the names the reference gives them.
+void RSFX(ref_rec_velocity)(int nx, int ny, int nz, const int *mask, double *inif, double tempo)
+{
+    RefLattice r;
+    build(r, nx, ny, nz, mask);
+    r.l.inif = inif;
+    rec_velocity(r.g, r.l, tempo);
+    release(r);
+}
+
+void RSFX(ref_rec_vorticity)(int nx, int ny, int nz, const int *mask, double *inif, unsigned int passo)
+{
+    RefLattice r;
+    build(r, nx, ny, nz, mask);
+    r.l.inif = inif;
+    rec_vorticity(r.g, r.l, passo);
+    release(r);
+}
+
+double RSFX(ref_enstrophy)(int nx, int ny, int nz, const int *mask, double *inif)
+{
+    RefLattice r;
+    build(r, nx, ny, nz, mask);
+    r.l.inif = inif;
+    const double e = calc_enstrophy(r.g, r.l);
+    release(r);
+    return e;
+}
+
+// A driver's main() with the given arguments (argv[0] included).
+// D2Q9 library: 0 main_taylor-green, 1 main_double-shear-layer,
+// 2 tools/error_calculator; D3Q19: 0 main_stability-map_taylor-green,
+// 1 main_stability-map_obstacle.
+int RSFX(ref_driver_main)(int which, int argc, char **argv)
+{
+#if LATT == 9
+    if (which == 0) return ref_main_taylor_green(argc, argv);
+    if (which == 1) return ref_main_double_shear(argc, argv);
+    if (which == 2) return ref_main_error_calculator(argc, argv);
+#else
+    if (which == 0) return ref_main_tg_map(argc, argv);
+    if (which == 1) return ref_main_obstacle_map(argc, argv);
+#endif
+    return -1;
+}
+
+#if LATT != 9
 // The obstacle geometry as geometria_obstaculo builds it (numbered ini).
 int ref_obstacle_geometry(int *ini_out)
 {

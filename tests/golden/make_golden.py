@@ -168,6 +168,90 @@ def trajectories():
     np.savez_compressed(os.path.join(HERE, "ref_trajectories.npz"), **data)
 
 
+def outputs():
+    """The reference's output files and monitors: rec_velocity /
+    rec_vorticity / calc_enstrophy on seeded non-equilibrium lattices with
+    solids; the double-shear driver's whole output directory at L = 16 and
+    32 (to t* = 0.125, output every 1/16); the error calculator on those two
+    runs; the Taylor-Green driver's printed line in both of its modes; the two
+    map drivers' files on 4 x 3 and 3 x 4 grids.  File contents are stored as
+    uint8 arrays."""
+    import tempfile
+    rng = np.random.default_rng(SEED + 7)
+    o = O.Oracle()
+    data = {}
+    b = lambda x: np.frombuffer(x, np.uint8)  # noqa: E731
+    for q, dims in ((9, (12, 10, 1)), (19, (6, 5, 4))):
+        r = O.Reference(q)
+        n = int(np.prod(dims))
+        m = (rng.uniform(size=n) > 0.2).astype(np.int32)
+        m[0] = 1
+        nf = int(m.sum())
+        f = perturbed_sites(q, nf, rng, o)
+        data[f"q{q}_mask"], data[f"q{q}_f"] = m, f
+        with tempfile.TemporaryDirectory() as d:
+            data[f"q{q}_vel"] = b(r.rec_velocity(*dims, m, f, 0.25, d))
+            data[f"q{q}_vor"] = b(r.rec_vorticity(*dims, m, f, 7, d))
+        data[f"q{q}_enstrophy"] = np.array([r.enstrophy(*dims, m, f)])
+    r9 = O.Reference(9)
+    with tempfile.TemporaryDirectory() as d:
+        names = []
+        for L in (16, 32):
+            sub = os.path.join(d, f"L{L}")
+            os.makedirs(sub)
+            args = ["--L", str(L), "--t-final", "0.125", "--dt-out", "0.0625", "--threads", "4"]
+            rc, out = r9.driver_main(1, args, sub)
+            assert rc == 0, out
+            for fn in sorted(os.listdir(sub)):
+                with open(os.path.join(sub, fn), "rb") as fh:
+                    data[f"dsl{L}/{fn}"] = b(fh.read())
+                names.append(f"dsl{L}/{fn}")
+        rc, out = r9.driver_main(2, [os.path.join(d, "L16", "vel_0.125000.vtk"),
+                                     os.path.join(d, "L32", "vel_0.125000.vtk")], d)
+        assert rc == 0, out
+        with open(os.path.join(d, "erro_L2.dat"), "rb") as fh:
+            data["error_calculator/erro_L2.dat"] = b(fh.read())
+        data["dsl_files"] = np.array(names)
+        for tag, args in (("tg_default", ["--L", "32", "--collision", "erlbm"]),
+                          ("tg_tau", ["--L", "32", "--tau", "0.6", "--u0", "0.001", "--collision", "reg",
+                                      "--init", "eq"])):
+            rc, out = r9.driver_main(0, args, d)
+            assert rc == 0
+            data[f"{tag}_line"] = b(out)
+    r19 = O.Reference(19)
+    with tempfile.TemporaryDirectory() as d:
+        rc, _ = r19.driver_main(0, ["--n-ma", "4", "--n-nu", "3", "--collision", "erlbm", "--tau-nh", "0.65",
+                                    "--out", "tg.vtk", "--threads", "8"], d)
+        assert rc == 0
+        rc, _ = r19.driver_main(1, ["--n-tau", "3", "--n-nu", "4", "--out", "obst.vtk", "--threads", "8"], d)
+        assert rc == 0
+        for fn in ("tg.vtk", "obst.vtk"):
+            with open(os.path.join(d, fn), "rb") as fh:
+                data[f"map/{fn}"] = b(fh.read())
+    np.savez_compressed(os.path.join(HERE, "ref_outputs.npz"), **data)
+
+
+def kappa4_strict():
+    """scripts/run_kappa4_check.sh's 96 runs of the Taylor-Green driver with
+    the strict (-ffp-contract=off) reference build; the published table came
+    from the FMA-contracted build and differs in the sixth digit of a few
+    near-zero errors."""
+    import tempfile
+    r9 = O.Reference(9)
+    rows = ["model,init,L,tau,tau_nh,U0,steps,E_vector,E_magnitude,rel_visc_error"]
+    with tempfile.TemporaryDirectory() as d:
+        for tau in ("0.505", "0.51", "0.52", "0.535", "0.55", "0.575", "0.6", "0.625", "0.65", "0.7", "0.75", "0.8",
+                    "0.9", "1.0", "1.1", "1.2"):
+            for name, col, tnh in (("BGK", "bgk", tau), ("REG", "reg", "1.0"), ("ERLBM065", "erlbm", "0.65")):
+                for init in ("eq", "neq"):
+                    rc, out = r9.driver_main(0, ["--L", "64", "--tau", tau, "--u0", "0.001", "--collision", col,
+                                                 "--tau-nh", tnh, "--init", init, "--threads", "8"], d)
+                    assert rc == 0
+                    rows.append(f"{name},{init}," + out.decode().strip().replace(" ", ","))
+    with open(os.path.join(HERE, "kappa4_check_L64_strict.csv"), "w") as fh:
+        fh.write("\n".join(rows) + "\n")
+
+
 def map_subsamples():
     """Classifications by the reference's roda_ponto on the README's 20 x 20
     subsample positions of the 400 x 400 maps (i = 20k + 19, j = 20k)."""
@@ -187,12 +271,21 @@ def map_subsamples():
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--maps", action="store_true", help="also rerun the 20x20 map subsamples (minutes)")
+    ap.add_argument("--only", choices=["outputs", "kappa4"], help="write only this group of fixtures")
     a = ap.parse_args()
     if not (O.Reference.available(9) and O.Reference.available(19, fma=True)):
         O.build(ref=True)
+    if a.only == "outputs":
+        outputs()
+        sys.exit(0)
+    if a.only == "kappa4":
+        kappa4_strict()
+        sys.exit(0)
     published()
     per_function()
     trajectories()
+    outputs()
+    kappa4_strict()
     if a.maps:
         map_subsamples()
     print("golden fixtures written to", HERE)
