@@ -104,6 +104,65 @@ __device__ __forceinline__ void feq_exact(double rho, double vx, double vy, doub
     });
 }
 
+// coll_MRT, D2Q9 (Collision_operators.cpp:611-701): moments m = M f and
+// m_eq = M f_eq, m <- m - s (m - m_eq) + (1 - s/2) F_m with F_m = +0 at zero
+// force, f = M^-1 m; M and M^-1 of :449-471, rates of :614-622.  Entries
+// of M are small integers, so M f is formed with exact +-f, 2f, 4f terms;
+// zero entries of M and M^-1 are skipped (a +-0 term added to a running sum
+// that starts at +0.0 never changes it).
+struct MrtD2Q9 {
+    static constexpr int M[9][9] = {
+        {1, 1, 1, 1, 1, 1, 1, 1, 1},     {-4, -1, -1, -1, -1, 2, 2, 2, 2}, {4, -2, -2, -2, -2, 1, 1, 1, 1},
+        {0, 1, -1, 0, 0, 1, -1, 1, -1},  {0, -2, 2, 0, 0, 1, -1, 1, -1},   {0, 0, 0, 1, -1, 1, -1, -1, 1},
+        {0, 0, 0, -2, 2, 1, -1, -1, 1},  {0, 1, 1, -1, -1, 0, 0, 0, 0},    {0, 0, 0, 0, 0, 1, 1, -1, -1}};
+    static constexpr double MI[9][9] = {
+        {1. / 9., -1. / 9., 1. / 9., 0., 0., 0., 0., 0., 0.},
+        {1. / 9., -1. / 36., -1. / 18., 1. / 6., -1. / 6., 0., 0., 1. / 4., 0.},
+        {1. / 9., -1. / 36., -1. / 18., -1. / 6., 1. / 6., 0., 0., 1. / 4., 0.},
+        {1. / 9., -1. / 36., -1. / 18., 0., 0., 1. / 6., -1. / 6., -1. / 4., 0.},
+        {1. / 9., -1. / 36., -1. / 18., 0., 0., -1. / 6., 1. / 6., -1. / 4., 0.},
+        {1. / 9., 1. / 18., 1. / 36., 1. / 6., 1. / 12., 1. / 6., 1. / 12., 0., 1. / 4.},
+        {1. / 9., 1. / 18., 1. / 36., -1. / 6., -1. / 12., -1. / 6., -1. / 12., 0., 1. / 4.},
+        {1. / 9., 1. / 18., 1. / 36., 1. / 6., 1. / 12., -1. / 6., -1. / 12., 0., -1. / 4.},
+        {1. / 9., 1. / 18., 1. / 36., -1. / 6., -1. / 12., 1. / 6., 1. / 12., 0., -1. / 4.}};
+    static constexpr double S_FIXED[9] = {0.0, 1.6, 1.8, 0.0, 1.9, 0.0, 1.9, -1.0, -1.0};   // -1: 1/tau
+};
+
+__device__ __forceinline__ void collide_mrt_d2q9(double (&g)[9], const double (&feq)[9], double oot)
+{
+    using T = MrtD2Q9;
+    double m[9];
+    static_for<9>([&](auto R) {
+        constexpr int r = R;
+        double a = 0.0, e = 0.0;
+        static_for<9>([&](auto Cc) {
+            constexpr int c = Cc;
+            constexpr int k = T::M[r][c];
+            if constexpr (k == 1) {
+                a = a + g[c];
+                e = e + feq[c];
+            } else if constexpr (k == -1) {
+                a = a - g[c];
+                e = e - feq[c];
+            } else if constexpr (k != 0) {
+                a = a + (double)k * g[c];
+                e = e + (double)k * feq[c];
+            }
+        });
+        const double s = T::S_FIXED[r] < 0.0 ? oot : T::S_FIXED[r];
+        m[r] = (a - s * (a - e)) + (1.0 - 0.5 * s) * 0.0;
+    });
+    static_for<9>([&](auto R) {
+        constexpr int r = R;
+        double a = 0.0;
+        static_for<9>([&](auto Cc) {
+            constexpr int c = Cc;
+            if constexpr (T::MI[r][c] != 0.0) a = a + T::MI[r][c] * m[c];
+        });
+        g[r] = a;
+    });
+}
+
 // One collision in place.  Returns nothing; g holds f* afterwards.
 // vx, vy, vz receive the pre-collision velocity (for monitors).
 template <int Q, int COL>
@@ -121,6 +180,10 @@ __device__ __forceinline__ void collide_exact(double (&g)[Q], const CollideConst
             constexpr int i = I;
             g[i] = g[i] + (feq[i] - g[i]) * k.oot;
         });
+        return;
+    } else if constexpr (COL == COL_MRT) {
+        static_assert(Q == 9, "coll_MRT exists for D2Q9 only");
+        collide_mrt_d2q9(g, feq, k.oot);
         return;
     } else {
         double d[Q];

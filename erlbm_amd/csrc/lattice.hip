@@ -519,6 +519,9 @@ int launch_step(erlbm_lattice *h, const StepArgs &a, bool stream)
     case ERLBM_COL_BGK: return launch_step_q<Q, COL_BGK>(h, a, stream);
     case ERLBM_COL_REG: return launch_step_q<Q, COL_REG>(h, a, stream);
     case ERLBM_COL_EXT_REG: return launch_step_q<Q, COL_EXT_REG>(h, a, stream);
+    case ERLBM_COL_MRT:
+        if constexpr (Q == 9) return launch_step_q<Q, COL_MRT>(h, a, stream);
+        [[fallthrough]];
     default: set_last_error_msg("collision operator not supported on this path"); return ERLBM_ERR_UNSUPPORTED;
     }
 }
@@ -679,9 +682,15 @@ extern "C" int erlbm_lattice_fluid_count(const erlbm_lattice *h, int *n_fluid)
 extern "C" int erlbm_set_params(erlbm_lattice *h, double tau, double tau_nh, double rho0, int collision)
 {
     if (!h) return ERLBM_ERR_ARG;
-    if (collision != ERLBM_COL_BGK && collision != ERLBM_COL_REG && collision != ERLBM_COL_EXT_REG) {
-        set_last_error_msg("erlbm_set_params: collision must be BGK, REG or EXT_REG");
-        return collision == ERLBM_COL_MRT ? ERLBM_ERR_UNSUPPORTED : ERLBM_ERR_ARG;
+    if (collision != ERLBM_COL_BGK && collision != ERLBM_COL_REG && collision != ERLBM_COL_EXT_REG &&
+        collision != ERLBM_COL_MRT) {
+        set_last_error_msg("erlbm_set_params: collision must be BGK, MRT, REG or EXT_REG");
+        return ERLBM_ERR_ARG;
+    }
+    if (collision == ERLBM_COL_MRT && (h->q != 9 || h->f32_arith)) {
+        // the reference defines coll_MRT for D2Q9 only (src/erlbm_common.h:75-77)
+        set_last_error_msg("erlbm_set_params: MRT is defined for D2Q9 (FP64 arithmetic) only");
+        return ERLBM_ERR_UNSUPPORTED;
     }
     if (!(tau > 0.0) || !(tau_nh > 0.0)) {
         set_last_error_msg("erlbm_set_params: tau and tau_nh must be positive");

@@ -294,7 +294,11 @@ void step_qcta(erlbm_lattice *h, const LPArgs &l, bool stream)
 template <int Q, int COL, typename T>
 int step_qct(erlbm_lattice *h, const LPArgs &l, bool stream)
 {
-    if constexpr (std::is_same<T, float>::value) {
+    if constexpr (COL == COL_MRT) {
+        // the FP32-arithmetic variant has no MRT form; FP64 arithmetic only
+        if (h->f32_arith) return ERLBM_ERR_UNSUPPORTED;
+        step_qcta<Q, COL, T, false>(h, l, stream);
+    } else if constexpr (std::is_same<T, float>::value) {
         if (h->f32_arith) step_qcta<Q, COL, T, true>(h, l, stream);
         else step_qcta<Q, COL, T, false>(h, l, stream);
     } else {
@@ -311,6 +315,9 @@ int step_qt(erlbm_lattice *h, const LPArgs &l, bool stream)
     case ERLBM_COL_BGK: return step_qct<Q, COL_BGK, T>(h, l, stream);
     case ERLBM_COL_REG: return step_qct<Q, COL_REG, T>(h, l, stream);
     case ERLBM_COL_EXT_REG: return step_qct<Q, COL_EXT_REG, T>(h, l, stream);
+    case ERLBM_COL_MRT:
+        if constexpr (Q == 9) return step_qct<Q, COL_MRT, T>(h, l, stream);
+        return ERLBM_ERR_UNSUPPORTED;
     default: return ERLBM_ERR_UNSUPPORTED;
     }
 }

@@ -168,6 +168,12 @@ int launch_qt(erlbm_lattice *h, const VecArgs &v, bool stream)
     case ERLBM_COL_BGK: launch_qct<Q, COL_BGK, T>(h, v, stream); break;
     case ERLBM_COL_REG: launch_qct<Q, COL_REG, T>(h, v, stream); break;
     case ERLBM_COL_EXT_REG: launch_qct<Q, COL_EXT_REG, T>(h, v, stream); break;
+    case ERLBM_COL_MRT:
+        if constexpr (Q == 9) {
+            launch_qct<Q, COL_MRT, T>(h, v, stream);
+            break;
+        }
+        return ERLBM_ERR_UNSUPPORTED;
     default: return ERLBM_ERR_UNSUPPORTED;
     }
     ERLBM_HIP_TRY(hipGetLastError());

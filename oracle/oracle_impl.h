@@ -278,6 +278,70 @@ static inline void SFX(coll_bgk)(double *f, double ax, double ay, double az,
     }
 }
 
+#if NV == 9
+/* ---- coll_MRT, D2Q9 branch: Collision_operators.cpp:611-701, with the
+ * moment matrix and its inverse of :449-471 (moments rho, e, epsilon, jx,
+ * qx, jy, qy, pxx, pxy over the velocity order of def_lattice_d2q9) and the
+ * relaxation rates of :614-622.  The reference dispatches MRT for D2Q9
+ * only (src/erlbm_common.h:75-77).                                          */
+static const double SFX(MRT_M)[9][9] = {
+    {1., 1., 1., 1., 1., 1., 1., 1., 1.},
+    {-4., -1., -1., -1., -1., 2., 2., 2., 2.},
+    {4., -2., -2., -2., -2., 1., 1., 1., 1.},
+    {0., 1., -1., 0., 0., 1., -1., 1., -1.},
+    {0., -2., 2., 0., 0., 1., -1., 1., -1.},
+    {0., 0., 0., 1., -1., 1., -1., -1., 1.},
+    {0., 0., 0., -2., 2., 1., -1., -1., 1.},
+    {0., 1., 1., -1., -1., 0., 0., 0., 0.},
+    {0., 0., 0., 0., 0., 1., 1., -1., -1.}};
+static const double SFX(MRT_MI)[9][9] = {
+    {1. / 9., -1. / 9., 1. / 9., 0., 0., 0., 0., 0., 0.},
+    {1. / 9., -1. / 36., -1. / 18., 1. / 6., -1. / 6., 0., 0., 1. / 4., 0.},
+    {1. / 9., -1. / 36., -1. / 18., -1. / 6., 1. / 6., 0., 0., 1. / 4., 0.},
+    {1. / 9., -1. / 36., -1. / 18., 0., 0., 1. / 6., -1. / 6., -1. / 4., 0.},
+    {1. / 9., -1. / 36., -1. / 18., 0., 0., -1. / 6., 1. / 6., -1. / 4., 0.},
+    {1. / 9., 1. / 18., 1. / 36., 1. / 6., 1. / 12., 1. / 6., 1. / 12., 0., 1. / 4.},
+    {1. / 9., 1. / 18., 1. / 36., -1. / 6., -1. / 12., -1. / 6., -1. / 12., 0., 1. / 4.},
+    {1. / 9., 1. / 18., 1. / 36., 1. / 6., 1. / 12., -1. / 6., -1. / 12., 0., -1. / 4.},
+    {1. / 9., 1. / 18., 1. / 36., -1. / 6., -1. / 12., 1. / 6., 1. / 12., 0., -1. / 4.}};
+
+static inline void SFX(coll_mrt)(double *f, double ax, double ay, double az, double tau)
+{
+    const double s[9] = {0.0, 1.6, 1.8, 0.0, 1.9, 0.0, 1.9, 1. / tau, 1. / tau};
+    double f_eq[9], vx, vy, vz, rho;
+    SFX(calcula)(f, &vx, &vy, &vz, &rho);
+    SFX(dist_eq)(f_eq, vx, vy, vz, rho);
+    double S[9] = {0.0};
+    if (ax != 0.0 || ay != 0.0 || az != 0.0) {
+        const double ocs2 = 3.;
+        double v[2] = {vx, vy};
+        double F[2] = {ax * rho, ay * rho};
+        const double vF = SFX(dot)(F, v);
+        for (int i = 0; i < 9; i++) {
+            const double cF = SFX(dot)(&SFX(C)[i * 2], F);
+            const double cv = SFX(dot)(&SFX(C)[i * 2], v);
+            S[i] = SFX(W)[i] * ocs2 * (cF + ocs2 * cF * cv - vF);
+        }
+    }
+    double m[9], meq[9], fm[9];
+    for (int r = 0; r < 9; r++) {
+        m[r] = 0.0;
+        meq[r] = 0.0;
+        fm[r] = 0.0;
+        for (int c = 0; c < 9; c++) {
+            m[r] = m[r] + SFX(MRT_M)[r][c] * f[c];
+            meq[r] = meq[r] + SFX(MRT_M)[r][c] * f_eq[c];
+            fm[r] = fm[r] + SFX(MRT_M)[r][c] * S[c];
+        }
+    }
+    for (int r = 0; r < 9; r++) m[r] = m[r] - s[r] * (m[r] - meq[r]) + (1.0 - 0.5 * s[r]) * fm[r];
+    for (int r = 0; r < 9; r++) {
+        f[r] = 0.0;
+        for (int c = 0; c < 9; c++) f[r] = f[r] + SFX(MRT_MI)[r][c] * m[c];
+    }
+}
+#endif
+
 /* ---- colisao_sitio dispatch, src/erlbm_common.h:68-81 --------------------- */
 static inline void SFX(collide)(double *f, double ax, double ay, double az,
                                 int col, double tau, double tau_nh)
@@ -286,6 +350,9 @@ static inline void SFX(collide)(double *f, double ax, double ay, double az,
     case ORC_COL_BGK: SFX(coll_bgk)(f, ax, ay, az, tau); break;
     case ORC_COL_REG: SFX(coll_reg)(f, ax, ay, az, tau); break;
     case ORC_COL_EXT_REG: SFX(coll_ext_reg)(f, ax, ay, az, tau, tau_nh); break;
+#if NV == 9
+    case ORC_COL_MRT: SFX(coll_mrt)(f, ax, ay, az, tau); break;
+#endif
     default: break;
     }
 }

@@ -71,7 +71,8 @@ def per_function():
         rs, rf = O.Reference(q, fma=False), O.Reference(q, fma=True)
         f = perturbed_sites(q, 64, rng, o)
         data[f"sites_q{q}"] = f
-        for col in (O.COL_BGK, O.COL_REG, O.COL_EXT_REG):
+        # coll_MRT exists for D2Q9 only (src/erlbm_common.h:75-77)
+        for col in (O.COL_BGK, O.COL_REG, O.COL_EXT_REG) + ((O.COL_MRT,) if q == 9 else ()):
             for tag, acc in (("", (0.0, 0.0, 0.0)), ("_forced", (1e-4, -2e-4, 3e-5 if q == 19 else 0.0))):
                 data[f"coll_q{q}_c{col}{tag}_strict"] = rs.collide(f, col, 0.62, 0.65, acc)
                 data[f"coll_q{q}_c{col}{tag}_fma"] = rf.collide(f, col, 0.62, 0.65, acc)
@@ -271,10 +272,13 @@ def map_subsamples():
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--maps", action="store_true", help="also rerun the 20x20 map subsamples (minutes)")
-    ap.add_argument("--only", choices=["outputs", "kappa4"], help="write only this group of fixtures")
+    ap.add_argument("--only", choices=["functions", "outputs", "kappa4"], help="write only this group of fixtures")
     a = ap.parse_args()
     if not (O.Reference.available(9) and O.Reference.available(19, fma=True)):
         O.build(ref=True)
+    if a.only == "functions":
+        per_function()
+        sys.exit(0)
     if a.only == "outputs":
         outputs()
         sys.exit(0)

@@ -26,12 +26,12 @@ def _periodic(gpu, q, L, tau, tnh, col, f0, nz=1):
     return lat
 
 
-@pytest.mark.parametrize("col", [O.COL_BGK, O.COL_REG, O.COL_EXT_REG])
+@pytest.mark.parametrize("col", [O.COL_BGK, O.COL_MRT, O.COL_REG, O.COL_EXT_REG])
 def test_d2q9_double_shear_bitwise(gpu, orc, col):
     L, n = 64, 200
     print(f"d2q9 dsl L={L} col={col} steps={n}", flush=True)
     tau = 0.5 + 3. * (1. / 32.) * L / 80000.
-    tnh = {O.COL_BGK: tau, O.COL_REG: 1.0, O.COL_EXT_REG: 0.65}[col]
+    tnh = {O.COL_BGK: tau, O.COL_MRT: 0.65, O.COL_REG: 1.0, O.COL_EXT_REG: 0.65}[col]
     f0 = gpu.init_double_shear(L)
     ini, st, nf = orc.stream_map(9, L, L, 1)
     want = orc.run_steps(9, f0, st, n, col, tau, tnh, nthreads=4)
@@ -231,3 +231,39 @@ def test_autotune_leaves_the_run_alone(gpu, orc, q, L, prec):
         assert np.array_equal(a.download(), b.download())
         if prec == "FP64":
             assert np.array_equal(a.download(), orc.run_steps(q, f0, st, 20, O.COL_EXT_REG, tau, 0.65))
+
+
+@pytest.mark.parametrize("vec", [0, 1])
+def test_mrt_channel_and_vector_kernel(gpu, orc, golden, monkeypatch, vec):
+    """coll_MRT (D2Q9) through the boundary-column kernel (Zou-He inlet,
+    outlet rescale, bounce-back) and the periodic kernels: bitwise the
+    oracle's MRT, which is pinned to the reference's coll_MRT."""
+    monkeypatch.setenv("ERLBM_VEC", str(vec))
+    g = golden("ref_trajectories.npz")
+    m = g["ch9_mask"]
+    ini, st, nf = orc.stream_map(9, 24, 10, 1, m)
+    want, n, v = orc.run_bc_steps(9, g["ch9_in"], 24, 10, 1, ini, st, 300, O.COL_MRT, 0.55, 0.7, inlet_x=1,
+                                  u_in=0.05, outlet_x=22, rho_out=1.0, dens_limit=10.0)
+    with gpu.Lattice(9, 24, 10, 1, mask=m) as lat:
+        lat.set_params(0.55, 0.7, 1.0, O.COL_MRT)
+        lat.set_zouhe_inlet(1, 0.05)
+        lat.set_outlet_rescale(22, 1.0)
+        lat.set_density_check(10.0)
+        lat.upload(g["ch9_in"])
+        lat.step(300)
+        assert lat.status() == (n, v)
+        assert np.array_equal(lat.download(), want)
+    L = 96
+    tau = 0.51
+    f0 = gpu.init_double_shear(L)
+    ini, st, nf = orc.stream_map(9, L, L, 1)
+    want = orc.run_steps(9, f0, st, 50, O.COL_MRT, tau, 0.65)
+    with _periodic(gpu, 9, L, tau, 0.65, O.COL_MRT, f0) as lat:
+        lat.step(50)
+        assert np.array_equal(lat.download(), want)
+
+
+def test_mrt_is_d2q9_only(gpu):
+    with gpu.Lattice(19, 8, 8, 2) as lat:
+        with pytest.raises(RuntimeError, match="MRT"):
+            lat.set_params(0.6, 0.65, 1.0, O.COL_MRT)

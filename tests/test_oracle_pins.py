@@ -127,6 +127,62 @@ def test_linearised_collision_matrix(orc, q):
     np.testing.assert_allclose(J, _linear_operator(c, w, tau, tnh), atol=1e-8)
 
 
+# ---------------------------------------------------------------- coll_MRT (D2Q9)
+MRT_M = np.array([[1, 1, 1, 1, 1, 1, 1, 1, 1], [-4, -1, -1, -1, -1, 2, 2, 2, 2], [4, -2, -2, -2, -2, 1, 1, 1, 1],
+                  [0, 1, -1, 0, 0, 1, -1, 1, -1], [0, -2, 2, 0, 0, 1, -1, 1, -1], [0, 0, 0, 1, -1, 1, -1, -1, 1],
+                  [0, 0, 0, -2, 2, 1, -1, -1, 1], [0, 1, 1, -1, -1, 0, 0, 0, 0], [0, 0, 0, 0, 0, 1, 1, -1, -1]], float)
+
+
+@pytest.mark.parametrize("forced", [False, True])
+def test_mrt_bitwise_vs_reference_fixture(orc, golden, forced):
+    g = golden("ref_functions.npz")
+    f = g["sites_q9"]
+    acc = (1e-4, -2e-4, 0.0) if forced else (0.0, 0.0, 0.0)
+    tag = "_forced" if forced else ""
+    out = orc.collide(f, O.COL_MRT, 0.62, 0.65, acc)
+    assert np.array_equal(out, g[f"coll_q9_c{O.COL_MRT}{tag}_strict"])
+    fma = g[f"coll_q9_c{O.COL_MRT}{tag}_fma"]
+    assert np.max(np.abs(out - fma) / np.abs(fma)) < 1e-13
+    # tau_nh plays no part; a different tau changes the result
+    assert np.array_equal(orc.collide(f, O.COL_MRT, 0.62, 0.9, acc), out)
+    assert not np.allclose(orc.collide(f, O.COL_MRT, 0.63, 0.65, acc), out, rtol=1e-12, atol=0)
+
+
+def test_mrt_closed_forms(orc):
+    """The moment matrix is the Gram-Schmidt basis of d'Humieres / Lallemand
+    and Luo over the D2Q9 velocities (rows = 1, e, eps, jx, qx, jy, qy, pxx,
+    pxy as polynomials in c); the linearisation at rest is
+    M^-1 [(I - S) M + S M E] with E the linear equilibrium; mass and momentum
+    are conserved."""
+    c, w, _, _ = orc.lattice(9)
+    cx, cy = c[:, 0], c[:, 1]
+    c2 = cx * cx + cy * cy
+    basis = np.array([np.ones(9), 3 * c2 - 4, (9 * c2 * c2 - 21 * c2 + 8) / 2, cx, (3 * c2 - 5) * cx, cy,
+                      (3 * c2 - 5) * cy, cx * cx - cy * cy, cx * cy])
+    assert np.array_equal(basis, MRT_M)
+    G = MRT_M @ MRT_M.T
+    assert np.array_equal(G, np.diag(np.diag(G)))                 # rows mutually orthogonal
+    tau = 0.62
+    S = np.diag([0.0, 1.6, 1.8, 0.0, 1.9, 0.0, 1.9, 1 / tau, 1 / tau])
+    E = w[:, None] * (1 + 3 * c @ c.T)
+    Minv = np.linalg.inv(MRT_M)
+    want = Minv @ ((np.eye(9) - S) @ MRT_M + S @ MRT_M @ E)
+    rest = orc.dist_eq(9, 0, 0, 0, 1.0)
+    eps = 1e-6
+    J = np.zeros((9, 9))
+    for j in range(9):
+        p = rest.copy(); p[j] += eps
+        m = rest.copy(); m[j] -= eps
+        J[:, j] = (orc.collide(p[None], O.COL_MRT, tau, 1.0)[0] - orc.collide(m[None], O.COL_MRT, tau, 1.0)[0]) / (2 * eps)
+    np.testing.assert_allclose(J, want, atol=1e-8)
+    rng = np.random.default_rng(3)
+    feq = orc.dist_eq(9, 0.02, -0.03, 0.0, 1.01)
+    f = feq[None] * (1 + 1e-3 * rng.uniform(-1, 1, (32, 9)))
+    out = orc.collide(f, O.COL_MRT, tau, 1.0)
+    np.testing.assert_allclose(out.sum(1), f.sum(1), rtol=1e-14)
+    np.testing.assert_allclose(out @ c, f @ c, rtol=0, atol=1e-15)
+
+
 # ---------------------------------------------------------------- streaming
 @pytest.mark.parametrize("q", [9, 19])
 def test_stream_maps_vs_fixture(orc, golden, q):
