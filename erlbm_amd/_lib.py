@@ -115,7 +115,7 @@ SIGNATURES = {
     "erlbm_velocity_l2_error": (_i, [_vp, _i, _i, _vp, _i, _i, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "erlbm_write_tg_map_vtk": (_i, [C.c_char_p, _i, _i, _i, C.c_double, _vp]),
     "erlbm_write_obstacle_map_vtk": (_i, [C.c_char_p, _i, _i, C.c_double, _vp]),
-    "erlbm_batch_autotune": (_i, [_vp, _vp, _i, C.c_longlong, _vp, _vp, _vp, _vp]),
+    "erlbm_batch_autotune": (_i, [_vp, _vp, _i, C.c_longlong, _vp, _vp, _vp]),
     "erlbm_tg_map_jobs": (_i, [_i, _i, _i, _d, _i, _i, _i, _i, _i, _vp]),
     "erlbm_obstacle_map_jobs": (_i, [_i, _i, _d, _i, _i, _i, _i, _i, _vp]),
     "erlbm_dist_eq": (_i, [_i, _d, _d, _d, _d, _dp]),

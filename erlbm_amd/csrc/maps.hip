@@ -917,11 +917,15 @@ int occupancy_max(const erlbm_batch *b, int precision, int col)
 }  // namespace
 
 extern "C" int erlbm_batch_autotune(erlbm_batch *b, const erlbm_job *sample, int n_sample, long long max_steps,
-                                    int *wg_per_cu_f64, int *wg_per_cu_f32, int *chunk_steps, double *points_per_s)
+                                    int *wg_per_cu_f64, int *wg_per_cu_f32, double *points_per_s)
 {
     if (!b || !sample || n_sample < 1 || max_steps < 1) return ERLBM_ERR_ARG;
-    std::vector<erlbm_job> all(sample, sample + n_sample), cls[2];
-    for (auto &j : all) {
+    // Every job of the sample gets the same budget (max_steps, or less if
+    // its own is shorter), so the time measures the per-step throughput of
+    // a launch shape rather than the sample's longest job.
+    std::vector<erlbm_job> cls[2];
+    for (int k = 0; k < n_sample; k++) {
+        erlbm_job j = sample[k];
         if (j.n_steps > max_steps) j.n_steps = max_steps;
         cls[resolve_precision(j) == ERLBM_FP64 ? 0 : 1].push_back(j);
     }
@@ -935,8 +939,9 @@ extern "C" int erlbm_batch_autotune(erlbm_batch *b, const erlbm_job *sample, int
     };
     const int chunk0 = b->chunk_steps;
     b->chunk_steps = 0;
+    double total = 0.0;
     // workgroups per CU, per precision class: 1 .. the occupancy limit
-    for (int c = 0; c < 2; c++) {
+    for (int c = 0; c < 2 && rc == ERLBM_OK; c++) {
         if (cls[c].empty()) continue;
         const int maxpc = occupancy_max(b, resolve_precision(cls[c][0]), cls[c][0].collision);
         if (timed(cls[c]) < 0) break;   // warm-up (buffer growth)
@@ -952,29 +957,13 @@ extern "C" int erlbm_batch_autotune(erlbm_batch *b, const erlbm_job *sample, int
             }
         }
         b->wg_per_cu[c] = best_v;
-        if (rc) break;
+        total += best;
     }
-    // run to retirement in one launch, or graph-replayed chunks with compaction
-    double best = 1e300;
-    int best_chunk = 0;
-    if (rc == ERLBM_OK) {
-        for (int ch : {0, 100, 1000}) {
-            b->chunk_steps = ch;
-            timed(all);   // warm-up (saved-state buffer)
-            const double t = std::min(timed(all), timed(all));
-            if (rc) break;
-            if (t < best) {
-                best = t;
-                best_chunk = ch;
-            }
-        }
-    }
-    b->chunk_steps = rc ? chunk0 : best_chunk;
+    b->chunk_steps = chunk0;
     if (rc) return rc;
     if (wg_per_cu_f64) *wg_per_cu_f64 = b->wg_per_cu[0];
     if (wg_per_cu_f32) *wg_per_cu_f32 = b->wg_per_cu[1];
-    if (chunk_steps) *chunk_steps = best_chunk;
-    if (points_per_s) *points_per_s = n_sample / best;
+    if (points_per_s) *points_per_s = total > 0 ? n_sample / total : 0.0;
     return ERLBM_OK;
 }
 

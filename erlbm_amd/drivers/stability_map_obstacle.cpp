@@ -37,11 +37,11 @@ int main(int argc, char **argv)
     erlbm_batch *b = nullptr;
     drv::check(erlbm_batch_create(22, 12, mask, 1, 20, nullptr, &b), "erlbm_batch_create");
     if (tune) {
-        int w64 = 0, w32 = 0, ch = 0;
+        int w64 = 0, w32 = 0;
         double pps = 0.0;
-        drv::check(erlbm_batch_autotune(b, jobs.data(), (int)std::min<size_t>(n, 4096), 2000, &w64, &w32, &ch, &pps),
+        drv::check(erlbm_batch_autotune(b, jobs.data(), (int)std::min<size_t>(n, 4096), 2000, &w64, &w32, &pps),
                    "erlbm_batch_autotune");
-        std::printf("autotune: %d / %d workgroups per CU (FP64 / FP32), chunk %d steps\n", w64, w32, ch);
+        std::printf("autotune: %d / %d workgroups per CU (FP64 / FP32)\n", w64, w32);
     }
     const auto t0 = std::chrono::steady_clock::now();
     drv::check(erlbm_batch_run(b, jobs.data(), (int)n, res.data()), "erlbm_batch_run");

@@ -57,16 +57,14 @@ class Batch:
         check(lib().erlbm_batch_set_chunk(self._h, chunk_steps), "erlbm_batch_set_chunk")
 
     def autotune(self, sample: np.ndarray, max_steps: int = 2000) -> dict:
-        """Time the sample (budgets capped at max_steps) under every launch
-        shape and chunking the batch supports; keep and return the fastest."""
+        """Time the sample, every budget cut to max_steps, under every
+        workgroups-per-CU value; keep and return the fastest."""
         sample = np.ascontiguousarray(sample, dtype=JOB_DTYPE)
-        w64, w32, ch = C.c_int(), C.c_int(), C.c_int()
+        w64, w32 = C.c_int(), C.c_int()
         pps = C.c_double()
         check(lib().erlbm_batch_autotune(self._h, sample.ctypes.data_as(C.c_void_p), sample.size, max_steps,
-                                         C.byref(w64), C.byref(w32), C.byref(ch), C.byref(pps)),
-              "erlbm_batch_autotune")
-        return {"wg_per_cu_f64": w64.value, "wg_per_cu_f32": w32.value, "chunk_steps": ch.value,
-                "sample_points_per_s": pps.value}
+                                         C.byref(w64), C.byref(w32), C.byref(pps)), "erlbm_batch_autotune")
+        return {"wg_per_cu_f64": w64.value, "wg_per_cu_f32": w32.value, "sample_points_per_s": pps.value}
 
     def run(self, jobs: np.ndarray) -> np.ndarray:
         """jobs: structured array of JOB_DTYPE; returns RESULT_DTYPE results."""

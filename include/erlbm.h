@@ -259,13 +259,15 @@ int erlbm_batch_set_launch(erlbm_batch *b, int wg_per_cu_f64, int wg_per_cu_f32)
  * its state saved in HBM, compaction of the live-job list], replayed until
  * no job is live.  Results are identical either way.                     */
 int erlbm_batch_set_chunk(erlbm_batch *b, int chunk_steps);
-/* Autotuner (no counterpart in the reference): times the sample, each job's
- * step budget capped at max_steps, under every workgroups-per-CU value up
- * to the occupancy limit (per precision class), then under chunk_steps in
- * {0, 100, 1000}; keeps the fastest on the batch and reports it.  Any out
- * pointer may be NULL; *points_per_s is the sample's rate at the choice.  */
+/* Autotuner (no counterpart in the reference): runs the sample with every
+ * job's budget cut to max_steps (so all jobs do equal work and the time
+ * measures per-step throughput, not the longest job) under every
+ * workgroups-per-CU value up to the occupancy limit, per precision class,
+ * and keeps the fastest on the batch.  The chunking set by
+ * erlbm_batch_set_chunk is left as it is.  Out pointers may be NULL;
+ * *points_per_s is the sample's rate at the choice.                      */
 int erlbm_batch_autotune(erlbm_batch *b, const erlbm_job *sample, int n_sample, long long max_steps,
-                         int *wg_per_cu_f64, int *wg_per_cu_f32, int *chunk_steps, double *points_per_s);
+                         int *wg_per_cu_f64, int *wg_per_cu_f32, double *points_per_s);
 
 /* Job lists of the two published maps, with the drivers' axis expressions:
  * Taylor-Green map (src/main_stability-map_taylor-green.cpp:134-164; L = 16,

@@ -166,5 +166,5 @@ def test_batch_autotune_keeps_results(gpu):
         pick = b.autotune(jobs, max_steps=500)
         print(f"batch autotune: {pick}", flush=True)
         assert pick["wg_per_cu_f64"] >= 1 and pick["wg_per_cu_f32"] >= 1
-        assert pick["chunk_steps"] in (0, 100, 1000) and pick["sample_points_per_s"] > 0
+        assert pick["sample_points_per_s"] > 0
         assert b.run(jobs).tobytes() == ref.tobytes()
