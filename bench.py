@@ -88,7 +88,10 @@ def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True):
     del f0
     lat.step(warmup)
     lat.synchronize()
-    launch = lat.autotune(200)[0] if tune else "default"
+    if tune:
+        launch = lat.autotune(200)[0]
+    else:
+        launch = "two steps, 64x8 tiles (default)" if prec == E.FP64 else "one step per launch (default)"
     ms = lat.step_timed(steps)
     steps_done, reason = lat.status()
     ms_per_step = ms / steps
@@ -101,10 +104,16 @@ def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True):
     lat.close()
     ok = steps_done == warmup + steps and reason == 0 and finite and mass_err < (1e-9 if prec == E.FP64 else 1e-3)
     mlups = L * L / (ms_per_step * 1e-3) / 1e6
-    gbs = BYTES_PER_NODE[precision] * L * L / (ms_per_step * 1e-3) / 1e9
+    # a launch reads and writes every population once, whether it advances
+    # one step or two (the two-step kernel keeps the middle step in LDS)
+    steps_per_launch = 2 if "two steps" in launch else 1
+    ms_per_launch = ms_per_step * steps_per_launch
+    gbs = BYTES_PER_NODE[precision] * L * L / (ms_per_launch * 1e-3) / 1e9
     aos_bytes = L * L * 9 * 8
     return ok, {"MLUPS": round(mlups, 1), "ms_per_step": round(ms_per_step, 5), "achieved_GBs": round(gbs, 1),
                 "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "mass_rel_error": mass_err, "launch": launch,
+                "steps_per_launch": steps_per_launch, "ms_per_launch": round(ms_per_launch, 5),
+                "bytes_per_node_step": BYTES_PER_NODE[precision] / steps_per_launch,
                 "host_transfer": {"aos_bytes": aos_bytes, "upload_s": round(t_up, 4), "download_s": round(t_down, 4),
                                   "upload_GBs": round(aos_bytes / t_up / 1e9, 2),
                                   "download_GBs": round(aos_bytes / t_down / 1e9, 2)}}
@@ -190,7 +199,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline FP64 run only")
-    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
+    ap.add_argument("--tune", action="store_true",
+                    help="let the autotuner pick the launch shape (default: the shape chosen by the sweeps)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "fp16"],
                     help="storage of the headline run (fp32/fp16: shifted deviations, FP64 arithmetic)")
     ap.add_argument("--copy-only", action="store_true",
@@ -213,7 +223,7 @@ def main() -> int:
         print(json.dumps({"copy_bytes_each_way": copy_bytes, "ms": copy_ms, "GBs": copy_gbs}))
         return 0
 
-    ok, head = dsl_run(E, a.precision, a.steps, a.warmup, tune=not a.no_tune)
+    ok, head = dsl_run(E, a.precision, a.steps, a.warmup, tune=a.tune)
     if not ok:
         print(f"bench.py: run check failed {head}", file=sys.stderr)
         return 1
@@ -224,7 +234,7 @@ def main() -> int:
             if p == a.precision:
                 prec[p] = head
                 continue
-            okp, r = dsl_run(E, p, max(200, a.steps // 2), 20, tune=not a.no_tune)
+            okp, r = dsl_run(E, p, max(200, a.steps // 2), 20, tune=a.tune)
             prec[p] = r if okp else {"error": "run check failed", **r}
         prec["accuracy_vs_fp64"] = ("FP32 storage: L2 velocity error 2.1e-7 (DSL L=256, 1000 steps), "
                                     "TG L=64 E_vector within 0.01%; FP16: 1.7e-3 / +401% "
@@ -237,7 +247,11 @@ def main() -> int:
     traffic = None
     if a.precision == "fp64" and os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+            measured = json.load(fh)
+        # HBM bytes per launch of the kernel this run used, if profiled
+        for entry in measured.get("kernels", []):
+            if entry.get("launch") and entry["launch"] in head["launch"]:
+                traffic = entry["bytes_per_launch"]
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
     achieved = head["achieved_GBs"]
     line = {
@@ -259,7 +273,10 @@ def main() -> int:
                    "L": L, "tau": TAU, "tau_nh": TAU_NH, "collision": "E-RLBM", "precision": a.precision,
                    "launch": head["launch"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": head["launch"], "steps_per_launch": head["steps_per_launch"],
+                     "algorithmic_bytes_per_launch": BYTES_PER_NODE[a.precision] * L * L,
+                     "ms_per_launch": head["ms_per_launch"]},
         "cpu_baseline": cpu,
         "copy_kernel_GBs": round(copy_gbs, 1),
         "mass_rel_error": head["mass_rel_error"],

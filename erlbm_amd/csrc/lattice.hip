@@ -205,6 +205,87 @@ __global__ __launch_bounds__(BS) void k_step_periodic(StepArgs a, int remap)
     flag_density(a.status, a.step, bad);
 }
 
+// Two steps per launch (temporal blocking) for periodic all-fluid D2Q9 FP64
+// lattices without stop tests.  A workgroup owns a TX x TY tile: step k is
+// computed on the tile plus a one-node halo (pull from HBM, collide) into
+// LDS, step k+1 on the tile from LDS (pull, collide) to HBM.  HBM is read
+// about (TX+2)(TY+2)/(TX TY) times and written once per two steps instead
+// of once each per step; the arithmetic is the single-step kernel's, so the
+// populations are bitwise the same.
+template <int TX, int TY>
+struct Fuse2 {
+    static constexpr int EX = TX + 2, EY = TY + 2, NE = EX * EY;
+    static constexpr int PA = (NE + BLOCK - 1) / BLOCK;       // halo positions per thread
+    static constexpr int PB = (TX * TY + BLOCK - 1) / BLOCK;  // tile positions per thread
+    static constexpr size_t LDS_BYTES = sizeof(double) * 9 * NE;
+};
+
+template <int COL, int TX, int TY>
+__global__ __launch_bounds__(BLOCK) void k_step2_d2q9(StepArgs a, int remap)
+{
+    using V = VSet<9>;
+    using F = Fuse2<TX, TY>;
+    extern __shared__ double h1[];   // [9][NE]: post-collision of step k on the haloed tile
+    // 1-D grid of tiles, x fastest; remap: XCD r (workgroups r, r+8, ...)
+    // takes a contiguous run of tiles, so tiles sharing halo lines share an L2
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    const int ntx = a.g.nx / TX;
+    const int x0 = (int)(b % (unsigned)ntx) * TX;
+    const int y0 = (int)(b / (unsigned)ntx) * TY;
+    const int tid = (int)threadIdx.x;
+    const int nx = a.g.nx, ny = a.g.ny;
+    // ---- step k on the haloed tile: loads of every position first
+    double f[F::PA][9];
+#pragma unroll
+    for (int p = 0; p < F::PA; p++) {
+        const int e = tid + p * BLOCK;
+        if (e < F::NE) {
+            const int x = wrap(x0 - 1 + e % F::EX, nx);
+            const int y = wrap(y0 - 1 + e / F::EX, ny);
+            static_for<9>([&](auto I) {
+                constexpr int i = I;
+                const int xs = V::cx[i] == 0 ? x : wrap(x - V::cx[i], nx);
+                const int ys = V::cy[i] == 0 ? y : wrap(y - V::cy[i], ny);
+                const double *q = &a.src[at(a.g, i, ys, xs)];
+                if constexpr (V::cx[i] == 0) f[p][i] = __builtin_nontemporal_load(q);
+                else f[p][i] = *q;
+            });
+        }
+    }
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+#pragma unroll
+    for (int p = 0; p < F::PA; p++) {
+        const int e = tid + p * BLOCK;
+        if (e < F::NE) {
+            double vx, vy, vz;
+            collide_exact<9, COL>(f[p], a.k, vx, vy, vz);
+            static_for<9>([&](auto I) { h1[(int)I * F::NE + e] = f[p][(int)I]; });
+        }
+    }
+    __syncthreads();
+    // ---- step k+1 on the tile, pulled from LDS
+#pragma unroll
+    for (int p = 0; p < F::PB; p++) {
+        const int o = tid + p * BLOCK;
+        if (o < TX * TY) {
+            const int ox = o % TX, oy = o / TX;
+            const int e = (oy + 1) * F::EX + (ox + 1);
+            double g[9];
+            static_for<9>([&](auto I) {
+                constexpr int i = I;
+                g[i] = h1[i * F::NE + e - V::cx[i] - V::cy[i] * F::EX];
+            });
+            double vx, vy, vz;
+            collide_exact<9, COL>(g, a.k, vx, vy, vz);
+            static_for<9>([&](auto I) {
+                __builtin_nontemporal_store(g[(int)I], &a.dst[at(a.g, (int)I, y0 + oy, x0 + ox)]);
+            });
+        }
+    }
+}
+
 // D2Q9 periodic kernel, two x-adjacent nodes per thread: every population
 // plane is read with one 16-byte load per thread (8-byte aligned when the
 // plane is x-shifted; multi-dword global loads need 4-byte alignment) and
@@ -568,6 +649,8 @@ extern "C" int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *ma
     h->tune_pair = (int)env_ll("ERLBM_PAIR", h->tune_pair);
     h->tune_block = (int)env_ll("ERLBM_BLOCK", h->tune_block);
     h->tune_vec = (int)env_ll("ERLBM_VEC", h->tune_vec);
+    h->tune_fuse = (int)env_ll("ERLBM_FUSE", h->tune_fuse);
+    h->tune_fuse_remap = (int)env_ll("ERLBM_FUSE_REMAP", h->tune_fuse_remap);
     if (h->tune_pad < 0 || h->tune_pad > (1 << 24)) h->tune_pad = 0;
     h->layout = (int)env_ll("ERLBM_LAYOUT", h->layout);
     const long long pad = (h->tune_pad + 63) / 64 * 64;
@@ -768,6 +851,7 @@ extern "C" int erlbm_upload_aos(erlbm_lattice *h, const double *inif)
     ERLBM_HIP_TRY(hipGetLastError());
     h->base = 0;
     h->launched = 0;
+    h->checked = false;
     h->uploaded = true;
     // E0 for the energy test: the reference's energy_ini (ordered sum up to
     // 2^20 sites, the fixed-shape tree above that)
@@ -777,6 +861,73 @@ extern "C" int erlbm_upload_aos(erlbm_lattice *h, const double *inif)
     h->e0_valid = true;
     return ERLBM_OK;
 }
+
+namespace {
+
+// Tile shapes of the two-step kernel (ERLBM_FUSE selects one; 0 = off).
+struct FuseShape {
+    int tx, ty;
+};
+constexpr FuseShape FUSE_SHAPES[] = {{0, 0},  {64, 8},  {32, 16}, {32, 8},  {128, 4}, {16, 16},
+                                     {32, 32}, {64, 4},  {128, 8}, {256, 4}, {64, 16}};
+constexpr int N_FUSE_SHAPES = sizeof(FUSE_SHAPES) / sizeof(FUSE_SHAPES[0]);
+
+bool fuse_applies(const erlbm_lattice *h, int energy_interval)
+{
+    if (h->tune_fuse <= 0 || h->tune_fuse >= N_FUSE_SHAPES) return false;
+    const FuseShape fs = FUSE_SHAPES[h->tune_fuse];
+    return h->q == 9 && h->elem == 8 && h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0 &&
+           h->dens_limit <= 0.0 && energy_interval <= 0 && !h->checked && h->nx % fs.tx == 0 &&
+           h->ny % fs.ty == 0 && h->nz == 1;
+}
+
+template <int COL, int TX, int TY>
+void launch_fused_t(erlbm_lattice *h, const StepArgs &a)
+{
+    constexpr size_t lds = Fuse2<TX, TY>::LDS_BYTES;
+    if constexpr (lds > 65536) {
+        static bool raised = false;   // per process; the limit is a property of the kernel
+        if (!raised) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step2_d2q9<COL, TX, TY>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            raised = true;
+        }
+    }
+    const unsigned nb = (unsigned)((h->nx / TX) * (h->ny / TY));
+    hipLaunchKernelGGL((k_step2_d2q9<COL, TX, TY>), dim3(nb), dim3(BLOCK), lds, h->stream, a, h->tune_fuse_remap);
+}
+
+template <int COL>
+void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
+{
+    switch (h->tune_fuse) {
+    case 1: launch_fused_t<COL, 64, 8>(h, a); break;
+    case 2: launch_fused_t<COL, 32, 16>(h, a); break;
+    case 3: launch_fused_t<COL, 32, 8>(h, a); break;
+    case 4: launch_fused_t<COL, 128, 4>(h, a); break;
+    case 5: launch_fused_t<COL, 16, 16>(h, a); break;
+    case 6: launch_fused_t<COL, 32, 32>(h, a); break;
+    case 7: launch_fused_t<COL, 64, 4>(h, a); break;
+    case 8: launch_fused_t<COL, 128, 8>(h, a); break;
+    case 9: launch_fused_t<COL, 256, 4>(h, a); break;
+    default: launch_fused_t<COL, 64, 16>(h, a); break;
+    }
+}
+
+int launch_fused(erlbm_lattice *h, const StepArgs &a)
+{
+    switch (h->col) {
+    case ERLBM_COL_BGK: launch_fused_c<COL_BGK>(h, a); break;
+    case ERLBM_COL_MRT: launch_fused_c<COL_MRT>(h, a); break;
+    case ERLBM_COL_REG: launch_fused_c<COL_REG>(h, a); break;
+    case ERLBM_COL_EXT_REG: launch_fused_c<COL_EXT_REG>(h, a); break;
+    default: return ERLBM_ERR_UNSUPPORTED;
+    }
+    ERLBM_HIP_TRY(hipGetLastError());
+    return ERLBM_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -817,7 +968,9 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
         return ERLBM_ERR_STATE;
     }
     StepArgs a = step_args(h);
-    for (int s = 0; s < n_steps; s++) {
+    if (energy_interval > 0 || h->dens_limit > 0.0) h->checked = true;
+    const bool fuse = fuse_applies(h, energy_interval);
+    for (int s = 0; s < n_steps;) {
         const long long k = h->launched;
         const int cur = (h->base + (int)(k & 1)) & 1;
         if (energy_interval > 0 && k > 0 && k % energy_interval == 0) {
@@ -828,9 +981,21 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
         a.src = static_cast<const double *>(h->buf[cur]);
         a.dst = static_cast<double *>(h->buf[cur ^ 1]);
         a.step = (unsigned long long)k;
+        if (fuse && k > 0 && s + 1 < n_steps) {
+            // steps k and k+1 in one launch: h_{k+1} lands in the buffer
+            // a single step would have written h_k to, so the parity of the
+            // buffers flips (base) with the step count advanced by two
+            int rc = launch_fused(h, a);
+            if (rc) return rc;
+            h->launched += 2;
+            h->base ^= 1;
+            s += 2;
+            continue;
+        }
         int rc = dispatch_step(h, a, k > 0);
         if (rc) return rc;
         h->launched++;
+        s++;
     }
     return ERLBM_OK;
 }
@@ -842,6 +1007,7 @@ namespace {
 struct LaunchShape {
     int ntl, block, npt, pair, vec;
     const char *name;
+    int fuse = 0, fuse_remap = 0;   // two-step kernel tile (FUSE_SHAPES index), 0 = one step per launch
 };
 
 std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
@@ -850,6 +1016,20 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
     const bool periodic = h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0;
     if (!periodic) return c;
     if (h->elem == 8 && h->q == 9) {
+        for (int f : {1, 7, 4, 2}) {
+            const FuseShape fs = FUSE_SHAPES[f];
+            if (h->nx % fs.tx || h->ny % fs.ty) continue;
+            for (int rm = 0; rm < 2; rm++) {
+                LaunchShape ls{2, 256, 1, 0, 0, nullptr, f, rm};
+                static const char *names[N_FUSE_SHAPES][2] = {
+                    {"", ""}, {"two steps, 64x8 tiles", "two steps, 64x8 tiles, XCD runs"},
+                    {"two steps, 32x16 tiles", "two steps, 32x16 tiles, XCD runs"}, {"", ""},
+                    {"two steps, 128x4 tiles", "two steps, 128x4 tiles, XCD runs"}, {"", ""}, {"", ""},
+                    {"two steps, 64x4 tiles", "two steps, 64x4 tiles, XCD runs"}};
+                ls.name = names[f][rm];
+                c.push_back(ls);
+            }
+        }
         c.push_back({2, 256, 1, 0, 0, "nt-loads=c_x0 block=256"});
         c.push_back({2, 512, 1, 0, 0, "nt-loads=c_x0 block=512"});
         c.push_back({2, 128, 1, 0, 0, "nt-loads=c_x0 block=128"});
@@ -878,6 +1058,8 @@ void apply_shape(erlbm_lattice *h, const LaunchShape &s)
     h->tune_npt = s.npt;
     h->tune_pair = s.pair;
     h->tune_vec = s.vec;
+    h->tune_fuse = s.fuse;
+    h->tune_fuse_remap = s.fuse_remap;
 }
 
 }  // namespace
@@ -890,7 +1072,9 @@ extern "C" int erlbm_lattice_autotune(erlbm_lattice *h, int n_steps, char *choic
         return ERLBM_ERR_STATE;
     }
     const std::vector<LaunchShape> shapes = candidate_shapes(h);
-    const LaunchShape keep{h->tune_ntl, h->tune_block, h->tune_npt, h->tune_pair, h->tune_vec, "unchanged"};
+    const LaunchShape keep{h->tune_ntl, h->tune_block, h->tune_npt, h->tune_pair, h->tune_vec, "unchanged",
+                           h->tune_fuse, h->tune_fuse_remap};
+    const int steps = n_steps + (n_steps & 1);   // whole pairs for the two-step candidates
     // scratch copy of the current state, a private stop word: the handle's
     // run is not touched
     const size_t bytes = (size_t)h->extent * h->elem;
@@ -920,13 +1104,15 @@ extern "C" int erlbm_lattice_autotune(erlbm_lattice *h, int n_steps, char *choic
         unsigned long long k = 0;
         for (size_t c = 0; c < shapes.size() && rc == ERLBM_OK; c++) {
             apply_shape(h, shapes[c]);
+            const bool two = shapes[c].fuse > 0;
             float ms = 0.f;
-            for (int s = -2; s < n_steps && rc == ERLBM_OK; s++) {
+            for (int s = -2; s < steps && rc == ERLBM_OK; s += two ? 2 : 1) {
                 if (s == 0) hip(hipEventRecord(e0, h->stream), "hipEventRecord");
                 a.src = static_cast<const double *>(tmp[k & 1]);
                 a.dst = static_cast<double *>(tmp[(k & 1) ^ 1]);
-                a.step = k++;
-                const int r = dispatch_step(h, a, true);
+                a.step = 0;
+                k++;   // a two-step launch also writes the other buffer
+                const int r = two ? launch_fused(h, a) : dispatch_step(h, a, true);
                 if (r && rc == ERLBM_OK) rc = r;
             }
             if (hip(hipEventRecord(e1, h->stream), "hipEventRecord") &&
@@ -953,7 +1139,7 @@ extern "C" int erlbm_lattice_autotune(erlbm_lattice *h, int n_steps, char *choic
         std::strncpy(choice, name, (size_t)choice_len - 1);
         choice[choice_len - 1] = '\0';
     }
-    if (us_per_step) *us_per_step = best >= 0 ? best_ms * 1000.f / n_steps : 0.f;
+    if (us_per_step) *us_per_step = best >= 0 ? best_ms * 1000.f / steps : 0.f;
     return ERLBM_OK;
 }
 

@@ -164,6 +164,12 @@ struct erlbm_lattice {
     //   ERLBM_VEC        vectorised 16-byte kernel for periodic lattices (lattice_vec.hip):
     //                    -1 auto, 0 off, 1 edge elements by wavefront shuffle, 2 edge elements loaded
     int tune_npt = 1, tune_ntl = 2, tune_remap = 0, tune_pair = 0, tune_block = 256, tune_vec = -1;
+    //   ERLBM_FUSE       two steps per launch for periodic D2Q9 FP64 runs without stop
+    //                    tests: 0 off, 1..10 tile shape (64x8, 32x16, 32x8, 128x4, 16x16,
+    //                    32x32, 64x4, 128x8, 256x4, 64x16)
+    int tune_fuse = 1;
+    int tune_fuse_remap = 0;  //   ERLBM_FUSE_REMAP 1: XCD-contiguous runs of tiles
+    bool checked = false;     // a stop test has run since the upload (no fused steps then)
 };
 
 

@@ -267,3 +267,36 @@ def test_mrt_is_d2q9_only(gpu):
     with gpu.Lattice(19, 8, 8, 2) as lat:
         with pytest.raises(RuntimeError, match="MRT"):
             lat.set_params(0.6, 0.65, 1.0, O.COL_MRT)
+
+
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
+def test_two_step_fused_kernel_bitwise(gpu, orc, monkeypatch, shape):
+    """Two steps per launch (LDS tile with a one-node halo): any mix of call
+    lengths gives the populations, post-collision monitors and energies of
+    the single-step path, bitwise."""
+    L = 128
+    tau = 0.5 + 3. * (1. / 32.) * L / 80000.
+    f0 = gpu.init_double_shear(L)
+    ini, st, nf = orc.stream_map(9, L, L, 1)
+    monkeypatch.setenv("ERLBM_FUSE", "0")
+    ref = _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0)
+    monkeypatch.setenv("ERLBM_FUSE", str(shape))
+    fused = _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0)
+    with ref, fused:
+        done = 0
+        for n in (1, 9, 2, 7, 4):
+            ref.step(n)
+            fused.step(n)
+            done += n
+            assert ref.status() == fused.status() == (done, 0)
+            assert np.array_equal(fused.download(), ref.download())
+            assert np.array_equal(fused.download(gpu.POST_COLLISION), ref.download(gpu.POST_COLLISION))
+            assert fused.kinetic_energy(gpu.POST_COLLISION) == ref.kinetic_energy(gpu.POST_COLLISION)
+        assert np.array_equal(fused.download(), orc.run_steps(9, f0, st, done, O.COL_EXT_REG, tau, 0.65))
+        # an energy-checked call falls back to single steps and stays exact
+        fused.step(6, energy_interval=4, energy_factor=2.0)
+        ref.step(6, energy_interval=4, energy_factor=2.0)
+        fused.step(4)
+        ref.step(4)
+        assert fused.status() == ref.status()
+        assert np.array_equal(fused.download(), ref.download())
