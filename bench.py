@@ -91,7 +91,8 @@ def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True):
     if tune:
         launch = lat.autotune(200)[0]
     else:
-        launch = "two steps, 64x8 tiles (default)" if prec == E.FP64 else "one step per launch (default)"
+        launch = ("two steps, 256-wide strips x 32 rows, XCD runs (default)" if prec == E.FP64
+                  else "one step per launch (default)")
     ms = lat.step_timed(steps)
     steps_done, reason = lat.status()
     ms_per_step = ms / steps

@@ -286,6 +286,86 @@ __global__ __launch_bounds__(BLOCK) void k_step2_d2q9(StepArgs a, int remap)
     }
 }
 
+// Two steps per launch, walking a TX-wide x-strip down SEG output rows:
+// only a ring of three rows of step-k results lives in LDS, so the tile can
+// be long in y (halo 2 rows per SEG) and wide in x (halo 2 columns per TX)
+// without the LDS of a whole 2-D tile.  Thread t computes step-k position
+// x0-1+t (t < TX+2) and step-(k+1) position x0+t (t < TX); the loads of the
+// next step-k row are in flight while the current output row is computed.
+// (Two rows in flight need 138 instead of 104 VGPRs and measured slower.)
+template <int TX>
+constexpr int strip_threads() { return TX + 64; }
+
+template <int COL, int TX, int SEG>
+__global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a, int remap)
+{
+    using V = VSet<9>;
+    constexpr int EX = TX + 2;
+    extern __shared__ double ring[];   // [3][9][EX]
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    const int nstrip = a.g.nx / TX;
+    const int x0 = (int)(b % (unsigned)nstrip) * TX;
+    const int ya = (int)(b / (unsigned)nstrip) * SEG;
+    const int t = (int)threadIdx.x;
+    const int nx = a.g.nx, ny = a.g.ny;
+    const bool act1 = t < EX;
+    const int x1 = act1 ? wrap(x0 - 1 + t, nx) : 0;
+    auto load_row = [&](int y, double (&f)[9]) {   // pulls of step-k row y (logical, may be -1 or ny)
+        if (!act1) return;
+        const int yw = wrap(y, ny);
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            const int xs = V::cx[i] == 0 ? x1 : wrap(x1 - V::cx[i], nx);
+            const int ys = V::cy[i] == 0 ? yw : wrap(yw - V::cy[i], ny);
+            const double *q = &a.src[at(a.g, i, ys, xs)];
+            if constexpr (V::cx[i] == 0) f[i] = __builtin_nontemporal_load(q);
+            else f[i] = *q;
+        });
+    };
+    auto slot = [&](int y) { return (y - ya + 1) % 3; };
+    auto collide_to_ring = [&](int y, double (&f)[9]) {
+        if (!act1) return;
+        double vx, vy, vz;
+        collide_exact<9, COL>(f, a.k, vx, vy, vz);
+        double *r = ring + slot(y) * 9 * EX;
+        static_for<9>([&](auto I) { r[(int)I * EX + t] = f[(int)I]; });
+    };
+    auto output_row = [&](int y) {            // step k+1, row y, from the ring
+        if (t < TX) {
+            double g[9];
+            static_for<9>([&](auto I) {
+                constexpr int i = I;
+                g[i] = ring[slot(y - V::cy[i]) * 9 * EX + i * EX + (t + 1 - V::cx[i])];
+            });
+            double vx, vy, vz;
+            collide_exact<9, COL>(g, a.k, vx, vy, vz);
+            static_for<9>([&](auto I) {
+                __builtin_nontemporal_store(g[(int)I], &a.dst[at(a.g, (int)I, y, x0 + t)]);
+            });
+        }
+    };
+    double fa[9], fb[9];
+    load_row(ya - 1, fa);
+    load_row(ya, fb);
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+    collide_to_ring(ya - 1, fa);
+    collide_to_ring(ya, fb);
+    load_row(ya + 1, fa);
+    // Three ring slots: row y+1 overwrites row y-2, which the previous
+    // iteration read, hence the second barrier.  (Four slots and one
+    // barrier per row measured the same, with a third more LDS.)
+    for (int r = 0; r < SEG; r++) {
+        const int y = ya + r;
+        collide_to_ring(y + 1, fa);           // step k, row y+1
+        if (r + 1 < SEG) load_row(y + 2, fa); // in flight while row y is computed
+        __syncthreads();
+        output_row(y);
+        __syncthreads();
+    }
+}
+
 // D2Q9 periodic kernel, two x-adjacent nodes per thread: every population
 // plane is read with one 16-byte load per thread (8-byte aligned when the
 // plane is x-shifted; multi-dword global loads need 4-byte alignment) and
@@ -866,10 +946,13 @@ namespace {
 
 // Tile shapes of the two-step kernel (ERLBM_FUSE selects one; 0 = off).
 struct FuseShape {
-    int tx, ty;
+    int tx, ty;       // tile, or (strip) strip width and rows walked
+    bool strip;
 };
-constexpr FuseShape FUSE_SHAPES[] = {{0, 0},  {64, 8},  {32, 16}, {32, 8},  {128, 4}, {16, 16},
-                                     {32, 32}, {64, 4},  {128, 8}, {256, 4}, {64, 16}};
+constexpr FuseShape FUSE_SHAPES[] = {{0, 0, false},   {64, 8, false},   {32, 16, false}, {32, 8, false},
+                                     {128, 4, false}, {16, 16, false},  {32, 32, false}, {64, 4, false},
+                                     {128, 8, false}, {256, 4, false},  {64, 16, false}, {256, 32, true},
+                                     {256, 16, true}, {128, 32, true},  {256, 64, true}, {128, 64, true}};
 constexpr int N_FUSE_SHAPES = sizeof(FUSE_SHAPES) / sizeof(FUSE_SHAPES[0]);
 
 bool fuse_applies(const erlbm_lattice *h, int energy_interval)
@@ -897,9 +980,34 @@ void launch_fused_t(erlbm_lattice *h, const StepArgs &a)
     hipLaunchKernelGGL((k_step2_d2q9<COL, TX, TY>), dim3(nb), dim3(BLOCK), lds, h->stream, a, h->tune_fuse_remap);
 }
 
+template <int COL, int TX, int SEG>
+void launch_strip_t(erlbm_lattice *h, const StepArgs &a)
+{
+    constexpr size_t lds = sizeof(double) * 3 * 9 * (TX + 2);
+    if constexpr (lds > 65536) {
+        static bool raised = false;
+        if (!raised) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step2_strip<COL, TX, SEG>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            raised = true;
+        }
+    }
+    const unsigned nb = (unsigned)((h->nx / TX) * (h->ny / SEG));
+    hipLaunchKernelGGL((k_step2_strip<COL, TX, SEG>), dim3(nb), dim3(strip_threads<TX>()), lds, h->stream, a,
+                       h->tune_fuse_remap);
+}
+
 template <int COL>
 void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
 {
+    switch (h->tune_fuse) {
+    case 11: launch_strip_t<COL, 256, 32>(h, a); return;
+    case 12: launch_strip_t<COL, 256, 16>(h, a); return;
+    case 13: launch_strip_t<COL, 128, 32>(h, a); return;
+    case 14: launch_strip_t<COL, 256, 64>(h, a); return;
+    case 15: launch_strip_t<COL, 128, 64>(h, a); return;
+    default: break;
+    }
     switch (h->tune_fuse) {
     case 1: launch_fused_t<COL, 64, 8>(h, a); break;
     case 2: launch_fused_t<COL, 32, 16>(h, a); break;
@@ -1016,17 +1124,20 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
     const bool periodic = h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0;
     if (!periodic) return c;
     if (h->elem == 8 && h->q == 9) {
-        for (int f : {1, 7, 4, 2}) {
-            const FuseShape fs = FUSE_SHAPES[f];
+        struct Named {
+            int f;
+            const char *name[2];
+        };
+        static const Named fused[] = {
+            {11, {"two steps, 256-wide strips x 32 rows", "two steps, 256-wide strips x 32 rows, XCD runs"}},
+            {13, {"two steps, 128-wide strips x 32 rows", "two steps, 128-wide strips x 32 rows, XCD runs"}},
+            {1, {"two steps, 64x8 tiles", "two steps, 64x8 tiles, XCD runs"}},
+            {7, {"two steps, 64x4 tiles", "two steps, 64x4 tiles, XCD runs"}}};
+        for (const Named &nm : fused) {
+            const FuseShape fs = FUSE_SHAPES[nm.f];
             if (h->nx % fs.tx || h->ny % fs.ty) continue;
             for (int rm = 0; rm < 2; rm++) {
-                LaunchShape ls{2, 256, 1, 0, 0, nullptr, f, rm};
-                static const char *names[N_FUSE_SHAPES][2] = {
-                    {"", ""}, {"two steps, 64x8 tiles", "two steps, 64x8 tiles, XCD runs"},
-                    {"two steps, 32x16 tiles", "two steps, 32x16 tiles, XCD runs"}, {"", ""},
-                    {"two steps, 128x4 tiles", "two steps, 128x4 tiles, XCD runs"}, {"", ""}, {"", ""},
-                    {"two steps, 64x4 tiles", "two steps, 64x4 tiles, XCD runs"}};
-                ls.name = names[f][rm];
+                LaunchShape ls{2, 256, 1, 0, 0, nm.name[rm], nm.f, rm};
                 c.push_back(ls);
             }
         }

@@ -166,9 +166,10 @@ struct erlbm_lattice {
     int tune_npt = 1, tune_ntl = 2, tune_remap = 0, tune_pair = 0, tune_block = 256, tune_vec = -1;
     //   ERLBM_FUSE       two steps per launch for periodic D2Q9 FP64 runs without stop
     //                    tests: 0 off, 1..10 tile shape (64x8, 32x16, 32x8, 128x4, 16x16,
-    //                    32x32, 64x4, 128x8, 256x4, 64x16)
-    int tune_fuse = 1;
-    int tune_fuse_remap = 0;  //   ERLBM_FUSE_REMAP 1: XCD-contiguous runs of tiles
+    //                    32x32, 64x4, 128x8, 256x4, 64x16), 11..15 strip walks (256x32,
+    //                    256x16, 128x32, 256x64, 128x64 rows)
+    int tune_fuse = 11;
+    int tune_fuse_remap = 1;  //   ERLBM_FUSE_REMAP 1: XCD-contiguous runs of tiles (0: dispatch order)
     bool checked = false;     // a stop test has run since the upload (no fused steps then)
 };
 

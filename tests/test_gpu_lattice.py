@@ -269,12 +269,12 @@ def test_mrt_is_d2q9_only(gpu):
             lat.set_params(0.6, 0.65, 1.0, O.COL_MRT)
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 11, 12, 13, 14, 15])
 def test_two_step_fused_kernel_bitwise(gpu, orc, monkeypatch, shape):
     """Two steps per launch (LDS tile with a one-node halo): any mix of call
     lengths gives the populations, post-collision monitors and energies of
     the single-step path, bitwise."""
-    L = 128
+    L = 256
     tau = 0.5 + 3. * (1. / 32.) * L / 80000.
     f0 = gpu.init_double_shear(L)
     ini, st, nf = orc.stream_map(9, L, L, 1)
