@@ -91,8 +91,8 @@ def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True):
     if tune:
         launch = lat.autotune(200)[0]
     else:
-        launch = ("two steps, 256-wide strips x 32 rows, XCD runs (default)" if prec == E.FP64
-                  else "one step per launch (default)")
+        launch = ("16-byte vectors, one step per launch (default)" if prec == E.FP16_SHIFTED
+                  else "two steps, 256-wide strips x 32 rows, XCD runs (default)")
     ms = lat.step_timed(steps)
     steps_done, reason = lat.status()
     ms_per_step = ms / steps

@@ -729,6 +729,7 @@ extern "C" int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *ma
     h->tune_pair = (int)env_ll("ERLBM_PAIR", h->tune_pair);
     h->tune_block = (int)env_ll("ERLBM_BLOCK", h->tune_block);
     h->tune_vec = (int)env_ll("ERLBM_VEC", h->tune_vec);
+    h->fuse_requested = getenv("ERLBM_FUSE") != nullptr;
     h->tune_fuse = (int)env_ll("ERLBM_FUSE", h->tune_fuse);
     h->tune_fuse_remap = (int)env_ll("ERLBM_FUSE_REMAP", h->tune_fuse_remap);
     if (h->tune_pad < 0 || h->tune_pad > (1 << 24)) h->tune_pad = 0;
@@ -748,6 +749,8 @@ extern "C" int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *ma
     h->f32_arith = precision == ERLBM_FP32_SHIFTED_F32;
     h->stream = (hipStream_t)stream;
     hipGetDevice(&h->device);
+    if (hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || h->n_cu < 1)
+        h->n_cu = 256;
 
     // bounce mask: bit i set when the pull source x - c_i is solid
     std::vector<uint32_t> nbr(nvox, 0);
@@ -955,13 +958,35 @@ constexpr FuseShape FUSE_SHAPES[] = {{0, 0, false},   {64, 8, false},   {32, 16,
                                      {256, 16, true}, {128, 32, true},  {256, 64, true}, {128, 64, true}};
 constexpr int N_FUSE_SHAPES = sizeof(FUSE_SHAPES) / sizeof(FUSE_SHAPES[0]);
 
-bool fuse_applies(const erlbm_lattice *h, int energy_interval)
+bool fuse_shape_fits(const erlbm_lattice *h, int f)
 {
-    if (h->tune_fuse <= 0 || h->tune_fuse >= N_FUSE_SHAPES) return false;
-    const FuseShape fs = FUSE_SHAPES[h->tune_fuse];
-    return h->q == 9 && h->elem == 8 && h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0 &&
-           h->dens_limit <= 0.0 && energy_interval <= 0 && !h->checked && h->nx % fs.tx == 0 &&
-           h->ny % fs.ty == 0 && h->nz == 1;
+    if (f <= 0 || f >= N_FUSE_SHAPES) return false;
+    const FuseShape fs = FUSE_SHAPES[f];
+    // shifted storage: the strip walks only, FP64 arithmetic only; FP16 by
+    // explicit request (ERLBM_FUSE) only -- its 16-byte vector kernel is
+    // faster (36.8 vs 53.2 us/step at L = 2048: the two collisions per
+    // launch make the walk VALU-bound once the bytes are that few)
+    if (h->elem != 8 && (h->f32_arith || !fs.strip || (f != 11 && f != 13))) return false;
+    if (h->elem == 2 && !h->fuse_requested) return false;
+    return h->nx % fs.tx == 0 && h->ny % fs.ty == 0;
+}
+
+// The two-step shape a call of erlbm_step uses (0: one step per launch).
+// Without an explicit ERLBM_FUSE, a shape must give every CU a workgroup
+// (small lattices have too few strips or tiles to fill the chip: there the
+// one-step kernel is faster); the 64x8 tiles are the fallback.
+int fuse_shape(const erlbm_lattice *h, int energy_interval)
+{
+    if (!(h->q == 9 && h->nz == 1 && h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0 && h->dens_limit <= 0.0 &&
+          energy_interval <= 0 && !h->checked))
+        return 0;
+    if (h->fuse_requested) return fuse_shape_fits(h, h->tune_fuse) ? h->tune_fuse : 0;
+    for (int f : {h->tune_fuse, 1}) {
+        if (!fuse_shape_fits(h, f)) continue;
+        const FuseShape fs = FUSE_SHAPES[f];
+        if ((long long)(h->nx / fs.tx) * (h->ny / fs.ty) >= h->n_cu) return f;
+    }
+    return 0;
 }
 
 template <int COL, int TX, int TY>
@@ -1000,7 +1025,7 @@ void launch_strip_t(erlbm_lattice *h, const StepArgs &a)
 template <int COL>
 void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
 {
-    switch (h->tune_fuse) {
+    switch (h->fuse_active) {
     case 11: launch_strip_t<COL, 256, 32>(h, a); return;
     case 12: launch_strip_t<COL, 256, 16>(h, a); return;
     case 13: launch_strip_t<COL, 128, 32>(h, a); return;
@@ -1008,7 +1033,7 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
     case 15: launch_strip_t<COL, 128, 64>(h, a); return;
     default: break;
     }
-    switch (h->tune_fuse) {
+    switch (h->fuse_active) {
     case 1: launch_fused_t<COL, 64, 8>(h, a); break;
     case 2: launch_fused_t<COL, 32, 16>(h, a); break;
     case 3: launch_fused_t<COL, 32, 8>(h, a); break;
@@ -1024,6 +1049,7 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
 
 int launch_fused(erlbm_lattice *h, const StepArgs &a)
 {
+    if (h->elem != 8) return lowp_step2(h, a);
     switch (h->col) {
     case ERLBM_COL_BGK: launch_fused_c<COL_BGK>(h, a); break;
     case ERLBM_COL_MRT: launch_fused_c<COL_MRT>(h, a); break;
@@ -1077,7 +1103,8 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
     }
     StepArgs a = step_args(h);
     if (energy_interval > 0 || h->dens_limit > 0.0) h->checked = true;
-    const bool fuse = fuse_applies(h, energy_interval);
+    h->fuse_active = fuse_shape(h, energy_interval);
+    const bool fuse = h->fuse_active > 0;
     for (int s = 0; s < n_steps;) {
         const long long k = h->launched;
         const int cur = (h->base + (int)(k & 1)) & 1;
@@ -1153,6 +1180,15 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
         c.push_back({1, 256, 1, 0, 0, "nt-loads=all"});
         c.push_back({0, 256, 1, 0, 0, "nt-loads=none"});
     } else {
+        if (h->q == 9 && !h->f32_arith && h->elem == 4) {
+            static const char *names[2][2] = {{"two steps, 256-wide strips x 32 rows", "two steps, 256-wide strips x 32 rows, XCD runs"},
+                                              {"two steps, 128-wide strips x 32 rows", "two steps, 128-wide strips x 32 rows, XCD runs"}};
+            const int fs[2] = {11, 13};
+            for (int k = 0; k < 2; k++) {
+                if (h->nx % FUSE_SHAPES[fs[k]].tx || h->ny % FUSE_SHAPES[fs[k]].ty) continue;
+                for (int rm = 0; rm < 2; rm++) c.push_back({h->tune_ntl, h->tune_block, 1, 0, 0, names[k][rm], fs[k], rm});
+            }
+        }
         c.push_back({h->tune_ntl, h->tune_block, 1, 0, 0, "one node/thread"});
     }
     if (!h->f32_arith) {
@@ -1171,6 +1207,7 @@ void apply_shape(erlbm_lattice *h, const LaunchShape &s)
     h->tune_vec = s.vec;
     h->tune_fuse = s.fuse;
     h->tune_fuse_remap = s.fuse_remap;
+    h->fuse_requested = true;   // the measured choice stands, whatever the lattice size
 }
 
 }  // namespace
@@ -1216,6 +1253,7 @@ extern "C" int erlbm_lattice_autotune(erlbm_lattice *h, int n_steps, char *choic
         for (size_t c = 0; c < shapes.size() && rc == ERLBM_OK; c++) {
             apply_shape(h, shapes[c]);
             const bool two = shapes[c].fuse > 0;
+            h->fuse_active = shapes[c].fuse;
             float ms = 0.f;
             for (int s = -2; s < steps && rc == ERLBM_OK; s += two ? 2 : 1) {
                 if (s == 0) hip(hipEventRecord(e0, h->stream), "hipEventRecord");

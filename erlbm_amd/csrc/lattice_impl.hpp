@@ -171,6 +171,9 @@ struct erlbm_lattice {
     int tune_fuse = 11;
     int tune_fuse_remap = 1;  //   ERLBM_FUSE_REMAP 1: XCD-contiguous runs of tiles (0: dispatch order)
     bool checked = false;     // a stop test has run since the upload (no fused steps then)
+    bool fuse_requested = false;   // ERLBM_FUSE set explicitly
+    int fuse_active = 0;      // two-step shape of the current erlbm_step call (0: none)
+    int n_cu = 256;           // compute units of the device
 };
 
 
@@ -184,6 +187,9 @@ inline Geometry geom(const erlbm_lattice *h) { return Geometry{h->nx, h->ny, h->
 // Shifted low-precision storage (lattice_lowp.hip).  `src`/`dst` are the
 // handle's population buffers of element size h->elem.
 int lowp_step(erlbm_lattice *h, const StepArgs &a, bool stream);
+// Two steps per launch for periodic D2Q9 shifted storage (FP64 arithmetic),
+// strip walk of h->tune_fuse (13: 128-wide, otherwise 256-wide), 32 rows.
+int lowp_step2(erlbm_lattice *h, const StepArgs &a);
 // AoS FP64 (device, reference fluid order) <-> shifted storage.
 int lowp_from_aos(erlbm_lattice *h, const double *aos, void *dst);
 int lowp_to_aos(erlbm_lattice *h, const void *src, bool stream, double *aos);

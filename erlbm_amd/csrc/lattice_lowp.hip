@@ -254,6 +254,108 @@ __global__ __launch_bounds__(BLOCK) void k_lp_monitor(const T *__restrict__ src,
     }
 }
 
+// Two steps per launch for periodic all-fluid D2Q9 lattices with shifted
+// storage and FP64 arithmetic: the strip walk of lattice.hip's
+// k_step2_strip (three-row LDS ring, one halo column each side), with the
+// step-k populations kept in the ring in the storage type T, exactly the
+// values a one-step launch would have written to HBM and read back.
+template <int TX>
+constexpr int lp_strip_threads() { return TX + 64; }
+
+template <int COL, typename T, int TX, int SEG>
+__global__ __launch_bounds__(lp_strip_threads<TX>()) void k_lp_step2_strip(LPArgs a, int remap)
+{
+    using V = VSet<9>;
+    constexpr int EX = TX + 2;
+    extern __shared__ double ring_raw[];
+    T *ring = reinterpret_cast<T *>(ring_raw);   // [3][9][EX]
+    const T *__restrict__ src = static_cast<const T *>(a.src);
+    T *__restrict__ dst = static_cast<T *>(a.dst);
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    const int nstrip = a.g.nx / TX;
+    const int x0 = (int)(b % (unsigned)nstrip) * TX;
+    const int ya = (int)(b / (unsigned)nstrip) * SEG;
+    const int t = (int)threadIdx.x;
+    const int nx = a.g.nx, ny = a.g.ny;
+    const bool act1 = t < EX;
+    const int x1 = act1 ? wrap(x0 - 1 + t, nx) : 0;
+    auto load_row = [&](int y, float (&d)[9]) {
+        if (!act1) return;
+        const int yw = wrap(y, ny);
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            const int xs = V::cx[i] == 0 ? x1 : wrap(x1 - V::cx[i], nx);
+            const int ys = V::cy[i] == 0 ? yw : wrap(yw - V::cy[i], ny);
+            d[i] = ld(&src[at(a.g, i, ys, xs)], V::cx[i] == 0);
+        });
+    };
+    auto slot = [&](int y) { return (y - ya + 1) % 3; };
+    auto collide_to_ring = [&](int y, float (&d)[9]) {
+        if (!act1) return;
+        (void)collide_lp<9, COL, false>(d, a);
+        T *r = ring + slot(y) * 9 * EX;
+        static_for<9>([&](auto I) { r[(int)I * EX + t] = (T)d[(int)I]; });
+    };
+    auto output_row = [&](int y) {
+        if (t < TX) {
+            float d[9];
+            static_for<9>([&](auto I) {
+                constexpr int i = I;
+                d[i] = (float)ring[slot(y - V::cy[i]) * 9 * EX + i * EX + (t + 1 - V::cx[i])];
+            });
+            (void)collide_lp<9, COL, false>(d, a);
+            static_for<9>([&](auto I) { st(&dst[at(a.g, (int)I, y, x0 + t)], d[(int)I]); });
+        }
+    };
+    float da[9], db[9];
+    load_row(ya - 1, da);
+    load_row(ya, db);
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+    collide_to_ring(ya - 1, da);
+    collide_to_ring(ya, db);
+    load_row(ya + 1, da);
+    for (int r = 0; r < SEG; r++) {
+        const int y = ya + r;
+        collide_to_ring(y + 1, da);
+        if (r + 1 < SEG) load_row(y + 2, da);
+        __syncthreads();
+        output_row(y);
+        __syncthreads();
+    }
+}
+
+template <int COL, typename T, int TX, int SEG>
+void launch_lp_strip(erlbm_lattice *h, const LPArgs &l)
+{
+    constexpr size_t lds = sizeof(T) * 3 * 9 * (TX + 2);
+    const unsigned nb = (unsigned)((h->nx / TX) * (h->ny / SEG));
+    hipLaunchKernelGGL((k_lp_step2_strip<COL, T, TX, SEG>), dim3(nb), dim3(lp_strip_threads<TX>()), lds, h->stream,
+                       l, h->tune_fuse_remap);
+}
+
+template <int COL, typename T>
+void launch_lp_step2_ct(erlbm_lattice *h, const LPArgs &l)
+{
+    if (h->fuse_active == 13) launch_lp_strip<COL, T, 128, 32>(h, l);
+    else launch_lp_strip<COL, T, 256, 32>(h, l);
+}
+
+template <typename T>
+int launch_lp_step2_t(erlbm_lattice *h, const LPArgs &l)
+{
+    switch (h->col) {
+    case ERLBM_COL_BGK: launch_lp_step2_ct<COL_BGK, T>(h, l); break;
+    case ERLBM_COL_MRT: launch_lp_step2_ct<COL_MRT, T>(h, l); break;
+    case ERLBM_COL_REG: launch_lp_step2_ct<COL_REG, T>(h, l); break;
+    case ERLBM_COL_EXT_REG: launch_lp_step2_ct<COL_EXT_REG, T>(h, l); break;
+    default: return ERLBM_ERR_UNSUPPORTED;
+    }
+    ERLBM_HIP_TRY(hipGetLastError());
+    return ERLBM_OK;
+}
+
 LPArgs make_args(erlbm_lattice *h, const StepArgs &a)
 {
     LPArgs l;
@@ -377,6 +479,12 @@ int lowp_step(erlbm_lattice *h, const StepArgs &a, bool stream)
 {
     const LPArgs l = make_args(h, a);
     ERLBM_LP_DISPATCH(step_qt, l, stream);
+}
+
+int lowp_step2(erlbm_lattice *h, const StepArgs &a)
+{
+    const LPArgs l = make_args(h, a);
+    return h->elem == 4 ? launch_lp_step2_t<float>(h, l) : launch_lp_step2_t<_Float16>(h, l);
 }
 
 int lowp_from_aos(erlbm_lattice *h, const double *aos, void *dst) { ERLBM_LP_DISPATCH(from_aos_qt, aos, dst); }

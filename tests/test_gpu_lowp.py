@@ -103,3 +103,32 @@ def test_shifted_energy_monitor_fp64(gpu, orc):
         e = lat.kinetic_energy(gpu.STATE, ordered=True)
     assert np.max(np.abs(back - f0)) < 1e-8
     assert e == orc.kinetic_energy(19, back)
+
+
+@pytest.mark.parametrize("prec", ["FP32_SHIFTED", "FP16_SHIFTED"])
+@pytest.mark.parametrize("shape,remap", [(11, 1), (13, 0)])
+def test_two_step_strip_equals_one_step(gpu, monkeypatch, prec, shape, remap):
+    """The two-step strip walk keeps the step-k populations in LDS in the
+    storage type, so its results equal the one-step shifted path's bitwise
+    (any call pattern, monitors included)."""
+    L = 256
+    tau = 0.5 + 3. * (1. / 32.) * L / 80000.
+    f0 = gpu.init_double_shear(L)
+    p = getattr(gpu, prec)
+    monkeypatch.setenv("ERLBM_FUSE", "0")
+    ref = gpu.Lattice(9, L, L, 1, precision=p)
+    monkeypatch.setenv("ERLBM_FUSE", str(shape))
+    monkeypatch.setenv("ERLBM_FUSE_REMAP", str(remap))
+    fused = gpu.Lattice(9, L, L, 1, precision=p)
+    with ref, fused:
+        for lat in (ref, fused):
+            lat.set_params(tau, 0.65, 1.0, O.COL_EXT_REG)
+            lat.upload(f0)
+        done = 0
+        for n in (1, 8, 3, 6):
+            ref.step(n)
+            fused.step(n)
+            done += n
+            assert fused.status() == ref.status() == (done, 0)
+            assert np.array_equal(fused.download(), ref.download())
+            assert fused.kinetic_energy(gpu.POST_COLLISION) == ref.kinetic_energy(gpu.POST_COLLISION)
