@@ -33,6 +33,17 @@ void set_last_error(const char *what, hipError_t e);
 void set_last_error_msg(const char *msg);
 
 // Wavefront (64-lane) sum via shuffles; lane order fixed, so reproducible.
+// Workgroup barrier that orders LDS only: waits for this wave's LDS
+// accesses (lgkmcnt) and meets the other waves, but leaves global loads and
+// stores in flight -- __syncthreads() also waits for those (its release
+// fence covers global memory), which would drain a prefetch at every
+// barrier.  The "memory" clobber keeps the compiler from moving memory
+// accesses across it.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll

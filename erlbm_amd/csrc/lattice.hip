@@ -291,14 +291,16 @@ __global__ __launch_bounds__(BLOCK) void k_step2_d2q9(StepArgs a, int remap)
 // be long in y (halo 2 rows per SEG) and wide in x (halo 2 columns per TX)
 // without the LDS of a whole 2-D tile.  Thread t computes step-k position
 // x0-1+t (t < TX+2) and step-(k+1) position x0+t (t < TX); the loads of the
-// next step-k row are in flight while the current output row is computed.
-// (Two rows in flight need 138 instead of 104 VGPRs and measured slower.)
+// next PD step-k rows are in flight while the current output row is
+// computed.  (PD = 2 and 4 -- 138 and 202 VGPRs -- measured 14-25% slower
+// than PD = 1 at L = 2048, so PD = 1 is the one instantiated.)
 template <int TX>
 constexpr int strip_threads() { return TX + 64; }
 
-template <int COL, int TX, int SEG>
+template <int COL, int TX, int SEG, int PD>
 __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a, int remap)
 {
+    static_assert(PD >= 1 && SEG % PD == 0, "rows in flight must divide the walk");
     using V = VSet<9>;
     constexpr int EX = TX + 2;
     extern __shared__ double ring[];   // [3][9][EX]
@@ -346,23 +348,27 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
             });
         }
     };
-    double fa[9], fb[9];
-    load_row(ya - 1, fa);
+    // PD rows of step-k loads in flight: register set J holds row y+1 when
+    // output row y = ya + r, r = J (mod PD), is next
+    double fq[PD][9], fb[9];
+    load_row(ya - 1, fq[0]);
     load_row(ya, fb);
     if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
-    collide_to_ring(ya - 1, fa);
+    collide_to_ring(ya - 1, fq[0]);
     collide_to_ring(ya, fb);
-    load_row(ya + 1, fa);
+    static_for<PD>([&](auto J) { load_row(ya + 1 + (int)J, fq[(int)J]); });
     // Three ring slots: row y+1 overwrites row y-2, which the previous
     // iteration read, hence the second barrier.  (Four slots and one
     // barrier per row measured the same, with a third more LDS.)
-    for (int r = 0; r < SEG; r++) {
-        const int y = ya + r;
-        collide_to_ring(y + 1, fa);           // step k, row y+1
-        if (r + 1 < SEG) load_row(y + 2, fa); // in flight while row y is computed
-        __syncthreads();
-        output_row(y);
-        __syncthreads();
+    for (int r = 0; r < SEG; r += PD) {
+        static_for<PD>([&](auto J) {
+            const int y = ya + r + (int)J;
+            collide_to_ring(y + 1, fq[(int)J]);                          // step k, row y+1
+            if (y + 1 + PD <= ya + SEG) load_row(y + 1 + PD, fq[(int)J]); // in flight meanwhile
+            lds_barrier();
+            output_row(y);
+            lds_barrier();
+        });
     }
 }
 
@@ -1005,20 +1011,20 @@ void launch_fused_t(erlbm_lattice *h, const StepArgs &a)
     hipLaunchKernelGGL((k_step2_d2q9<COL, TX, TY>), dim3(nb), dim3(BLOCK), lds, h->stream, a, h->tune_fuse_remap);
 }
 
-template <int COL, int TX, int SEG>
+template <int COL, int TX, int SEG, int PD = 1>
 void launch_strip_t(erlbm_lattice *h, const StepArgs &a)
 {
     constexpr size_t lds = sizeof(double) * 3 * 9 * (TX + 2);
     if constexpr (lds > 65536) {
         static bool raised = false;
         if (!raised) {
-            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step2_strip<COL, TX, SEG>),
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step2_strip<COL, TX, SEG, PD>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             raised = true;
         }
     }
     const unsigned nb = (unsigned)((h->nx / TX) * (h->ny / SEG));
-    hipLaunchKernelGGL((k_step2_strip<COL, TX, SEG>), dim3(nb), dim3(strip_threads<TX>()), lds, h->stream, a,
+    hipLaunchKernelGGL((k_step2_strip<COL, TX, SEG, PD>), dim3(nb), dim3(strip_threads<TX>()), lds, h->stream, a,
                        h->tune_fuse_remap);
 }
 

@@ -320,9 +320,9 @@ __global__ __launch_bounds__(lp_strip_threads<TX>()) void k_lp_step2_strip(LPArg
         const int y = ya + r;
         collide_to_ring(y + 1, da);
         if (r + 1 < SEG) load_row(y + 2, da);
-        __syncthreads();
+        lds_barrier();
         output_row(y);
-        __syncthreads();
+        lds_barrier();
     }
 }
 
