@@ -277,7 +277,9 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": head["launch"], "steps_per_launch": head["steps_per_launch"],
                      "algorithmic_bytes_per_launch": BYTES_PER_NODE[a.precision] * L * L,
-                     "ms_per_launch": head["ms_per_launch"]},
+                     "ms_per_launch": head["ms_per_launch"],
+                     # what a one-step-per-launch kernel would have to stream for this MLUPS
+                     "one_step_equivalent_GBs": round(head["MLUPS"] * 1e6 * BYTES_PER_NODE[a.precision] / 1e9, 1)},
         "cpu_baseline": cpu,
         "copy_kernel_GBs": round(copy_gbs, 1),
         "mass_rel_error": head["mass_rel_error"],

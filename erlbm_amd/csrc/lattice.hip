@@ -999,14 +999,9 @@ template <int COL, int TX, int TY>
 void launch_fused_t(erlbm_lattice *h, const StepArgs &a)
 {
     constexpr size_t lds = Fuse2<TX, TY>::LDS_BYTES;
-    if constexpr (lds > 65536) {
-        static bool raised = false;   // per process; the limit is a property of the kernel
-        if (!raised) {
-            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step2_d2q9<COL, TX, TY>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            raised = true;
-        }
-    }
+    if constexpr (lds > 65536)   // above the default dynamic-LDS limit (gfx950 has 160 KB)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step2_d2q9<COL, TX, TY>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     const unsigned nb = (unsigned)((h->nx / TX) * (h->ny / TY));
     hipLaunchKernelGGL((k_step2_d2q9<COL, TX, TY>), dim3(nb), dim3(BLOCK), lds, h->stream, a, h->tune_fuse_remap);
 }
@@ -1015,14 +1010,9 @@ template <int COL, int TX, int SEG, int PD = 1>
 void launch_strip_t(erlbm_lattice *h, const StepArgs &a)
 {
     constexpr size_t lds = sizeof(double) * 3 * 9 * (TX + 2);
-    if constexpr (lds > 65536) {
-        static bool raised = false;
-        if (!raised) {
-            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step2_strip<COL, TX, SEG, PD>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            raised = true;
-        }
-    }
+    if constexpr (lds > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step2_strip<COL, TX, SEG, PD>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     const unsigned nb = (unsigned)((h->nx / TX) * (h->ny / SEG));
     hipLaunchKernelGGL((k_step2_strip<COL, TX, SEG, PD>), dim3(nb), dim3(strip_threads<TX>()), lds, h->stream, a,
                        h->tune_fuse_remap);
