@@ -182,14 +182,26 @@ def maps_section(E):
 
 
 def serving_section(n: int = 120, rate: float = 30.0):
+    """The seeded serving mix, continuous batching vs one run at a time on
+    the plain FP64 path; map points once in FP64 and once in FP32-shifted
+    storage (FP64 below tau - 1/2 < 1e-3), with the classification agreement
+    of the two."""
     from erlbm_amd import serve as S
     import erlbm_amd as E
-    served = S.run_served(S.poisson_mix(n, rate, map_precision=E.FP64))
-    plain = S.run_plain(S.poisson_mix(n, rate, map_precision=E.FP64))
     keep = ("requests", "wall_s", "throughput_req_per_s", "p50_s", "p99_s")
+    mix64 = S.poisson_mix(n, rate, map_precision=E.FP64)
+    served = S.run_served(mix64)
+    mix32 = S.poisson_mix(n, rate, map_precision=E.FP_AUTO)
+    served32 = S.run_served(mix32)
+    maps = [(a, b) for a, b in zip(mix64, mix32) if a.req.kind in (S.REQ_TG_MAP_POINT, S.REQ_OBSTACLE_MAP_POINT)]
+    agree = sum(a.response["unstable"] == b.response["unstable"] for a, b in maps) / max(1, len(maps))
+    plain = S.run_plain(S.poisson_mix(n, rate, map_precision=E.FP64))
     return {"mix": f"{n} seeded Poisson arrivals at {rate}/s: 60% TG map points, 25% obstacle map points, "
                    "10% D2Q9 TG L in {128,256,512}, 5% D2Q9 double shear L=512",
-            "served": {k: served[k] for k in keep}, "plain": {k: plain[k] for k in keep}}
+            "served": {k: served[k] for k in keep},
+            "served_fp32_maps": {**{k: served32[k] for k in keep}, "map_points": len(maps),
+                                 "classification_agreement_vs_fp64": round(agree, 4)},
+            "plain": {k: plain[k] for k in keep}}
 
 
 def main() -> int:

@@ -129,3 +129,34 @@ def run_obstacle_map(precision=L.FP64, n=400, ma=0.1, batch: Optional[Batch] = N
         if own:
             b.close()
     return res
+
+
+def main(argv=None) -> int:
+    """python -m erlbm_amd.maps [--map tg|tg_reg|obstacle] [--n 400] [--precision fp64|fp32|auto]:
+    one full stability map, timed; prints one JSON line."""
+    import argparse
+    import json
+    import time
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--map", default="tg", choices=["tg", "tg_reg", "obstacle"])
+    ap.add_argument("--n", type=int, default=400)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "auto"])
+    a = ap.parse_args(argv)
+    prec = {"fp64": L.FP64, "fp32": L.FP32_SHIFTED, "auto": L.FP_AUTO}[a.precision]
+    if a.map == "obstacle":
+        jobs, mk, nf = obstacle_map_jobs(a.n, a.n, 0.1, prec), obstacle_batch, 232
+    else:
+        col, tnh = (L.COL_REG, 1.0) if a.map == "tg_reg" else (L.COL_EXT_REG, 0.65)
+        jobs, mk, nf = tg_map_jobs(a.n, a.n, col, tnh, prec), tg_batch, 256
+    with mk() as b:
+        t = time.perf_counter()
+        res = b.run(jobs)
+        dt = time.perf_counter() - t
+    print(json.dumps({"map": a.map, "points": int(jobs.size), "seconds": round(dt, 3),
+                      "points_per_s": round(jobs.size / dt, 1), "unstable": int(res["unstable"].sum()),
+                      "node_steps_per_s": float(res["steps"].astype(float).sum() * nf / dt)}))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

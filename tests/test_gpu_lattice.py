@@ -300,3 +300,50 @@ def test_two_step_fused_kernel_bitwise(gpu, orc, monkeypatch, shape):
         ref.step(4)
         assert fused.status() == ref.status()
         assert np.array_equal(fused.download(), ref.download())
+
+
+@pytest.mark.parametrize("q,dims,seed", [(9, (300, 77, 1), 1), (9, (5, 3, 1), 2), (19, (37, 19, 11), 3),
+                                         (19, (4, 3, 2), 4)])
+def test_ragged_sizes_random_solids(gpu, orc, q, dims, seed):
+    """Sizes that are no multiple of any block or tile, down to a few
+    voxels, with random solids (bounce-back everywhere): bitwise the oracle,
+    including zero-step calls and the state / post-collision downloads."""
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(dims))
+    m = (rng.uniform(size=n) > 0.3).astype(np.int32)
+    m[0] = 1
+    ini, st, nf = orc.stream_map(q, *dims, m)
+    feq = orc.dist_eq(q, 0.02, -0.01, 0.01 if q == 19 else 0.0, 1.0)
+    f0 = feq[None] * (1 + 1e-3 * rng.uniform(-1, 1, (nf, q)))
+    with gpu.Lattice(q, *dims, mask=m) as lat:
+        lat.set_params(0.57, 0.68, 1.0, O.COL_EXT_REG)
+        lat.upload(f0)
+        lat.step(0)
+        assert lat.status() == (0, 0)
+        assert np.array_equal(lat.download(), f0)
+        lat.step(13)
+        got = lat.download()
+    want = orc.run_bc_steps(q, f0, *dims, ini, st, 13, O.COL_EXT_REG, 0.57, 0.68)[0]
+    assert np.array_equal(got, want)
+
+
+def test_lattice_errors_leave_the_handle_usable(gpu, orc):
+    """Invalid calls return an error code with a message and change nothing."""
+    L = 32
+    f0 = gpu.init_double_shear(L)
+    with gpu.Lattice(9, L, L, 1) as lat:
+        with pytest.raises(RuntimeError, match="upload"):
+            lat.step(1)
+        with pytest.raises(ValueError):
+            lat.upload(f0[:-1])
+        lat.set_params(0.6, 0.65, 1.0, O.COL_EXT_REG)
+        with pytest.raises(RuntimeError):
+            lat.set_params(-1.0, 0.65, 1.0, O.COL_EXT_REG)
+        with pytest.raises(RuntimeError):
+            lat.set_params(0.6, 0.65, 1.0, 7)
+        lat.upload(f0)
+        with pytest.raises(RuntimeError):
+            lat.download(gpu.POST_COLLISION)      # no step since the upload
+        lat.step(3)
+        ini, st, nf = orc.stream_map(9, L, L, 1)
+        assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, 3, O.COL_EXT_REG, 0.6, 0.65))

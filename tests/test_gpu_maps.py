@@ -168,3 +168,25 @@ def test_batch_autotune_keeps_results(gpu):
         assert pick["wg_per_cu_f64"] >= 1 and pick["wg_per_cu_f32"] >= 1
         assert pick["sample_points_per_s"] > 0
         assert b.run(jobs).tobytes() == ref.tobytes()
+
+
+def test_batch_edge_cases(gpu):
+    """An empty job list, zero-step jobs and a single job; invalid jobs are
+    rejected before anything runs."""
+    with gpu.tg_batch() as b:
+        assert b.run(gpu.tg_map_jobs(400, 400, O.COL_EXT_REG, 0.65, gpu.FP64, 0, 0, 0, 0)).size == 0
+        jobs = gpu.tg_map_jobs(400, 400, O.COL_EXT_REG, 0.65, gpu.FP64, 10, 13, 20, 21)
+        jobs["n_steps"][:] = 0
+        res = b.run(jobs)
+        assert (res["steps"] == 0).all() and (res["unstable"] == 0).all() and (res["done"] == 1).all()
+        one = gpu.tg_map_jobs(400, 400, O.COL_EXT_REG, 0.65, gpu.FP64, 399, 400, 0, 1)
+        one["n_steps"] = 300
+        assert b.run(one)["steps"][0] in (100, 200, 300)
+        bad = one.copy()
+        bad["tau"] = -1.0
+        with pytest.raises(RuntimeError, match="invalid job"):
+            b.run(bad)
+        bad = one.copy()
+        bad["collision"] = O.COL_MRT          # D2Q9-only operator on a D3Q19 batch
+        with pytest.raises(RuntimeError):
+            b.run(bad)
