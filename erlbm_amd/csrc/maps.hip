@@ -243,6 +243,28 @@ __device__ __forceinline__ void gather(const T *s, int nvp, int x, int y, int nx
     });
 }
 
+// The LDS element offsets gather() reads for a streaming step, computed once
+// per job: the pulls of a voxel never change during a run.
+__device__ __forceinline__ void pull_offsets(int nvp, int x, int y, int nx, int ny, uint32_t bounce, int (&off)[Q])
+{
+    using V = VSet<Q>;
+    const int v = x + nx * y;
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        if (V::cx[i] == 0 && V::cy[i] == 0) {
+            off[i] = i * nvp + v;
+        } else if (bounce & (1u << i)) {
+            off[i] = V::opp[i] * nvp + v;
+        } else {
+            const int xs = V::cx[i] == 0 ? x : (V::cx[i] > 0 ? xm : xp);
+            const int ys = V::cy[i] == 0 ? y : (V::cy[i] > 0 ? ym : yp);
+            off[i] = i * nvp + xs + nx * ys;
+        }
+    });
+}
+
 // Ordered sum of n doubles held in LDS (the reference's sequential loop).
 __device__ __forceinline__ double ordered_sum(const double *v, int n)
 {
@@ -332,18 +354,26 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
     double e_last = resume ? prev.energy : e0;
     double e_max = e0;      // largest energy seen by the energy test
     double r_max = 0.0;     // largest |rho| seen by this thread's density test
+    // The voxel this thread gathers: its own, or on the outlet column the
+    // upstream neighbour, whose populations devnull_rho_x copies
+    // (Boundary_conditions.cpp:117-126).  Its pull offsets are fixed for the
+    // run, so they are computed once.
+    const bool outlet_copy = GENERAL && active && my_x == a.g.outlet_x && out_fluid;
+    const int gx = outlet_copy ? my_x + a.g.outlet_dx : my_x;
+    int off[Q];
+    if (active) pull_offsets(nvp, gx, my_y, nx, ny, outlet_copy ? out_bounce : my_bounce, off);
     for (; s < end; s++) {
-        const bool stream = s > 0;
         double fator = 0.0;   // outlet column: rho0 / rho(upstream neighbour)
         if (active) {
-            if (GENERAL && my_x == a.g.outlet_x && out_fluid) {
-                // devnull_rho_x copies the upstream neighbour (Boundary_conditions.cpp:117-126)
-                gather(sf, nvp, my_x + a.g.outlet_dx, my_y, nx, ny, out_bounce, stream, f);
+            if (s > 0) {
+                static_for<Q>([&](auto I) { f[(int)I] = sf[off[(int)I]]; });
+            } else {                 // first step after the initial state: no stream
+                static_for<Q>([&](auto I) { f[(int)I] = sf[(int)I * nvp + gx + nx * my_y]; });
+            }
+            if (outlet_copy) {
                 double fn[Q];
                 P::to_abs(f, fn, rho0);
                 fator = rho0 / density_exact<Q>(fn);
-            } else {
-                gather(sf, nvp, my_x, my_y, nx, ny, my_bounce, stream, f);
             }
         }
         if (J.check_every > 0 && s > 0 && s % J.check_every == 0) {
