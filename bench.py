@@ -120,6 +120,31 @@ def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True):
                                   "download_GBs": round(aos_bytes / t_down / 1e9, 2)}}
 
 
+def cpu_maps_baseline():
+    """The reference's own map-point runner (roda_ponto of both map drivers,
+    compiled from its sources into oracle/_ref, OpenMP over points as the
+    drivers run) on an 8 x 8 subsample of the 400 x 400 grids (i, j =
+    50k + 19, 50k), on the host cores."""
+    import oracle
+    if not oracle.Reference.available(19):
+        return None
+    n_threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    ref = oracle.Reference(19)
+    ii = np.array([50 * a + 19 for a in range(8) for _ in range(8)], np.int32)
+    jj = np.array([50 * b for _ in range(8) for b in range(8)], np.int32)
+    out = {"cores": n_threads, "kind": "reference", "sample": "8 x 8 points (i = 50a+19, j = 50b) of each 400 x 400 map"}
+    t = time.perf_counter()
+    ref.tg_map_points(E_COL_EXT_REG, 0.65, ii, jj)
+    out["tg_erlbm065_points_per_s"] = round(ii.size / (time.perf_counter() - t), 2)
+    t = time.perf_counter()
+    ref.obstacle_map_points(ii, jj)
+    out["obstacle_points_per_s"] = round(ii.size / (time.perf_counter() - t), 2)
+    return out
+
+
+E_COL_EXT_REG = 4
+
+
 def d3q19_section(E, n: int = 256, steps: int = 200):
     """The D3Q19 plain path at n^3 (periodic, E-RLBM, FP64): 304 algorithmic
     bytes per node-step (19 FP64 populations read + 19 written)."""
@@ -255,6 +280,12 @@ def main() -> int:
         extra["precision"] = prec
         extra["d3q19"] = d3q19_section(E)
         extra["maps"] = maps_section(E)
+        if not a.no_cpu:
+            cm = cpu_maps_baseline()
+            if cm:
+                cm["gpu_speedup"] = {"tg": round(extra["maps"]["tg_erlbm065"]["points_per_s"] / cm["tg_erlbm065_points_per_s"], 1),
+                                     "obstacle": round(extra["maps"]["obstacle"]["points_per_s"] / cm["obstacle_points_per_s"], 1)}
+            extra["maps"]["cpu_baseline"] = cm
         extra["serving"] = serving_section()
 
     traffic = None
