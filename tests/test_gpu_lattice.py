@@ -347,3 +347,39 @@ def test_lattice_errors_leave_the_handle_usable(gpu, orc):
         lat.step(3)
         ini, st, nf = orc.stream_map(9, L, L, 1)
         assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, 3, O.COL_EXT_REG, 0.6, 0.65))
+
+
+def test_headline_two_step_kernel_full_size(gpu, orc):
+    """The headline workload as the bench runs it (D2Q9 double shear,
+    L = 2048, default two-step strip kernel): 25 steps bitwise the strict
+    oracle; total mass conserved to rounding."""
+    L, n = 2048, 25
+    tau = 0.5 + 3. * (1. / 32.) * L / 80000.
+    f0 = gpu.init_double_shear(L)
+    ini, st, nf = orc.stream_map(9, L, L, 1)
+    want = orc.run_steps(9, f0, st, n, O.COL_EXT_REG, tau, 0.65, nthreads=16)
+    with _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0) as lat:
+        lat.step(n)
+        got = lat.download()
+    assert np.array_equal(got, want)
+    assert abs(got.sum() - f0.sum()) <= 1e-12 * f0.sum()
+
+
+def test_two_step_kernel_fma_tolerance_1000_steps(gpu, monkeypatch):
+    """The north-star tolerance on the two-step kernel: <= 1e-10 max relative
+    population error against the FMA-contracted oracle after 1000 steps
+    (D2Q9 double shear, L = 512, 256-wide strips)."""
+    import oracle
+    monkeypatch.setenv("ERLBM_FUSE", "11")
+    ofma = oracle.Oracle(fma=True)
+    L = 512
+    tau = 0.5 + 3. * (1. / 32.) * L / 80000.
+    f0 = gpu.init_double_shear(L)
+    ini, st, nf = ofma.stream_map(9, L, L, 1)
+    want = ofma.run_steps(9, f0, st, 1000, O.COL_EXT_REG, tau, 0.65, nthreads=16)
+    with _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0) as lat:
+        lat.step(1000)
+        got = lat.download()
+    err = _rel(got, want)
+    print(f"two-step kernel vs FMA oracle, L={L}, 1000 steps: max rel err {err:.2e}", flush=True)
+    assert err <= TOL_FMA
