@@ -372,6 +372,146 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
     }
 }
 
+// Lane l of a wavefront receives the value of lane l-1 (WAVE_SHR1) or l+1
+// (WAVE_SHL1): one DPP move per dword, no LDS.  Lanes with no source keep 0.
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
+template <int CTRL>
+__device__ __forceinline__ double wave_shift(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Two steps per launch, one wavefront per strip, with neither LDS nor
+// barriers: every wavefront walks its strip on its own.  Lane l computes
+// step k at x = x0-1+l and keeps the step-k results of the last three rows
+// in registers; the step-(k+1) pull takes a c_y = +-1 population from the
+// row before / after in registers and a c_x = +-1 one from lane l-1 / l+1
+// by a DPP move.  Lanes 1..62 write step k+1, so a strip is 62 wide (the
+// last one ragged); a wavefront walks a segment of the strip's rows, and
+// the loads of the next PD step-k rows are in flight while a row is output.
+// Odd segments walk upwards: the two step-k rows that neighbouring segments
+// both read are then read by both at the start or both at the end of their
+// walks, while they are still in L2.
+constexpr int WAVE_STRIP = 62;
+
+typedef unsigned int wave_u2 __attribute__((ext_vector_type(2)));
+constexpr int BUF_DWORD3 = 0x00020000;   // raw (stride 0) buffer of dwords
+constexpr int BUF_NT = 2;                // nontemporal cache policy
+// A VGPR offset at or above the buffer's size is out of range: the load
+// returns 0 without touching memory, the store is dropped.  Masking lanes
+// and skipped prefetches this way keeps the walk free of branches around
+// memory instructions, so the wait for a row's loads never has to wait for
+// the stores issued after them.
+constexpr unsigned BUF_OOB = 0x80000000u;
+
+template <int COL, int PD, int DIR>
+__device__ __forceinline__ void wave_walk(const StepArgs &a, int x0, int ya, int rows)
+{
+    using V = VSet<9>;
+    const int nx = a.g.nx, ny = a.g.ny;
+    const int lane = (int)(threadIdx.x & 63);
+    const int xk = wrap(x0 - 1 + lane, nx);     // step-k column of this lane
+    const bool out = lane >= 1 && lane <= WAVE_STRIP && x0 + lane - 1 < nx;
+    const int y0 = DIR > 0 ? ya : ya + rows - 1;   // first output row
+    // Raw buffers over the population arrays: the per-lane column is the
+    // VGPR offset, the wave-uniform plane + row the SGPR offset of a load
+    // (the launcher checks that every byte offset is below 2^31).
+    const unsigned nbytes = (unsigned)(8 * at(a.g, 8, (long long)ny - 1, nx));
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)a.src, (short)0, (int)nbytes, BUF_DWORD3);
+    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc((void *)a.dst, (short)0, (int)nbytes, BUF_DWORD3);
+    const unsigned vo[3] = {8u * wrap(xk + 1, nx), 8u * xk, 8u * wrap(xk - 1, nx)};   // columns of c_x = -1, 0, +1
+    const unsigned vst = out ? 8u * (x0 + lane - 1) : BUF_OOB;
+    // pulls of step-k row y (logical, -1 - PD .. ny + PD); skip: no memory access
+    auto load_row = [&](int y, double (&f)[9], bool skip) {
+        const unsigned off = skip ? BUF_OOB : 0u;
+        const int yw = wrap(y, ny);
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            const int ys = V::cy[i] == 0 ? yw : wrap(yw - V::cy[i], ny);
+            const int so = (int)(8 * at(a.g, i, ys, 0));
+            f[i] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                                  rs, vo[V::cx[i] + 1] + off, so, V::cx[i] == 0 ? BUF_NT : 0));
+        });
+    };
+    auto collide = [&](double (&f)[9]) {
+        double vx, vy, vz;
+        collide_exact<9, COL>(f, a.k, vx, vy, vz);
+    };
+    // step-k results of output row y: hb row y-DIR, h0 row y, hn row y+DIR
+    double hb[9], h0[9], hn[9], fq[PD][9];
+    load_row(y0 - DIR, hb, false);
+    load_row(y0, h0, false);
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+    static_for<PD>([&](auto J) { load_row(y0 + DIR * (1 + (int)J), fq[(int)J], (int)J >= rows); });
+    collide(hb);
+    collide(h0);
+    // nine dropped stores: entering the loop, the prefetched rows are then
+    // followed by nine stores as at its back edge, so the wait for them at
+    // the top of the loop can leave the previous row's stores outstanding
+    static_for<9>([&](auto I) {
+        __builtin_amdgcn_raw_buffer_store_b64(wave_u2{0u, 0u}, rd, BUF_OOB + 8u * (int)I, 0, BUF_NT);
+    });
+    // output row number k of the walk from register set j
+    auto row = [&](int k, auto J) {
+        constexpr int j = J;
+        const int y = y0 + DIR * k;
+        static_for<9>([&](auto I) { hn[(int)I] = fq[j][(int)I]; });
+        collide(hn);                                                       // step k, row y+DIR
+        load_row(y + DIR * (1 + PD), fq[j], k + 1 + PD > rows);           // in flight meanwhile
+        double g[9];
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            // population i of row y pulls from row y - c_y
+            const double v = V::cy[i] == DIR ? hb[i] : (V::cy[i] == -DIR ? hn[i] : h0[i]);
+            if constexpr (V::cx[i] > 0) g[i] = wave_shift<DPP_WAVE_SHR1>(v);
+            else if constexpr (V::cx[i] < 0) g[i] = wave_shift<DPP_WAVE_SHL1>(v);
+            else g[i] = v;
+        });
+        collide(g);                                                        // step k+1, row y
+        static_for<9>([&](auto I) {
+            const unsigned so = (unsigned)(8 * at(a.g, (int)I, y, 0));
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(wave_u2, g[(int)I]), rd, vst + so, 0, BUF_NT);
+        });
+        static_for<9>([&](auto I) {
+            hb[(int)I] = h0[(int)I];
+            h0[(int)I] = hn[(int)I];
+        });
+    };
+    int k = 0;
+    for (; k + PD <= rows; k += PD) static_for<PD>([&](auto J) { row(k + (int)J, J); });
+    static_for<PD - 1>([&](auto J) {    // rows % PD left over
+        if (k + (int)J < rows) row(k + (int)J, J);
+    });
+}
+
+// One launch: nstrip x nseg wavefronts, the rows of a strip split evenly
+// among its nseg segments (the launcher sizes nseg so that every wavefront
+// is resident at once: a second, partial round of wavefronts would walk its
+// rows at latency-bound speed after the first one is done).
+template <int COL, int PD, int WPB, int WPE>
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(WPE)))
+void k_step2_wave(StepArgs a, int nstrip, int nseg, int remap)
+{
+    const int ny = a.g.ny;
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    // the wavefront index is uniform across the wavefront; readfirstlane
+    // lets the compiler keep everything derived from it in SGPRs
+    const int wv = (int)b * WPB + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (wv >= nstrip * nseg) return;
+    const int seg = wv / nstrip;
+    const int x0 = (wv % nstrip) * WAVE_STRIP;
+    const int ya = (int)((long long)seg * ny / nseg);
+    const int rows = (int)((long long)(seg + 1) * ny / nseg) - ya;
+    if (seg & 1) wave_walk<COL, PD, -1>(a, x0, ya, rows);
+    else wave_walk<COL, PD, 1>(a, x0, ya, rows);
+}
+
 // D2Q9 periodic kernel, two x-adjacent nodes per thread: every population
 // plane is read with one 16-byte load per thread (8-byte aligned when the
 // plane is x-shifted; multi-dword global loads need 4-byte alignment) and
@@ -953,16 +1093,48 @@ extern "C" int erlbm_upload_aos(erlbm_lattice *h, const double *inif)
 
 namespace {
 
-// Tile shapes of the two-step kernel (ERLBM_FUSE selects one; 0 = off).
+// Shapes of the two-step kernel (ERLBM_FUSE selects one; 0 = off).
+enum FuseKind { FUSE_TILE, FUSE_STRIP, FUSE_WAVE };
 struct FuseShape {
-    int tx, ty;       // tile, or (strip) strip width and rows walked
-    bool strip;
+    int tx, ty;       // tile; (strip) strip width and rows walked; (wave) rows walked
+    FuseKind kind;
+    int pd, wpb;      // (wave) rows of loads in flight, wavefronts per workgroup
 };
-constexpr FuseShape FUSE_SHAPES[] = {{0, 0, false},   {64, 8, false},   {32, 16, false}, {32, 8, false},
-                                     {128, 4, false}, {16, 16, false},  {32, 32, false}, {64, 4, false},
-                                     {128, 8, false}, {256, 4, false},  {64, 16, false}, {256, 32, true},
-                                     {256, 16, true}, {128, 32, true},  {256, 64, true}, {128, 64, true}};
+constexpr FuseShape FUSE_SHAPES[] = {
+    {0, 0, FUSE_TILE},     {64, 8, FUSE_TILE},    {32, 16, FUSE_TILE},   {32, 8, FUSE_TILE},
+    {128, 4, FUSE_TILE},   {16, 16, FUSE_TILE},   {32, 32, FUSE_TILE},   {64, 4, FUSE_TILE},
+    {128, 8, FUSE_TILE},   {256, 4, FUSE_TILE},   {64, 16, FUSE_TILE},   {256, 32, FUSE_STRIP},
+    {256, 16, FUSE_STRIP}, {128, 32, FUSE_STRIP}, {256, 64, FUSE_STRIP}, {128, 64, FUSE_STRIP},
+    // wavefront strips: ty = rounds of resident wavefronts per launch
+    {WAVE_STRIP, 2, FUSE_WAVE, 1, 4}, {WAVE_STRIP, 1, FUSE_WAVE, 1, 4}, {WAVE_STRIP, 2, FUSE_WAVE, 2, 4}};
 constexpr int N_FUSE_SHAPES = sizeof(FUSE_SHAPES) / sizeof(FUSE_SHAPES[0]);
+
+// Segments per strip for the wavefront-strip kernel: `rounds` times as many
+// wavefronts as the GPU holds at once, at most one per row.
+int wave_segments(const erlbm_lattice *h, int waves_per_cu, int rounds)
+{
+    const int nstrip = (h->nx + WAVE_STRIP - 1) / WAVE_STRIP;
+    const long long cap = (long long)h->n_cu * waves_per_cu * rounds;
+    return (int)std::max(1ll, std::min<long long>(h->ny, cap / nstrip));
+}
+
+// Whether the lattice splits into whole tiles / strips of the shape (the
+// wavefront strips take ragged edges but need 64 distinct columns).
+bool fuse_geometry_fits(const erlbm_lattice *h, const FuseShape &fs)
+{
+    if (fs.kind == FUSE_WAVE) return h->nx >= 64 && h->extent * 8 < (1ll << 31);   // 32-bit buffer offsets
+    return h->nx % fs.tx == 0 && h->ny % fs.ty == 0;
+}
+
+// Workgroups of one launch of the shape.
+long long fuse_workgroups(const erlbm_lattice *h, const FuseShape &fs)
+{
+    if (fs.kind == FUSE_WAVE) {   // (as if four wavefronts per SIMD fit)
+        const long long waves = (long long)((h->nx + WAVE_STRIP - 1) / WAVE_STRIP) * wave_segments(h, 16, fs.ty);
+        return (waves + fs.wpb - 1) / fs.wpb;
+    }
+    return (long long)(h->nx / fs.tx) * (h->ny / fs.ty);
+}
 
 bool fuse_shape_fits(const erlbm_lattice *h, int f)
 {
@@ -972,25 +1144,25 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
     // explicit request (ERLBM_FUSE) only -- its 16-byte vector kernel is
     // faster (36.8 vs 53.2 us/step at L = 2048: the two collisions per
     // launch make the walk VALU-bound once the bytes are that few)
-    if (h->elem != 8 && (h->f32_arith || !fs.strip || (f != 11 && f != 13))) return false;
+    if (h->elem != 8 && (h->f32_arith || fs.kind != FUSE_STRIP || (f != 11 && f != 13))) return false;
     if (h->elem == 2 && !h->fuse_requested) return false;
-    return h->nx % fs.tx == 0 && h->ny % fs.ty == 0;
+    return fuse_geometry_fits(h, fs);
 }
 
 // The two-step shape a call of erlbm_step uses (0: one step per launch).
 // Without an explicit ERLBM_FUSE, a shape must give every CU a workgroup
 // (small lattices have too few strips or tiles to fill the chip: there the
-// one-step kernel is faster); the 64x8 tiles are the fallback.
+// one-step kernel is faster); the 64x8 tiles are the fallback, and the
+// wavefront strips, which take any width from 64 up, the last resort.
 int fuse_shape(const erlbm_lattice *h, int energy_interval)
 {
     if (!(h->q == 9 && h->nz == 1 && h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0 && h->dens_limit <= 0.0 &&
           energy_interval <= 0 && !h->checked))
         return 0;
     if (h->fuse_requested) return fuse_shape_fits(h, h->tune_fuse) ? h->tune_fuse : 0;
-    for (int f : {h->tune_fuse, 1}) {
+    for (int f : {h->tune_fuse, 1, 16}) {
         if (!fuse_shape_fits(h, f)) continue;
-        const FuseShape fs = FUSE_SHAPES[f];
-        if ((long long)(h->nx / fs.tx) * (h->ny / fs.ty) >= h->n_cu) return f;
+        if (fuse_workgroups(h, FUSE_SHAPES[f]) >= h->n_cu) return f;
     }
     return 0;
 }
@@ -1018,10 +1190,30 @@ void launch_strip_t(erlbm_lattice *h, const StepArgs &a)
                        h->tune_fuse_remap);
 }
 
+template <int COL, int PD, int WPB, int WPE>
+void launch_wave_t(erlbm_lattice *h, const StepArgs &a, int rounds)
+{
+    static const int waves_per_cu = [] {
+        int nblk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, reinterpret_cast<const void *>(&k_step2_wave<COL, PD, WPB, WPE>),
+                                                         64 * WPB, 0) != hipSuccess || nblk < 1)
+            nblk = 1;
+        return nblk * WPB;
+    }();
+    const int nstrip = (h->nx + WAVE_STRIP - 1) / WAVE_STRIP;
+    const int nseg = wave_segments(h, waves_per_cu, rounds);
+    const unsigned nb = (unsigned)(((long long)nstrip * nseg + WPB - 1) / WPB);
+    hipLaunchKernelGGL((k_step2_wave<COL, PD, WPB, WPE>), dim3(nb), dim3(64 * WPB), 0, h->stream, a, nstrip, nseg,
+                       h->tune_fuse_remap);
+}
+
 template <int COL>
 void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
 {
     switch (h->fuse_active) {
+    case 16: launch_wave_t<COL, 1, 4, 1>(h, a, 2); return;
+    case 17: launch_wave_t<COL, 1, 4, 1>(h, a, 1); return;
+    case 18: launch_wave_t<COL, 2, 4, 1>(h, a, 2); return;
     case 11: launch_strip_t<COL, 256, 32>(h, a); return;
     case 12: launch_strip_t<COL, 256, 16>(h, a); return;
     case 13: launch_strip_t<COL, 128, 32>(h, a); return;
@@ -1155,10 +1347,10 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
             {11, {"two steps, 256-wide strips x 32 rows", "two steps, 256-wide strips x 32 rows, XCD runs"}},
             {13, {"two steps, 128-wide strips x 32 rows", "two steps, 128-wide strips x 32 rows, XCD runs"}},
             {1, {"two steps, 64x8 tiles", "two steps, 64x8 tiles, XCD runs"}},
-            {7, {"two steps, 64x4 tiles", "two steps, 64x4 tiles, XCD runs"}}};
+            {7, {"two steps, 64x4 tiles", "two steps, 64x4 tiles, XCD runs"}},
+            {16, {"two steps, wavefront strips", "two steps, wavefront strips, XCD runs"}}};
         for (const Named &nm : fused) {
-            const FuseShape fs = FUSE_SHAPES[nm.f];
-            if (h->nx % fs.tx || h->ny % fs.ty) continue;
+            if (!fuse_geometry_fits(h, FUSE_SHAPES[nm.f])) continue;
             for (int rm = 0; rm < 2; rm++) {
                 LaunchShape ls{2, 256, 1, 0, 0, nm.name[rm], nm.f, rm};
                 c.push_back(ls);

@@ -269,7 +269,7 @@ def test_mrt_is_d2q9_only(gpu):
             lat.set_params(0.6, 0.65, 1.0, O.COL_MRT)
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 11, 12, 13, 14, 15])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17, 18])
 def test_two_step_fused_kernel_bitwise(gpu, orc, monkeypatch, shape):
     """Two steps per launch (LDS tile with a one-node halo): any mix of call
     lengths gives the populations, post-collision monitors and energies of
@@ -300,6 +300,49 @@ def test_two_step_fused_kernel_bitwise(gpu, orc, monkeypatch, shape):
         ref.step(4)
         assert fused.status() == ref.status()
         assert np.array_equal(fused.download(), ref.download())
+
+
+@pytest.mark.parametrize("col", [O.COL_BGK, O.COL_MRT, O.COL_REG, O.COL_EXT_REG])
+@pytest.mark.parametrize("shape,dims", [(16, (300, 77)), (18, (64, 5)), (17, (130, 33)), (18, (125, 64))])
+def test_wavefront_strips_ragged_bitwise(gpu, orc, monkeypatch, col, shape, dims):
+    """The wavefront-strip two-step kernel on lattices whose width is no
+    multiple of its 62-wide strips and whose height is no multiple of its
+    segments (down to the 64-column minimum): bitwise the oracle for odd and
+    even call lengths, for every collision model."""
+    nx, ny = dims
+    rng = np.random.default_rng(nx * ny + col)
+    ini, st, nf = orc.stream_map(9, nx, ny, 1)
+    feq = orc.dist_eq(9, 0.03, -0.02, 0.0, 1.0)
+    f0 = feq[None] * (1 + 1e-2 * rng.uniform(-1, 1, (nf, 9)))
+    tau, tnh = 0.52, {O.COL_BGK: 0.52, O.COL_MRT: 0.65, O.COL_REG: 1.0, O.COL_EXT_REG: 0.65}[col]
+    monkeypatch.setenv("ERLBM_FUSE", str(shape))
+    print(f"wave strips shape={shape} dims={dims} col={col}", flush=True)
+    with gpu.Lattice(9, nx, ny, 1) as lat:
+        lat.set_params(tau, tnh, 1.0, col)
+        lat.upload(f0)
+        done = 0
+        for n in (1, 6, 3, 10):
+            lat.step(n)
+            done += n
+            assert lat.status() == (done, 0)
+            assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, done, col, tau, tnh, nthreads=4)), done
+
+
+def test_default_two_step_on_ragged_width(gpu, orc):
+    """A 1000 x 1000 periodic lattice (width no multiple of 64: neither the
+    strips nor the tiles fit, so the default rule takes the wavefront
+    strips): 1 + 2 x 4 steps bitwise the oracle."""
+    nx = ny = 1000
+    rng = np.random.default_rng(7)
+    ini, st, nf = orc.stream_map(9, nx, ny, 1)
+    feq = orc.dist_eq(9, 0.03, -0.02, 0.0, 1.0)
+    f0 = feq[None] * (1 + 1e-2 * rng.uniform(-1, 1, (nf, 9)))
+    with gpu.Lattice(9, nx, ny, 1) as lat:
+        lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
+        lat.upload(f0)
+        lat.step(9)
+        assert lat.status() == (9, 0)
+        assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, 9, O.COL_EXT_REG, 0.51, 0.65, nthreads=16))
 
 
 @pytest.mark.parametrize("q,dims,seed", [(9, (300, 77, 1), 1), (9, (5, 3, 1), 2), (19, (37, 19, 11), 3),
