@@ -286,6 +286,42 @@ __global__ __launch_bounds__(BLOCK) void k_step2_d2q9(StepArgs a, int remap)
     }
 }
 
+typedef unsigned int buf_u2 __attribute__((ext_vector_type(2)));
+constexpr int BUF_DWORD3 = 0x00020000;   // raw (stride 0) buffer of dwords
+constexpr int BUF_NT = 2;                // nontemporal cache policy
+// A VGPR offset at or above the buffer's size is out of range: the load
+// returns 0 without touching memory, the store is dropped.  Masking lanes
+// and skipped prefetches this way keeps the walk free of branches around
+// memory instructions, so the wait for a row's loads never has to wait for
+// the stores issued after them.
+constexpr unsigned BUF_OOB = 0x80000000u;
+
+// Raw buffer over a population array (byte size below 2^31, checked by the
+// launchers): 8-byte loads and stores at a per-lane VGPR offset plus a
+// wave-uniform SGPR offset; an offset with BUF_OOB or-ed in is out of range.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pop_buffer(const double *p, const Geometry &g)
+{
+    const unsigned nbytes = (unsigned)(8 * at(g, 8, (long long)g.ny * g.nz - 1, g.nx));
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)nbytes, BUF_DWORD3);
+}
+template <int AUX>
+__device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX));
+}
+__device__ __forceinline__ void buf_store_nt(double v, __amdgpu_buffer_rsrc_t r, unsigned voff)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(buf_u2, v), r, voff, 0, BUF_NT);
+}
+// Nine dropped stores: issued after the loads a loop waits for at its top,
+// they make the state entering the loop match the one at its back edge
+// (those loads, then one row's nine stores), so the wait at the top can
+// leave the previous row's stores in flight instead of draining them.
+__device__ __forceinline__ void nine_dropped_stores(__amdgpu_buffer_rsrc_t r)
+{
+    static_for<9>([&](auto I) { buf_store_nt(0.0, r, BUF_OOB | (8u * (int)I)); });
+}
+
 // Two steps per launch, walking a TX-wide x-strip down SEG output rows:
 // only a ring of three rows of step-k results lives in LDS, so the tile can
 // be long in y (halo 2 rows per SEG) and wide in x (halo 2 columns per TX)
@@ -297,13 +333,24 @@ __global__ __launch_bounds__(BLOCK) void k_step2_d2q9(StepArgs a, int remap)
 template <int TX>
 constexpr int strip_threads() { return TX + 64; }
 
+// The LDS ring of the strip walk keeps each population only as long as it
+// is read: output row y pulls the c_y = +1 populations from step-k row y-1,
+// c_y = 0 from row y and c_y = -1 from row y+1, and row y+1 is written
+// before output row y reads, so the c_y = +1, 0, -1 populations need 3, 2
+// and 1 row slots: 18 rows of TX+2 doubles in all (37 KB at TX = 256)
+// instead of 27.
+__host__ __device__ constexpr int ring_depth(int i) { return VSet<9>::cy[i] > 0 ? 3 : (VSet<9>::cy[i] == 0 ? 2 : 1); }
+__host__ __device__ constexpr int ring_base(int i) { return i == 0 ? 0 : ring_base(i - 1) + ring_depth(i - 1); }
+constexpr int RING_ROWS = ring_base(8) + ring_depth(8);
+static_assert(RING_ROWS == 18, "D2Q9 ring rows");
+
 template <int COL, int TX, int SEG, int PD>
 __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a, int remap)
 {
     static_assert(PD >= 1 && SEG % PD == 0, "rows in flight must divide the walk");
     using V = VSet<9>;
     constexpr int EX = TX + 2;
-    extern __shared__ double ring[];   // [3][9][EX]
+    extern __shared__ double ring[];   // [RING_ROWS][EX]
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
     if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
@@ -314,57 +361,62 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
     const int nx = a.g.nx, ny = a.g.ny;
     const bool act1 = t < EX;
     const int x1 = act1 ? wrap(x0 - 1 + t, nx) : 0;
-    auto load_row = [&](int y, double (&f)[9]) {   // pulls of step-k row y (logical, may be -1 or ny)
-        if (!act1) return;
+    // Memory instructions take no branches (threads without a position get
+    // out-of-range offsets), so the wait for a row's loads at the top of the
+    // walk never has to drain the stores issued after them.
+    const __amdgpu_buffer_rsrc_t rs = pop_buffer(a.src, a.g), rd = pop_buffer(a.dst, a.g);
+    const unsigned oob1 = act1 ? 0u : BUF_OOB;
+    const unsigned vo[3] = {8u * wrap(x1 + 1, nx) | oob1, 8u * x1 | oob1, 8u * wrap(x1 - 1, nx) | oob1};
+    const unsigned vst = t < TX ? 8u * (x0 + t) : 0u;   // column of the store, or-ed with
+    const unsigned sto = t < TX ? 0u : BUF_OOB;          // the out-of-range bit
+    // pulls of step-k row y (logical, -1 .. ny + PD); skip: no memory access
+    auto load_row = [&](int y, double (&f)[9], bool skip) {
+        const unsigned off = skip ? BUF_OOB : 0u;
         const int yw = wrap(y, ny);
         static_for<9>([&](auto I) {
             constexpr int i = I;
-            const int xs = V::cx[i] == 0 ? x1 : wrap(x1 - V::cx[i], nx);
             const int ys = V::cy[i] == 0 ? yw : wrap(yw - V::cy[i], ny);
-            const double *q = &a.src[at(a.g, i, ys, xs)];
-            if constexpr (V::cx[i] == 0) f[i] = __builtin_nontemporal_load(q);
-            else f[i] = *q;
+            f[i] = buf_load<V::cx[i] == 0 ? BUF_NT : 0>(rs, vo[V::cx[i] + 1] | off, (int)(8 * at(a.g, i, ys, 0)));
         });
     };
-    auto slot = [&](int y) { return (y - ya + 1) % 3; };
+    // ring row of population i of step-k row y
+    auto ring_row = [&](int i, int y) { return ring_base(i) + (y - ya + 1) % ring_depth(i); };
     auto collide_to_ring = [&](int y, double (&f)[9]) {
         if (!act1) return;
         double vx, vy, vz;
         collide_exact<9, COL>(f, a.k, vx, vy, vz);
-        double *r = ring + slot(y) * 9 * EX;
-        static_for<9>([&](auto I) { r[(int)I * EX + t] = f[(int)I]; });
+        static_for<9>([&](auto I) { ring[ring_row((int)I, y) * EX + t] = f[(int)I]; });
     };
     auto output_row = [&](int y) {            // step k+1, row y, from the ring
+        double g[9] = {};
         if (t < TX) {
-            double g[9];
             static_for<9>([&](auto I) {
                 constexpr int i = I;
-                g[i] = ring[slot(y - V::cy[i]) * 9 * EX + i * EX + (t + 1 - V::cx[i])];
+                g[i] = ring[ring_row(i, y - V::cy[i]) * EX + (t + 1 - V::cx[i])];
             });
             double vx, vy, vz;
             collide_exact<9, COL>(g, a.k, vx, vy, vz);
-            static_for<9>([&](auto I) {
-                __builtin_nontemporal_store(g[(int)I], &a.dst[at(a.g, (int)I, y, x0 + t)]);
-            });
         }
+        static_for<9>([&](auto I) { buf_store_nt(g[(int)I], rd, (vst + (unsigned)(8 * at(a.g, (int)I, y, 0))) | sto); });
     };
     // PD rows of step-k loads in flight: register set J holds row y+1 when
     // output row y = ya + r, r = J (mod PD), is next
     double fq[PD][9], fb[9];
-    load_row(ya - 1, fq[0]);
-    load_row(ya, fb);
+    load_row(ya - 1, fq[0], false);
+    load_row(ya, fb, false);
     if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
     collide_to_ring(ya - 1, fq[0]);
     collide_to_ring(ya, fb);
-    static_for<PD>([&](auto J) { load_row(ya + 1 + (int)J, fq[(int)J]); });
-    // Three ring slots: row y+1 overwrites row y-2, which the previous
-    // iteration read, hence the second barrier.  (Four slots and one
-    // barrier per row measured the same, with a third more LDS.)
+    static_for<PD>([&](auto J) { load_row(ya + 1 + (int)J, fq[(int)J], false); });
+    nine_dropped_stores(rd);
+    // Row y+1 overwrites rows that output row y-1 read in the previous
+    // iteration, hence the second barrier.  (Four slots per population and
+    // one barrier per row measured the same, with more LDS.)
     for (int r = 0; r < SEG; r += PD) {
         static_for<PD>([&](auto J) {
             const int y = ya + r + (int)J;
-            collide_to_ring(y + 1, fq[(int)J]);                          // step k, row y+1
-            if (y + 1 + PD <= ya + SEG) load_row(y + 1 + PD, fq[(int)J]); // in flight meanwhile
+            collide_to_ring(y + 1, fq[(int)J]);                                   // step k, row y+1
+            load_row(y + 1 + PD, fq[(int)J], y + 1 + PD > ya + SEG);            // in flight meanwhile
             lds_barrier();
             output_row(y);
             lds_barrier();
@@ -398,16 +450,6 @@ __device__ __forceinline__ double wave_shift(double v)
 // walks, while they are still in L2.
 constexpr int WAVE_STRIP = 62;
 
-typedef unsigned int wave_u2 __attribute__((ext_vector_type(2)));
-constexpr int BUF_DWORD3 = 0x00020000;   // raw (stride 0) buffer of dwords
-constexpr int BUF_NT = 2;                // nontemporal cache policy
-// A VGPR offset at or above the buffer's size is out of range: the load
-// returns 0 without touching memory, the store is dropped.  Masking lanes
-// and skipped prefetches this way keeps the walk free of branches around
-// memory instructions, so the wait for a row's loads never has to wait for
-// the stores issued after them.
-constexpr unsigned BUF_OOB = 0x80000000u;
-
 template <int COL, int PD, int DIR>
 __device__ __forceinline__ void wave_walk(const StepArgs &a, int x0, int ya, int rows)
 {
@@ -420,11 +462,10 @@ __device__ __forceinline__ void wave_walk(const StepArgs &a, int x0, int ya, int
     // Raw buffers over the population arrays: the per-lane column is the
     // VGPR offset, the wave-uniform plane + row the SGPR offset of a load
     // (the launcher checks that every byte offset is below 2^31).
-    const unsigned nbytes = (unsigned)(8 * at(a.g, 8, (long long)ny - 1, nx));
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)a.src, (short)0, (int)nbytes, BUF_DWORD3);
-    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc((void *)a.dst, (short)0, (int)nbytes, BUF_DWORD3);
+    const __amdgpu_buffer_rsrc_t rs = pop_buffer(a.src, a.g), rd = pop_buffer(a.dst, a.g);
     const unsigned vo[3] = {8u * wrap(xk + 1, nx), 8u * xk, 8u * wrap(xk - 1, nx)};   // columns of c_x = -1, 0, +1
-    const unsigned vst = out ? 8u * (x0 + lane - 1) : BUF_OOB;
+    const unsigned vst = out ? 8u * (x0 + lane - 1) : 0u;   // column of the store, or-ed with
+    const unsigned sto = out ? 0u : BUF_OOB;                // the out-of-range bit
     // pulls of step-k row y (logical, -1 - PD .. ny + PD); skip: no memory access
     auto load_row = [&](int y, double (&f)[9], bool skip) {
         const unsigned off = skip ? BUF_OOB : 0u;
@@ -433,8 +474,7 @@ __device__ __forceinline__ void wave_walk(const StepArgs &a, int x0, int ya, int
             constexpr int i = I;
             const int ys = V::cy[i] == 0 ? yw : wrap(yw - V::cy[i], ny);
             const int so = (int)(8 * at(a.g, i, ys, 0));
-            f[i] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
-                                                  rs, vo[V::cx[i] + 1] + off, so, V::cx[i] == 0 ? BUF_NT : 0));
+            f[i] = buf_load<V::cx[i] == 0 ? BUF_NT : 0>(rs, vo[V::cx[i] + 1] | off, so);
         });
     };
     auto collide = [&](double (&f)[9]) {
@@ -452,9 +492,7 @@ __device__ __forceinline__ void wave_walk(const StepArgs &a, int x0, int ya, int
     // nine dropped stores: entering the loop, the prefetched rows are then
     // followed by nine stores as at its back edge, so the wait for them at
     // the top of the loop can leave the previous row's stores outstanding
-    static_for<9>([&](auto I) {
-        __builtin_amdgcn_raw_buffer_store_b64(wave_u2{0u, 0u}, rd, BUF_OOB + 8u * (int)I, 0, BUF_NT);
-    });
+    nine_dropped_stores(rd);
     // output row number k of the walk from register set j
     auto row = [&](int k, auto J) {
         constexpr int j = J;
@@ -473,8 +511,7 @@ __device__ __forceinline__ void wave_walk(const StepArgs &a, int x0, int ya, int
         });
         collide(g);                                                        // step k+1, row y
         static_for<9>([&](auto I) {
-            const unsigned so = (unsigned)(8 * at(a.g, (int)I, y, 0));
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(wave_u2, g[(int)I]), rd, vst + so, 0, BUF_NT);
+            buf_store_nt(g[(int)I], rd, (vst + (unsigned)(8 * at(a.g, (int)I, y, 0))) | sto);
         });
         static_for<9>([&](auto I) {
             hb[(int)I] = h0[(int)I];
@@ -1106,7 +1143,8 @@ constexpr FuseShape FUSE_SHAPES[] = {
     {128, 8, FUSE_TILE},   {256, 4, FUSE_TILE},   {64, 16, FUSE_TILE},   {256, 32, FUSE_STRIP},
     {256, 16, FUSE_STRIP}, {128, 32, FUSE_STRIP}, {256, 64, FUSE_STRIP}, {128, 64, FUSE_STRIP},
     // wavefront strips: ty = rounds of resident wavefronts per launch
-    {WAVE_STRIP, 2, FUSE_WAVE, 1, 4}, {WAVE_STRIP, 1, FUSE_WAVE, 1, 4}, {WAVE_STRIP, 2, FUSE_WAVE, 2, 4}};
+    {WAVE_STRIP, 2, FUSE_WAVE, 1, 4}, {WAVE_STRIP, 1, FUSE_WAVE, 1, 4}, {WAVE_STRIP, 2, FUSE_WAVE, 2, 4},
+    {256, 8, FUSE_STRIP},              {128, 16, FUSE_STRIP},            {128, 8, FUSE_STRIP}};
 constexpr int N_FUSE_SHAPES = sizeof(FUSE_SHAPES) / sizeof(FUSE_SHAPES[0]);
 
 // Segments per strip for the wavefront-strip kernel: `rounds` times as many
@@ -1122,7 +1160,8 @@ int wave_segments(const erlbm_lattice *h, int waves_per_cu, int rounds)
 // wavefront strips take ragged edges but need 64 distinct columns).
 bool fuse_geometry_fits(const erlbm_lattice *h, const FuseShape &fs)
 {
-    if (fs.kind == FUSE_WAVE) return h->nx >= 64 && h->extent * 8 < (1ll << 31);   // 32-bit buffer offsets
+    if (fs.kind == FUSE_WAVE) return h->nx >= 64 && h->extent * 8 < (1ll << 31);   // 31-bit buffer offsets
+    if (fs.kind == FUSE_STRIP && h->elem == 8 && h->extent * 8 >= (1ll << 31)) return false;
     return h->nx % fs.tx == 0 && h->ny % fs.ty == 0;
 }
 
@@ -1181,7 +1220,7 @@ void launch_fused_t(erlbm_lattice *h, const StepArgs &a)
 template <int COL, int TX, int SEG, int PD = 1>
 void launch_strip_t(erlbm_lattice *h, const StepArgs &a)
 {
-    constexpr size_t lds = sizeof(double) * 3 * 9 * (TX + 2);
+    constexpr size_t lds = sizeof(double) * RING_ROWS * (TX + 2);
     if constexpr (lds > 65536)
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step2_strip<COL, TX, SEG, PD>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1219,6 +1258,9 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
     case 13: launch_strip_t<COL, 128, 32>(h, a); return;
     case 14: launch_strip_t<COL, 256, 64>(h, a); return;
     case 15: launch_strip_t<COL, 128, 64>(h, a); return;
+    case 19: launch_strip_t<COL, 256, 8>(h, a); return;
+    case 20: launch_strip_t<COL, 128, 16>(h, a); return;
+    case 21: launch_strip_t<COL, 128, 8>(h, a); return;
     default: break;
     }
     switch (h->fuse_active) {

@@ -269,7 +269,7 @@ def test_mrt_is_d2q9_only(gpu):
             lat.set_params(0.6, 0.65, 1.0, O.COL_MRT)
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17, 18])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21])
 def test_two_step_fused_kernel_bitwise(gpu, orc, monkeypatch, shape):
     """Two steps per launch (LDS tile with a one-node halo): any mix of call
     lengths gives the populations, post-collision monitors and energies of
@@ -326,6 +326,26 @@ def test_wavefront_strips_ragged_bitwise(gpu, orc, monkeypatch, col, shape, dims
             done += n
             assert lat.status() == (done, 0)
             assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, done, col, tau, tnh, nthreads=4)), done
+
+
+@pytest.mark.parametrize("shape,dims", [(13, (384, 96)), (11, (768, 64)), (12, (768, 48)), (1, (192, 40)), (16, (1000, 24)),
+                                        (20, (384, 80)), (19, (768, 40))])
+def test_two_step_kernels_non_power_of_two(gpu, orc, monkeypatch, shape, dims):
+    """The two-step kernels on widths and heights that are no power of two
+    (row and plane offsets then share bits with column offsets): bitwise
+    the oracle."""
+    nx, ny = dims
+    rng = np.random.default_rng(nx + ny)
+    ini, st, nf = orc.stream_map(9, nx, ny, 1)
+    feq = orc.dist_eq(9, 0.03, -0.02, 0.0, 1.0)
+    f0 = feq[None] * (1 + 1e-2 * rng.uniform(-1, 1, (nf, 9)))
+    monkeypatch.setenv("ERLBM_FUSE", str(shape))
+    print(f"two-step shape={shape} dims={dims}", flush=True)
+    with gpu.Lattice(9, nx, ny, 1) as lat:
+        lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
+        lat.upload(f0)
+        lat.step(7)
+        assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, 7, O.COL_EXT_REG, 0.51, 0.65, nthreads=16))
 
 
 def test_default_two_step_on_ragged_width(gpu, orc):
