@@ -92,7 +92,8 @@ def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True):
         launch = lat.autotune(200)[0]
     else:
         launch = ("16-byte vectors, one step per launch (default)" if prec == E.FP16_SHIFTED
-                  else "two steps, 256-wide strips x 32 rows, XCD runs (default)")
+                  else "two steps, 256-wide strips x 32 rows, XCD runs (default)" if prec == E.FP32_SHIFTED
+                  else "three steps, 256-wide strips x 32 rows, XCD runs (default)")
     ms = lat.step_timed(steps)
     steps_done, reason = lat.status()
     ms_per_step = ms / steps
@@ -106,8 +107,9 @@ def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True):
     ok = steps_done == warmup + steps and reason == 0 and finite and mass_err < (1e-9 if prec == E.FP64 else 1e-3)
     mlups = L * L / (ms_per_step * 1e-3) / 1e6
     # a launch reads and writes every population once, whether it advances
-    # one step or two (the two-step kernel keeps the middle step in LDS)
-    steps_per_launch = 2 if "two steps" in launch else 1
+    # one step, two or three (the multi-step kernels keep the middle steps in
+    # LDS)
+    steps_per_launch = 3 if "three steps" in launch else 2 if "two steps" in launch else 1
     ms_per_launch = ms_per_step * steps_per_launch
     gbs = BYTES_PER_NODE[precision] * L * L / (ms_per_launch * 1e-3) / 1e9
     aos_bytes = L * L * 9 * 8

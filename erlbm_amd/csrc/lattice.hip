@@ -424,6 +424,117 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
     }
 }
 
+// Three steps per launch, the strip walk with two LDS rings: the step-k
+// results of positions x0-2 .. x0+TX+1 (ring 1) and the step-(k+1) results
+// of x0-1 .. x0+TX (ring 2), each keeping a population only while it is
+// read (18 rows: 3 / 2 / 1 row slots for the populations whose source row
+// is the oldest / middle / newest of the three a pull reads, which depends
+// on the walking direction).  Step r of the walk collides step k of row r
+// into ring 1 (phase 1), step k+1 of the row before it from ring 1 into
+// ring 2 (phase 2) and step k+2 of the row two before from ring 2 to HBM
+// (phase 3); the loads of the next row are in flight meanwhile.  Two
+// barriers per row: phase 1 of the next row overwrites ring-1 rows that
+// phase 2 read, and phase 2 ring-2 rows that phase 3 of the previous row
+// read, each across a barrier.  Odd segments walk upwards, so that the two
+// halo rows neighbouring segments share are read by both at the start (or
+// both at the end) of their walks, while they are still in L2.
+template <int DIR>
+__host__ __device__ constexpr int ring3_depth(int i)
+{
+    return VSet<9>::cy[i] * DIR > 0 ? 3 : (VSet<9>::cy[i] == 0 ? 2 : 1);
+}
+template <int DIR>
+__host__ __device__ constexpr int ring3_base(int i)
+{
+    return i == 0 ? 0 : ring3_base<DIR>(i - 1) + ring3_depth<DIR>(i - 1);
+}
+static_assert(ring3_base<1>(8) + ring3_depth<1>(8) == RING_ROWS && ring3_base<-1>(8) + ring3_depth<-1>(8) == RING_ROWS,
+              "ring rows");
+
+template <int COL, int TX, int SEG, int DIR>
+__device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int ya, double *ring1, double *ring2)
+{
+    using V = VSet<9>;
+    constexpr int E1 = TX + 4, E2 = TX + 2;
+    const int t = (int)threadIdx.x;
+    const int nx = a.g.nx, ny = a.g.ny;
+    const bool act1 = t < E1, act2 = t < E2, act3 = t < TX;
+    const int x1 = act1 ? wrap(x0 - 2 + t, nx) : 0;
+    const __amdgpu_buffer_rsrc_t rs = pop_buffer(a.src, a.g), rd = pop_buffer(a.dst, a.g);
+    const unsigned oob1 = act1 ? 0u : BUF_OOB;
+    const unsigned vo[3] = {8u * wrap(x1 + 1, nx) | oob1, 8u * x1 | oob1, 8u * wrap(x1 - 1, nx) | oob1};
+    const unsigned vst = act3 ? 8u * (x0 + t) : 0u;
+    auto load_row = [&](int y, double (&f)[9], bool skip) {   // pulls of step-k row y (-3 .. ny+2)
+        const unsigned off = skip ? BUF_OOB : 0u;
+        const int yw = wrap(y, ny);
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            const int ys = V::cy[i] == 0 ? yw : wrap(yw - V::cy[i], ny);
+            f[i] = buf_load<V::cx[i] == 0 ? BUF_NT : 0>(rs, vo[V::cx[i] + 1] | off, (int)(8 * at(a.g, i, ys, 0)));
+        });
+    };
+    auto rrow = [&](int i, int y) { return ring3_base<DIR>(i) + (y - ya + 2) % ring3_depth<DIR>(i); };
+    auto collide = [&](double (&f)[9]) {
+        double vx, vy, vz;
+        collide_exact<9, COL>(f, a.k, vx, vy, vz);
+    };
+    // step-k rows ya-2 .. ya+SEG+1 in walking order
+    const int r0 = DIR > 0 ? ya - 2 : ya + SEG + 1;
+    double fq[9];
+    load_row(r0, fq, false);
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+    nine_dropped_stores(rd);
+    for (int n = 0; n < SEG + 4; n++) {
+        const int r = r0 + DIR * n;
+        double f[9];
+        static_for<9>([&](auto I) { f[(int)I] = fq[(int)I]; });
+        load_row(r + DIR, fq, n + 1 >= SEG + 4);                     // in flight meanwhile
+        if (act1) {                                                   // phase 1: step k, row r
+            collide(f);
+            static_for<9>([&](auto I) { ring1[rrow((int)I, r) * E1 + t] = f[(int)I]; });
+        }
+        lds_barrier();
+        if (act2 && n >= 2) {                                         // phase 2: step k+1, row r-DIR
+            const int y = r - DIR;
+            double g[9];
+            static_for<9>([&](auto I) {
+                constexpr int i = I;
+                g[i] = ring1[rrow(i, y - V::cy[i]) * E1 + (t + 1 - V::cx[i])];
+            });
+            collide(g);
+            static_for<9>([&](auto I) { ring2[rrow((int)I, y) * E2 + t] = g[(int)I]; });
+        }
+        lds_barrier();
+        const bool out = n >= 4;                                      // phase 3: step k+2, row r-2 DIR
+        const int y = out ? r - 2 * DIR : ya;
+        double g[9] = {};
+        if (act3 && out) {
+            static_for<9>([&](auto I) {
+                constexpr int i = I;
+                g[i] = ring2[rrow(i, y - V::cy[i]) * E2 + (t + 1 - V::cx[i])];
+            });
+            collide(g);
+        }
+        const unsigned sto = act3 && out ? 0u : BUF_OOB;
+        static_for<9>([&](auto I) { buf_store_nt(g[(int)I], rd, (vst + (unsigned)(8 * at(a.g, (int)I, y, 0))) | sto); });
+    }
+}
+
+template <int COL, int TX, int SEG>
+__global__ __launch_bounds__(strip_threads<TX>()) void k_step3_strip(StepArgs a, int remap)
+{
+    constexpr int E1 = TX + 4;
+    extern __shared__ double rings[];   // ring 1 [RING_ROWS][TX+4], ring 2 [RING_ROWS][TX+2]
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    const int nstrip = a.g.nx / TX;
+    const int x0 = (int)(b % (unsigned)nstrip) * TX;
+    const int seg = (int)(b / (unsigned)nstrip);
+    if (seg & 1) strip3_walk<COL, TX, SEG, -1>(a, x0, seg * SEG, rings, rings + RING_ROWS * E1);
+    else strip3_walk<COL, TX, SEG, 1>(a, x0, seg * SEG, rings, rings + RING_ROWS * E1);
+}
+
 // Lane l of a wavefront receives the value of lane l-1 (WAVE_SHR1) or l+1
 // (WAVE_SHL1): one DPP move per dword, no LDS.  Lanes with no source keep 0.
 constexpr int DPP_WAVE_SHL1 = 0x130;
@@ -1131,7 +1242,7 @@ extern "C" int erlbm_upload_aos(erlbm_lattice *h, const double *inif)
 namespace {
 
 // Shapes of the two-step kernel (ERLBM_FUSE selects one; 0 = off).
-enum FuseKind { FUSE_TILE, FUSE_STRIP, FUSE_WAVE };
+enum FuseKind { FUSE_TILE, FUSE_STRIP, FUSE_WAVE, FUSE_STRIP3 };
 struct FuseShape {
     int tx, ty;       // tile; (strip) strip width and rows walked; (wave) rows walked
     FuseKind kind;
@@ -1144,8 +1255,17 @@ constexpr FuseShape FUSE_SHAPES[] = {
     {256, 16, FUSE_STRIP}, {128, 32, FUSE_STRIP}, {256, 64, FUSE_STRIP}, {128, 64, FUSE_STRIP},
     // wavefront strips: ty = rounds of resident wavefronts per launch
     {WAVE_STRIP, 2, FUSE_WAVE, 1, 4}, {WAVE_STRIP, 1, FUSE_WAVE, 1, 4}, {WAVE_STRIP, 2, FUSE_WAVE, 2, 4},
-    {256, 8, FUSE_STRIP},              {128, 16, FUSE_STRIP},            {128, 8, FUSE_STRIP}};
+    {256, 8, FUSE_STRIP},              {128, 16, FUSE_STRIP},            {128, 8, FUSE_STRIP},
+    // three steps per launch
+    {256, 32, FUSE_STRIP3},            {256, 16, FUSE_STRIP3},           {128, 32, FUSE_STRIP3},
+    {256, 64, FUSE_STRIP3}};
 constexpr int N_FUSE_SHAPES = sizeof(FUSE_SHAPES) / sizeof(FUSE_SHAPES[0]);
+
+// Steps one launch of the shape advances.
+int fuse_steps(int f)
+{
+    return f > 0 && f < N_FUSE_SHAPES && FUSE_SHAPES[f].kind == FUSE_STRIP3 ? 3 : 2;
+}
 
 // Segments per strip for the wavefront-strip kernel: `rounds` times as many
 // wavefronts as the GPU holds at once, at most one per row.
@@ -1161,7 +1281,7 @@ int wave_segments(const erlbm_lattice *h, int waves_per_cu, int rounds)
 bool fuse_geometry_fits(const erlbm_lattice *h, const FuseShape &fs)
 {
     if (fs.kind == FUSE_WAVE) return h->nx >= 64 && h->extent * 8 < (1ll << 31);   // 31-bit buffer offsets
-    if (fs.kind == FUSE_STRIP && h->elem == 8 && h->extent * 8 >= (1ll << 31)) return false;
+    if ((fs.kind == FUSE_STRIP || fs.kind == FUSE_STRIP3) && h->elem == 8 && h->extent * 8 >= (1ll << 31)) return false;
     return h->nx % fs.tx == 0 && h->ny % fs.ty == 0;
 }
 
@@ -1188,18 +1308,19 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
     return fuse_geometry_fits(h, fs);
 }
 
-// The two-step shape a call of erlbm_step uses (0: one step per launch).
+// The multi-step shape a call of erlbm_step uses (0: one step per launch).
 // Without an explicit ERLBM_FUSE, a shape must give every CU a workgroup
 // (small lattices have too few strips or tiles to fill the chip: there the
-// one-step kernel is faster); the 64x8 tiles are the fallback, and the
-// wavefront strips, which take any width from 64 up, the last resort.
+// one-step kernel is faster).  FP64: the three-step strip walk, else the
+// two-step one, else the 64x8 tiles, else the wavefront strips (any width
+// from 64 up); shifted FP32 storage: the two-step strip walk.
 int fuse_shape(const erlbm_lattice *h, int energy_interval)
 {
     if (!(h->q == 9 && h->nz == 1 && h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0 && h->dens_limit <= 0.0 &&
           energy_interval <= 0 && !h->checked))
         return 0;
     if (h->fuse_requested) return fuse_shape_fits(h, h->tune_fuse) ? h->tune_fuse : 0;
-    for (int f : {h->tune_fuse, 1, 16}) {
+    for (int f : {h->tune_fuse, 11, 1, 16}) {
         if (!fuse_shape_fits(h, f)) continue;
         if (fuse_workgroups(h, FUSE_SHAPES[f]) >= h->n_cu) return f;
     }
@@ -1246,10 +1367,26 @@ void launch_wave_t(erlbm_lattice *h, const StepArgs &a, int rounds)
                        h->tune_fuse_remap);
 }
 
+template <int COL, int TX, int SEG>
+void launch_strip3_t(erlbm_lattice *h, const StepArgs &a)
+{
+    constexpr size_t lds = sizeof(double) * RING_ROWS * ((TX + 4) + (TX + 2));
+    if constexpr (lds > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step3_strip<COL, TX, SEG>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const unsigned nb = (unsigned)((h->nx / TX) * (h->ny / SEG));
+    hipLaunchKernelGGL((k_step3_strip<COL, TX, SEG>), dim3(nb), dim3(strip_threads<TX>()), lds, h->stream, a,
+                       h->tune_fuse_remap);
+}
+
 template <int COL>
 void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
 {
     switch (h->fuse_active) {
+    case 22: launch_strip3_t<COL, 256, 32>(h, a); return;
+    case 23: launch_strip3_t<COL, 256, 16>(h, a); return;
+    case 24: launch_strip3_t<COL, 128, 32>(h, a); return;
+    case 25: launch_strip3_t<COL, 256, 64>(h, a); return;
     case 16: launch_wave_t<COL, 1, 4, 1>(h, a, 2); return;
     case 17: launch_wave_t<COL, 1, 4, 1>(h, a, 1); return;
     case 18: launch_wave_t<COL, 2, 4, 1>(h, a, 2); return;
@@ -1335,6 +1472,7 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
     if (energy_interval > 0 || h->dens_limit > 0.0) h->checked = true;
     h->fuse_active = fuse_shape(h, energy_interval);
     const bool fuse = h->fuse_active > 0;
+    const int spl = fuse_steps(h->fuse_active);   // steps per fused launch
     for (int s = 0; s < n_steps;) {
         const long long k = h->launched;
         const int cur = (h->base + (int)(k & 1)) & 1;
@@ -1346,15 +1484,15 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
         a.src = static_cast<const double *>(h->buf[cur]);
         a.dst = static_cast<double *>(h->buf[cur ^ 1]);
         a.step = (unsigned long long)k;
-        if (fuse && k > 0 && s + 1 < n_steps) {
-            // steps k and k+1 in one launch: h_{k+1} lands in the buffer
-            // a single step would have written h_k to, so the parity of the
-            // buffers flips (base) with the step count advanced by two
+        if (fuse && k > 0 && s + spl <= n_steps) {
+            // steps k .. k+spl-1 in one launch, written to the other buffer:
+            // after two steps that is the buffer a single step would have
+            // written h_k to, so the parity of the buffers flips (base)
             int rc = launch_fused(h, a);
             if (rc) return rc;
-            h->launched += 2;
-            h->base ^= 1;
-            s += 2;
+            h->launched += spl;
+            if (spl == 2) h->base ^= 1;
+            s += spl;
             continue;
         }
         int rc = dispatch_step(h, a, k > 0);
@@ -1390,7 +1528,8 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
             {13, {"two steps, 128-wide strips x 32 rows", "two steps, 128-wide strips x 32 rows, XCD runs"}},
             {1, {"two steps, 64x8 tiles", "two steps, 64x8 tiles, XCD runs"}},
             {7, {"two steps, 64x4 tiles", "two steps, 64x4 tiles, XCD runs"}},
-            {16, {"two steps, wavefront strips", "two steps, wavefront strips, XCD runs"}}};
+            {16, {"two steps, wavefront strips", "two steps, wavefront strips, XCD runs"}},
+            {22, {"three steps, 256-wide strips x 32 rows", "three steps, 256-wide strips x 32 rows, XCD runs"}}};
         for (const Named &nm : fused) {
             if (!fuse_geometry_fits(h, FUSE_SHAPES[nm.f])) continue;
             for (int rm = 0; rm < 2; rm++) {
@@ -1452,7 +1591,7 @@ extern "C" int erlbm_lattice_autotune(erlbm_lattice *h, int n_steps, char *choic
     const std::vector<LaunchShape> shapes = candidate_shapes(h);
     const LaunchShape keep{h->tune_ntl, h->tune_block, h->tune_npt, h->tune_pair, h->tune_vec, "unchanged",
                            h->tune_fuse, h->tune_fuse_remap};
-    const int steps = n_steps + (n_steps & 1);   // whole pairs for the two-step candidates
+    const int steps = (n_steps + 5) / 6 * 6;   // whole launches for the two- and three-step candidates
     // scratch copy of the current state, a private stop word: the handle's
     // run is not touched
     const size_t bytes = (size_t)h->extent * h->elem;
@@ -1482,16 +1621,18 @@ extern "C" int erlbm_lattice_autotune(erlbm_lattice *h, int n_steps, char *choic
         unsigned long long k = 0;
         for (size_t c = 0; c < shapes.size() && rc == ERLBM_OK; c++) {
             apply_shape(h, shapes[c]);
-            const bool two = shapes[c].fuse > 0;
+            const bool multi = shapes[c].fuse > 0;
             h->fuse_active = shapes[c].fuse;
+            const int spl = multi ? fuse_steps(shapes[c].fuse) : 1;
             float ms = 0.f;
-            for (int s = -2; s < steps && rc == ERLBM_OK; s += two ? 2 : 1) {
+            // one untimed launch, then `steps` steps (a multiple of 6)
+            for (int s = -spl; s < steps && rc == ERLBM_OK; s += spl) {
                 if (s == 0) hip(hipEventRecord(e0, h->stream), "hipEventRecord");
                 a.src = static_cast<const double *>(tmp[k & 1]);
                 a.dst = static_cast<double *>(tmp[(k & 1) ^ 1]);
                 a.step = 0;
-                k++;   // a two-step launch also writes the other buffer
-                const int r = two ? launch_fused(h, a) : dispatch_step(h, a, true);
+                k++;   // a multi-step launch also writes the other buffer
+                const int r = multi ? launch_fused(h, a) : dispatch_step(h, a, true);
                 if (r && rc == ERLBM_OK) rc = r;
             }
             if (hip(hipEventRecord(e1, h->stream), "hipEventRecord") &&

@@ -164,15 +164,17 @@ struct erlbm_lattice {
     //   ERLBM_VEC        vectorised 16-byte kernel for periodic lattices (lattice_vec.hip):
     //                    -1 auto, 0 off, 1 edge elements by wavefront shuffle, 2 edge elements loaded
     int tune_npt = 1, tune_ntl = 2, tune_remap = 0, tune_pair = 0, tune_block = 256, tune_vec = -1;
-    //   ERLBM_FUSE       two steps per launch for periodic D2Q9 FP64 runs without stop
-    //                    tests: 0 off, 1..10 tile shape (64x8, 32x16, 32x8, 128x4, 16x16,
-    //                    32x32, 64x4, 128x8, 256x4, 64x16), 11..15 strip walks (256x32,
-    //                    256x16, 128x32, 256x64, 128x64 rows)
-    int tune_fuse = 11;
+    //   ERLBM_FUSE       several steps per launch for periodic D2Q9 runs without stop
+    //                    tests: 0 off; two steps: 1..10 tile shape (64x8, 32x16, 32x8,
+    //                    128x4, 16x16, 32x32, 64x4, 128x8, 256x4, 64x16), 11..15 and
+    //                    19..21 strip walks (256x32, 256x16, 128x32, 256x64, 128x64;
+    //                    256x8, 128x16, 128x8), 16..18 wavefront strips; three steps:
+    //                    22..25 strip walks (256x32, 256x16, 128x32, 256x64)
+    int tune_fuse = 22;
     int tune_fuse_remap = 1;  //   ERLBM_FUSE_REMAP 1: XCD-contiguous runs of tiles (0: dispatch order)
     bool checked = false;     // a stop test has run since the upload (no fused steps then)
     bool fuse_requested = false;   // ERLBM_FUSE set explicitly
-    int fuse_active = 0;      // two-step shape of the current erlbm_step call (0: none)
+    int fuse_active = 0;      // multi-step shape of the current erlbm_step call (0: none)
     int n_cu = 256;           // compute units of the device
 };
 
