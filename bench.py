@@ -290,7 +290,7 @@ def main() -> int:
             extra["maps"]["cpu_baseline"] = cm
         extra["serving"] = serving_section()
 
-    traffic = None
+    traffic = valu_busy = None
     if a.precision == "fp64" and os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:
             measured = json.load(fh)
@@ -298,6 +298,7 @@ def main() -> int:
         for entry in measured.get("kernels", []):
             if entry.get("launch") and entry["launch"] in head["launch"]:
                 traffic = entry["bytes_per_launch"]
+                valu_busy = entry.get("valu_busy")
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
     achieved = head["achieved_GBs"]
     line = {
@@ -324,7 +325,10 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": BYTES_PER_NODE[a.precision] * L * L,
                      "ms_per_launch": head["ms_per_launch"],
                      # what a one-step-per-launch kernel would have to stream for this MLUPS
-                     "one_step_equivalent_GBs": round(head["MLUPS"] * 1e6 * BYTES_PER_NODE[a.precision] / 1e9, 1)},
+                     "one_step_equivalent_GBs": round(head["MLUPS"] * 1e6 * BYTES_PER_NODE[a.precision] / 1e9, 1),
+                     # the multi-step kernels trade HBM bytes for FP64 VALU work: the
+                     # profiled VALU busy fraction of the same kernel (rocprofv3)
+                     "valu_busy_profiled": valu_busy},
         "cpu_baseline": cpu,
         "copy_kernel_GBs": round(copy_gbs, 1),
         "mass_rel_error": head["mass_rel_error"],

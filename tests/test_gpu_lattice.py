@@ -329,7 +329,7 @@ def test_wavefront_strips_ragged_bitwise(gpu, orc, monkeypatch, col, shape, dims
 
 
 @pytest.mark.parametrize("shape,dims", [(13, (384, 96)), (11, (768, 64)), (12, (768, 48)), (1, (192, 40)), (16, (1000, 24)),
-                                        (20, (384, 80)), (19, (768, 40)), (22, (768, 64)), (24, (384, 96))])
+                                        (20, (384, 80)), (19, (768, 40)), (22, (768, 64)), (24, (384, 96)), (25, (1536, 64))])
 def test_two_step_kernels_non_power_of_two(gpu, orc, monkeypatch, shape, dims):
     """The two-step kernels on widths and heights that are no power of two
     (row and plane offsets then share bits with column offsets): bitwise
