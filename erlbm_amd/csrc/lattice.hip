@@ -29,6 +29,7 @@
 #include "collide.hpp"
 #include "common.hpp"
 #include "lattice_impl.hpp"
+#include "walk.hpp"
 
 using namespace erlbm;
 using namespace erlbm::lat;
@@ -286,63 +287,14 @@ __global__ __launch_bounds__(BLOCK) void k_step2_d2q9(StepArgs a, int remap)
     }
 }
 
-typedef unsigned int buf_u2 __attribute__((ext_vector_type(2)));
-constexpr int BUF_DWORD3 = 0x00020000;   // raw (stride 0) buffer of dwords
-constexpr int BUF_NT = 2;                // nontemporal cache policy
-// A VGPR offset at or above the buffer's size is out of range: the load
-// returns 0 without touching memory, the store is dropped.  Masking lanes
-// and skipped prefetches this way keeps the walk free of branches around
-// memory instructions, so the wait for a row's loads never has to wait for
-// the stores issued after them.
-constexpr unsigned BUF_OOB = 0x80000000u;
-
-// Raw buffer over a population array (byte size below 2^31, checked by the
-// launchers): 8-byte loads and stores at a per-lane VGPR offset plus a
-// wave-uniform SGPR offset; an offset with BUF_OOB or-ed in is out of range.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t pop_buffer(const double *p, const Geometry &g)
-{
-    const unsigned nbytes = (unsigned)(8 * at(g, 8, (long long)g.ny * g.nz - 1, g.nx));
-    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)nbytes, BUF_DWORD3);
-}
-template <int AUX>
-__device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff)
-{
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX));
-}
-__device__ __forceinline__ void buf_store_nt(double v, __amdgpu_buffer_rsrc_t r, unsigned voff)
-{
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(buf_u2, v), r, voff, 0, BUF_NT);
-}
-// Nine dropped stores: issued after the loads a loop waits for at its top,
-// they make the state entering the loop match the one at its back edge
-// (those loads, then one row's nine stores), so the wait at the top can
-// leave the previous row's stores in flight instead of draining them.
-__device__ __forceinline__ void nine_dropped_stores(__amdgpu_buffer_rsrc_t r)
-{
-    static_for<9>([&](auto I) { buf_store_nt(0.0, r, BUF_OOB | (8u * (int)I)); });
-}
-
 // Two steps per launch, walking a TX-wide x-strip down SEG output rows:
-// only a ring of three rows of step-k results lives in LDS, so the tile can
-// be long in y (halo 2 rows per SEG) and wide in x (halo 2 columns per TX)
-// without the LDS of a whole 2-D tile.  Thread t computes step-k position
+// only a ring of recent step-k rows (walk.hpp) lives in LDS, so the tile
+// can be long in y (halo 2 rows per SEG) and wide in x (halo 2 columns per
+// TX) without the LDS of a whole 2-D tile.  Thread t computes step-k position
 // x0-1+t (t < TX+2) and step-(k+1) position x0+t (t < TX); the loads of the
 // next PD step-k rows are in flight while the current output row is
 // computed.  (PD = 2 and 4 -- 138 and 202 VGPRs -- measured 14-25% slower
 // than PD = 1 at L = 2048, so PD = 1 is the one instantiated.)
-template <int TX>
-constexpr int strip_threads() { return TX + 64; }
-
-// The LDS ring of the strip walk keeps each population only as long as it
-// is read: output row y pulls the c_y = +1 populations from step-k row y-1,
-// c_y = 0 from row y and c_y = -1 from row y+1, and row y+1 is written
-// before output row y reads, so the c_y = +1, 0, -1 populations need 3, 2
-// and 1 row slots: 18 rows of TX+2 doubles in all (37 KB at TX = 256)
-// instead of 27.
-__host__ __device__ constexpr int ring_depth(int i) { return VSet<9>::cy[i] > 0 ? 3 : (VSet<9>::cy[i] == 0 ? 2 : 1); }
-__host__ __device__ constexpr int ring_base(int i) { return i == 0 ? 0 : ring_base(i - 1) + ring_depth(i - 1); }
-constexpr int RING_ROWS = ring_base(8) + ring_depth(8);
-static_assert(RING_ROWS == 18, "D2Q9 ring rows");
 
 template <int COL, int TX, int SEG, int PD>
 __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a, int remap)
@@ -364,7 +316,7 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
     // Memory instructions take no branches (threads without a position get
     // out-of-range offsets), so the wait for a row's loads at the top of the
     // walk never has to drain the stores issued after them.
-    const __amdgpu_buffer_rsrc_t rs = pop_buffer(a.src, a.g), rd = pop_buffer(a.dst, a.g);
+    const __amdgpu_buffer_rsrc_t rs = pop_buffer<8>(a.src, a.g), rd = pop_buffer<8>(a.dst, a.g);
     const unsigned oob1 = act1 ? 0u : BUF_OOB;
     const unsigned vo[3] = {8u * wrap(x1 + 1, nx) | oob1, 8u * x1 | oob1, 8u * wrap(x1 - 1, nx) | oob1};
     const unsigned vst = t < TX ? 8u * (x0 + t) : 0u;   // column of the store, or-ed with
@@ -380,7 +332,7 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
         });
     };
     // ring row of population i of step-k row y
-    auto ring_row = [&](int i, int y) { return ring_base(i) + (y - ya + 1) % ring_depth(i); };
+    auto ring_row = [&](int i, int y) { return ring_base<1>(i) + (y - ya + 1) % ring_depth<1>(i); };
     auto collide_to_ring = [&](int y, double (&f)[9]) {
         if (!act1) return;
         double vx, vy, vz;
@@ -408,7 +360,7 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
     collide_to_ring(ya - 1, fq[0]);
     collide_to_ring(ya, fb);
     static_for<PD>([&](auto J) { load_row(ya + 1 + (int)J, fq[(int)J], false); });
-    nine_dropped_stores(rd);
+    nine_dropped_stores<double>(rd);
     // Row y+1 overwrites rows that output row y-1 read in the previous
     // iteration, hence the second barrier.  (Four slots per population and
     // one barrier per row measured the same, with more LDS.)
@@ -438,18 +390,6 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
 // read, each across a barrier.  Odd segments walk upwards, so that the two
 // halo rows neighbouring segments share are read by both at the start (or
 // both at the end) of their walks, while they are still in L2.
-template <int DIR>
-__host__ __device__ constexpr int ring3_depth(int i)
-{
-    return VSet<9>::cy[i] * DIR > 0 ? 3 : (VSet<9>::cy[i] == 0 ? 2 : 1);
-}
-template <int DIR>
-__host__ __device__ constexpr int ring3_base(int i)
-{
-    return i == 0 ? 0 : ring3_base<DIR>(i - 1) + ring3_depth<DIR>(i - 1);
-}
-static_assert(ring3_base<1>(8) + ring3_depth<1>(8) == RING_ROWS && ring3_base<-1>(8) + ring3_depth<-1>(8) == RING_ROWS,
-              "ring rows");
 
 template <int COL, int TX, int SEG, int DIR>
 __device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int ya, double *ring1, double *ring2)
@@ -460,7 +400,7 @@ __device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int ya, d
     const int nx = a.g.nx, ny = a.g.ny;
     const bool act1 = t < E1, act2 = t < E2, act3 = t < TX;
     const int x1 = act1 ? wrap(x0 - 2 + t, nx) : 0;
-    const __amdgpu_buffer_rsrc_t rs = pop_buffer(a.src, a.g), rd = pop_buffer(a.dst, a.g);
+    const __amdgpu_buffer_rsrc_t rs = pop_buffer<8>(a.src, a.g), rd = pop_buffer<8>(a.dst, a.g);
     const unsigned oob1 = act1 ? 0u : BUF_OOB;
     const unsigned vo[3] = {8u * wrap(x1 + 1, nx) | oob1, 8u * x1 | oob1, 8u * wrap(x1 - 1, nx) | oob1};
     const unsigned vst = act3 ? 8u * (x0 + t) : 0u;
@@ -473,7 +413,7 @@ __device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int ya, d
             f[i] = buf_load<V::cx[i] == 0 ? BUF_NT : 0>(rs, vo[V::cx[i] + 1] | off, (int)(8 * at(a.g, i, ys, 0)));
         });
     };
-    auto rrow = [&](int i, int y) { return ring3_base<DIR>(i) + (y - ya + 2) % ring3_depth<DIR>(i); };
+    auto rrow = [&](int i, int y) { return ring_base<DIR>(i) + (y - ya + 2) % ring_depth<DIR>(i); };
     auto collide = [&](double (&f)[9]) {
         double vx, vy, vz;
         collide_exact<9, COL>(f, a.k, vx, vy, vz);
@@ -483,7 +423,7 @@ __device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int ya, d
     double fq[9];
     load_row(r0, fq, false);
     if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
-    nine_dropped_stores(rd);
+    nine_dropped_stores<double>(rd);
     for (int n = 0; n < SEG + 4; n++) {
         const int r = r0 + DIR * n;
         double f[9];
@@ -573,7 +513,7 @@ __device__ __forceinline__ void wave_walk(const StepArgs &a, int x0, int ya, int
     // Raw buffers over the population arrays: the per-lane column is the
     // VGPR offset, the wave-uniform plane + row the SGPR offset of a load
     // (the launcher checks that every byte offset is below 2^31).
-    const __amdgpu_buffer_rsrc_t rs = pop_buffer(a.src, a.g), rd = pop_buffer(a.dst, a.g);
+    const __amdgpu_buffer_rsrc_t rs = pop_buffer<8>(a.src, a.g), rd = pop_buffer<8>(a.dst, a.g);
     const unsigned vo[3] = {8u * wrap(xk + 1, nx), 8u * xk, 8u * wrap(xk - 1, nx)};   // columns of c_x = -1, 0, +1
     const unsigned vst = out ? 8u * (x0 + lane - 1) : 0u;   // column of the store, or-ed with
     const unsigned sto = out ? 0u : BUF_OOB;                // the out-of-range bit
@@ -603,7 +543,7 @@ __device__ __forceinline__ void wave_walk(const StepArgs &a, int x0, int ya, int
     // nine dropped stores: entering the loop, the prefetched rows are then
     // followed by nine stores as at its back edge, so the wait for them at
     // the top of the loop can leave the previous row's stores outstanding
-    nine_dropped_stores(rd);
+    nine_dropped_stores<double>(rd);
     // output row number k of the walk from register set j
     auto row = [&](int k, auto J) {
         constexpr int j = J;
@@ -1299,12 +1239,18 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
 {
     if (f <= 0 || f >= N_FUSE_SHAPES) return false;
     const FuseShape fs = FUSE_SHAPES[f];
-    // shifted storage: the strip walks only, FP64 arithmetic only; FP16 by
-    // explicit request (ERLBM_FUSE) only -- its 16-byte vector kernel is
-    // faster (36.8 vs 53.2 us/step at L = 2048: the two collisions per
-    // launch make the walk VALU-bound once the bytes are that few)
-    if (h->elem != 8 && (h->f32_arith || fs.kind != FUSE_STRIP || (f != 11 && f != 13))) return false;
-    if (h->elem == 2 && !h->fuse_requested) return false;
+    // shifted storage: the strip walks only, FP64 arithmetic only (FP32
+    // storage: two or three steps; FP16: two, by explicit request
+    // (ERLBM_FUSE) only -- its 16-byte vector kernel is faster, 36.8 vs 53.2
+    // us/step at L = 2048: the two collisions per launch make the walk
+    // VALU-bound once the bytes are that few)
+    if (h->elem != 8) {
+        const bool two = f == FUSE2_STRIP_256x32 || f == FUSE2_STRIP_128x32;
+        const bool three = f == FUSE3_STRIP_256x32 || f == FUSE3_STRIP_128x32;
+        if (h->f32_arith || !(two || (three && h->elem == 4))) return false;
+        if (h->elem == 2 && !h->fuse_requested) return false;
+        if (h->extent * h->elem >= (1ll << 31)) return false;   // 31-bit buffer offsets
+    }
     return fuse_geometry_fits(h, fs);
 }
 
@@ -1320,7 +1266,11 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
           energy_interval <= 0 && !h->checked))
         return 0;
     if (h->fuse_requested) return fuse_shape_fits(h, h->tune_fuse) ? h->tune_fuse : 0;
-    for (int f : {h->tune_fuse, 11, 1, 16}) {
+    // FP32 storage: the two-step walk unless asked (its three-step walk
+    // measured 52.2 vs 50.3 us/step at L = 2048: with half the bytes the
+    // extra halo collisions cost more than the traffic saved)
+    const int first = h->elem == 8 ? h->tune_fuse : FUSE2_STRIP_256x32;
+    for (int f : {first, FUSE2_STRIP_256x32, 1, 16}) {
         if (!fuse_shape_fits(h, f)) continue;
         if (fuse_workgroups(h, FUSE_SHAPES[f]) >= h->n_cu) return f;
     }
@@ -1550,10 +1500,12 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
         c.push_back({0, 256, 1, 0, 0, "nt-loads=none"});
     } else {
         if (h->q == 9 && !h->f32_arith && h->elem == 4) {
-            static const char *names[2][2] = {{"two steps, 256-wide strips x 32 rows", "two steps, 256-wide strips x 32 rows, XCD runs"},
-                                              {"two steps, 128-wide strips x 32 rows", "two steps, 128-wide strips x 32 rows, XCD runs"}};
-            const int fs[2] = {11, 13};
-            for (int k = 0; k < 2; k++) {
+            static const char *names[4][2] = {{"two steps, 256-wide strips x 32 rows", "two steps, 256-wide strips x 32 rows, XCD runs"},
+                                              {"two steps, 128-wide strips x 32 rows", "two steps, 128-wide strips x 32 rows, XCD runs"},
+                                              {"three steps, 256-wide strips x 32 rows", "three steps, 256-wide strips x 32 rows, XCD runs"},
+                                              {"three steps, 128-wide strips x 32 rows", "three steps, 128-wide strips x 32 rows, XCD runs"}};
+            const int fs[4] = {FUSE2_STRIP_256x32, FUSE2_STRIP_128x32, FUSE3_STRIP_256x32, FUSE3_STRIP_128x32};
+            for (int k = 0; k < 4; k++) {
                 if (h->nx % FUSE_SHAPES[fs[k]].tx || h->ny % FUSE_SHAPES[fs[k]].ty) continue;
                 for (int rm = 0; rm < 2; rm++) c.push_back({h->tune_ntl, h->tune_block, 1, 0, 0, names[k][rm], fs[k], rm});
             }

@@ -119,6 +119,11 @@ __device__ inline void zouhe(double (&f)[Q], double vx, bool west)
 }  // namespace lat
 }  // namespace erlbm
 
+// Multi-step shapes (ERLBM_FUSE values) that the FP32 shifted-storage path
+// also has.
+constexpr int FUSE2_STRIP_256x32 = 11, FUSE2_STRIP_128x32 = 13;
+constexpr int FUSE3_STRIP_256x32 = 22, FUSE3_STRIP_128x32 = 24;
+
 struct erlbm_lattice {
     int q = 0, nx = 0, ny = 0, nz = 0;
     long long nvox = 0, si = 0, sr = 0, extent = 0;

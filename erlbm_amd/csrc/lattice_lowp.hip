@@ -14,6 +14,7 @@
 
 #include "collide_f32.hpp"
 #include "lattice_impl.hpp"
+#include "walk.hpp"
 
 namespace erlbm {
 namespace lat {
@@ -259,11 +260,9 @@ __global__ __launch_bounds__(BLOCK) void k_lp_monitor(const T *__restrict__ src,
 // k_step2_strip (three-row LDS ring, one halo column each side), with the
 // step-k populations kept in the ring in the storage type T, exactly the
 // values a one-step launch would have written to HBM and read back.
-template <int TX>
-constexpr int lp_strip_threads() { return TX + 64; }
 
 template <int COL, typename T, int TX, int SEG>
-__global__ __launch_bounds__(lp_strip_threads<TX>()) void k_lp_step2_strip(LPArgs a, int remap)
+__global__ __launch_bounds__(strip_threads<TX>()) void k_lp_step2_strip(LPArgs a, int remap)
 {
     using V = VSet<9>;
     constexpr int EX = TX + 2;
@@ -326,19 +325,116 @@ __global__ __launch_bounds__(lp_strip_threads<TX>()) void k_lp_step2_strip(LPArg
     }
 }
 
+// Three steps per launch for FP32 shifted storage with FP64 arithmetic:
+// the three-step strip walk of lattice.hip's k_step3_strip (two LDS rings,
+// odd segments walking upwards), with the step-k and step-(k+1)
+// populations kept in the rings in FP32, exactly the values one-step
+// launches would have written to HBM and read back.
+template <int COL, int TX, int SEG, int DIR>
+__device__ __forceinline__ void lp_strip3_walk(const LPArgs &a, int x0, int ya, float *ring1, float *ring2)
+{
+    using V = VSet<9>;
+    constexpr int E1 = TX + 4, E2 = TX + 2;
+    const int t = (int)threadIdx.x;
+    const int nx = a.g.nx, ny = a.g.ny;
+    const bool act1 = t < E1, act2 = t < E2, act3 = t < TX;
+    const int x1 = act1 ? wrap(x0 - 2 + t, nx) : 0;
+    const __amdgpu_buffer_rsrc_t rs = pop_buffer<4>(a.src, a.g), rd = pop_buffer<4>(a.dst, a.g);
+    const unsigned oob1 = act1 ? 0u : BUF_OOB;
+    const unsigned vo[3] = {4u * wrap(x1 + 1, nx) | oob1, 4u * x1 | oob1, 4u * wrap(x1 - 1, nx) | oob1};
+    const unsigned vst = act3 ? 4u * (x0 + t) : 0u;
+    auto load_row = [&](int y, float (&d)[9], bool skip) {   // pulls of step-k row y (-3 .. ny+2)
+        const unsigned off = skip ? BUF_OOB : 0u;
+        const int yw = wrap(y, ny);
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            const int ys = V::cy[i] == 0 ? yw : wrap(yw - V::cy[i], ny);
+            d[i] = buf_load_f32<V::cx[i] == 0 ? BUF_NT : 0>(rs, vo[V::cx[i] + 1] | off, (int)(4 * at(a.g, i, ys, 0)));
+        });
+    };
+    auto rrow = [&](int i, int y) { return ring_base<DIR>(i) + (y - ya + 2) % ring_depth<DIR>(i); };
+    const int r0 = DIR > 0 ? ya - 2 : ya + SEG + 1;
+    float fq[9];
+    load_row(r0, fq, false);
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+    nine_dropped_stores<float>(rd);
+    for (int n = 0; n < SEG + 4; n++) {
+        const int r = r0 + DIR * n;
+        float d[9];
+        static_for<9>([&](auto I) { d[(int)I] = fq[(int)I]; });
+        load_row(r + DIR, fq, n + 1 >= SEG + 4);                     // in flight meanwhile
+        if (act1) {                                                   // step k, row r
+            (void)collide_lp<9, COL, false>(d, a);
+            static_for<9>([&](auto I) { ring1[rrow((int)I, r) * E1 + t] = d[(int)I]; });
+        }
+        lds_barrier();
+        if (act2 && n >= 2) {                                         // step k+1, row r-DIR
+            const int y = r - DIR;
+            float g[9];
+            static_for<9>([&](auto I) {
+                constexpr int i = I;
+                g[i] = ring1[rrow(i, y - V::cy[i]) * E1 + (t + 1 - V::cx[i])];
+            });
+            (void)collide_lp<9, COL, false>(g, a);
+            static_for<9>([&](auto I) { ring2[rrow((int)I, y) * E2 + t] = g[(int)I]; });
+        }
+        lds_barrier();
+        const bool out = n >= 4;                                      // step k+2, row r-2 DIR
+        const int y = out ? r - 2 * DIR : ya;
+        float g[9] = {};
+        if (act3 && out) {
+            static_for<9>([&](auto I) {
+                constexpr int i = I;
+                g[i] = ring2[rrow(i, y - V::cy[i]) * E2 + (t + 1 - V::cx[i])];
+            });
+            (void)collide_lp<9, COL, false>(g, a);
+        }
+        const unsigned sto = act3 && out ? 0u : BUF_OOB;
+        static_for<9>([&](auto I) { buf_store_nt(g[(int)I], rd, (vst + (unsigned)(4 * at(a.g, (int)I, y, 0))) | sto); });
+    }
+}
+
+template <int COL, int TX, int SEG>
+__global__ __launch_bounds__(strip_threads<TX>()) void k_lp_step3_strip(LPArgs a, int remap)
+{
+    constexpr int E1 = TX + 4;
+    extern __shared__ float lp_rings[];   // ring 1 [RING_ROWS][TX+4], ring 2 [RING_ROWS][TX+2]
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    const int nstrip = a.g.nx / TX;
+    const int x0 = (int)(b % (unsigned)nstrip) * TX;
+    const int seg = (int)(b / (unsigned)nstrip);
+    if (seg & 1) lp_strip3_walk<COL, TX, SEG, -1>(a, x0, seg * SEG, lp_rings, lp_rings + RING_ROWS * E1);
+    else lp_strip3_walk<COL, TX, SEG, 1>(a, x0, seg * SEG, lp_rings, lp_rings + RING_ROWS * E1);
+}
+
+template <int COL, int TX, int SEG>
+void launch_lp_strip3(erlbm_lattice *h, const LPArgs &l)
+{
+    constexpr size_t lds = sizeof(float) * RING_ROWS * ((TX + 4) + (TX + 2));
+    const unsigned nb = (unsigned)((h->nx / TX) * (h->ny / SEG));
+    hipLaunchKernelGGL((k_lp_step3_strip<COL, TX, SEG>), dim3(nb), dim3(strip_threads<TX>()), lds, h->stream, l,
+                       h->tune_fuse_remap);
+}
+
 template <int COL, typename T, int TX, int SEG>
 void launch_lp_strip(erlbm_lattice *h, const LPArgs &l)
 {
     constexpr size_t lds = sizeof(T) * 3 * 9 * (TX + 2);
     const unsigned nb = (unsigned)((h->nx / TX) * (h->ny / SEG));
-    hipLaunchKernelGGL((k_lp_step2_strip<COL, T, TX, SEG>), dim3(nb), dim3(lp_strip_threads<TX>()), lds, h->stream,
+    hipLaunchKernelGGL((k_lp_step2_strip<COL, T, TX, SEG>), dim3(nb), dim3(strip_threads<TX>()), lds, h->stream,
                        l, h->tune_fuse_remap);
 }
 
 template <int COL, typename T>
 void launch_lp_step2_ct(erlbm_lattice *h, const LPArgs &l)
 {
-    if (h->fuse_active == 13) launch_lp_strip<COL, T, 128, 32>(h, l);
+    if constexpr (sizeof(T) == 4) {
+        if (h->fuse_active == FUSE3_STRIP_256x32) return launch_lp_strip3<COL, 256, 32>(h, l);
+        if (h->fuse_active == FUSE3_STRIP_128x32) return launch_lp_strip3<COL, 128, 32>(h, l);
+    }
+    if (h->fuse_active == FUSE2_STRIP_128x32) launch_lp_strip<COL, T, 128, 32>(h, l);
     else launch_lp_strip<COL, T, 256, 32>(h, l);
 }
 

@@ -1,0 +1,88 @@
+// walk.hpp -- pieces shared by the multi-step strip walks (lattice.hip for
+// FP64 populations, lattice_lowp.hip for FP32 shifted storage): raw-buffer
+// access to the population arrays and the row layout of the LDS rings.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lattice_impl.hpp"
+#include "vset.hpp"
+
+namespace erlbm {
+namespace lat {
+
+typedef unsigned int buf_u2 __attribute__((ext_vector_type(2)));
+constexpr int BUF_DWORD3 = 0x00020000;   // raw (stride 0) buffer of dwords
+constexpr int BUF_NT = 2;                // nontemporal cache policy
+// A VGPR offset at or above the buffer's size is out of range: the load
+// returns 0 without touching memory, the store is dropped.  Masking lanes
+// and skipped prefetches this way keeps the walks free of branches around
+// memory instructions, so the wait for a row's loads never has to wait for
+// the stores issued after them.
+constexpr unsigned BUF_OOB = 0x80000000u;
+
+// Raw buffer over a population array of E-byte elements (byte size below
+// 2^31, checked by the launchers): loads and stores at a per-lane VGPR
+// offset plus a wave-uniform SGPR offset; an offset with BUF_OOB or-ed in
+// is out of range.
+template <int E>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pop_buffer(const void *p, const Geometry &g)
+{
+    const unsigned nbytes = (unsigned)(E * at(g, 8, (long long)g.ny * g.nz - 1, g.nx));
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)nbytes, BUF_DWORD3);
+}
+template <int AUX>
+__device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX));
+}
+template <int AUX>
+__device__ __forceinline__ float buf_load_f32(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
+}
+__device__ __forceinline__ void buf_store_nt(double v, __amdgpu_buffer_rsrc_t r, unsigned voff)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(buf_u2, v), r, voff, 0, BUF_NT);
+}
+__device__ __forceinline__ void buf_store_nt(float v, __amdgpu_buffer_rsrc_t r, unsigned voff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, 0, BUF_NT);
+}
+// Nine dropped stores: issued after the loads a loop waits for at its top,
+// they make the state entering the loop match the one at its back edge
+// (those loads, then one row's nine stores), so the wait at the top can
+// leave the previous row's stores in flight instead of draining them.
+template <typename T>
+__device__ __forceinline__ void nine_dropped_stores(__amdgpu_buffer_rsrc_t r)
+{
+    static_for<9>([&](auto I) { buf_store_nt(T(0), r, BUF_OOB | ((unsigned)sizeof(T) * (int)I)); });
+}
+
+// The LDS rings of the D2Q9 strip walks keep each population only as long
+// as it is read.  A pull of row y reads the c_y = +1 populations from row
+// y-1, c_y = 0 from row y and c_y = -1 from row y+1, and the newest of the
+// three is written just before the pull, so walking in direction DIR
+// (+1: rows ascending) the populations whose source row is the oldest
+// (c_y = DIR), the middle and the newest need 3, 2 and 1 row slots: 18 rows
+// in all instead of 27.
+template <int DIR>
+__host__ __device__ constexpr int ring_depth(int i)
+{
+    return VSet<9>::cy[i] * DIR > 0 ? 3 : (VSet<9>::cy[i] == 0 ? 2 : 1);
+}
+template <int DIR>
+__host__ __device__ constexpr int ring_base(int i)
+{
+    return i == 0 ? 0 : ring_base<DIR>(i - 1) + ring_depth<DIR>(i - 1);
+}
+constexpr int RING_ROWS = ring_base<1>(8) + ring_depth<1>(8);
+static_assert(RING_ROWS == 18 && ring_base<-1>(8) + ring_depth<-1>(8) == RING_ROWS, "D2Q9 ring rows");
+
+// Threads of a strip-walk workgroup for TX-wide strips (the halo positions
+// of the first steps spill into a fifth wavefront at TX = 256).
+template <int TX>
+constexpr int strip_threads() { return TX + 64; }
+
+}  // namespace lat
+}  // namespace erlbm

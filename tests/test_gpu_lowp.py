@@ -105,12 +105,13 @@ def test_shifted_energy_monitor_fp64(gpu, orc):
     assert e == orc.kinetic_energy(19, back)
 
 
-@pytest.mark.parametrize("prec", ["FP32_SHIFTED", "FP16_SHIFTED"])
-@pytest.mark.parametrize("shape,remap", [(11, 1), (13, 0)])
+@pytest.mark.parametrize("prec,shape,remap", [("FP32_SHIFTED", 11, 1), ("FP32_SHIFTED", 13, 0),
+                                              ("FP16_SHIFTED", 11, 1), ("FP16_SHIFTED", 13, 0),
+                                              ("FP32_SHIFTED", 22, 1), ("FP32_SHIFTED", 24, 0)])
 def test_two_step_strip_equals_one_step(gpu, monkeypatch, prec, shape, remap):
-    """The two-step strip walk keeps the step-k populations in LDS in the
-    storage type, so its results equal the one-step shifted path's bitwise
-    (any call pattern, monitors included)."""
+    """The two- and three-step strip walks keep the intermediate populations
+    in LDS in the storage type, so their results equal the one-step shifted
+    path's bitwise (any call pattern, monitors included)."""
     L = 256
     tau = 0.5 + 3. * (1. / 32.) * L / 80000.
     f0 = gpu.init_double_shear(L)
@@ -132,3 +133,22 @@ def test_two_step_strip_equals_one_step(gpu, monkeypatch, prec, shape, remap):
             assert fused.status() == ref.status() == (done, 0)
             assert np.array_equal(fused.download(), ref.download())
             assert fused.kinetic_energy(gpu.POST_COLLISION) == ref.kinetic_energy(gpu.POST_COLLISION)
+
+
+@pytest.mark.parametrize("shape,dims", [(22, (768, 64)), (24, (384, 96))])
+def test_three_step_fp32_non_power_of_two(gpu, orc, monkeypatch, shape, dims):
+    """The FP32 three-step walk on sizes that are no power of two: bitwise
+    the one-step FP32 path."""
+    nx, ny = dims
+    rng = np.random.default_rng(nx)
+    feq = orc.dist_eq(9, 0.03, -0.02, 0.0, 1.0)
+    f0 = feq[None] * (1 + 1e-2 * rng.uniform(-1, 1, (nx * ny, 9)))
+    out = []
+    for fuse in ("0", str(shape)):
+        monkeypatch.setenv("ERLBM_FUSE", fuse)
+        with gpu.Lattice(9, nx, ny, 1, precision=gpu.FP32_SHIFTED) as lat:
+            lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
+            lat.upload(f0)
+            lat.step(10)
+            out.append(lat.download())
+    assert np.array_equal(out[0], out[1])
