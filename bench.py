@@ -122,6 +122,34 @@ def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True):
                                   "download_GBs": round(aos_bytes / t_down / 1e9, 2)}}
 
 
+def dsl_full_run_accuracy(E):
+    """The double-shear run to t = L/U0 (round(L/U0) + 1 = 65,537 steps at
+    L = 2048, as src/main_double-shear-layer.cpp counts them) in each
+    storage precision: wall time of the whole run, and the L2 velocity error
+    of the FP32 / FP16 runs against the FP64 one (tools/error_calculator's
+    E_vector / E_magnitude)."""
+    from erlbm_amd import fields
+    n = int(round(L / U0)) + 1
+    out, ref = {"steps": n}, None
+    for p, prec in (("fp64", E.FP64), ("fp32", E.FP32_SHIFTED), ("fp16", E.FP16_SHIFTED)):
+        f0 = E.init_double_shear(L)
+        with E.Lattice(9, L, L, 1, precision=prec) as lat:
+            lat.set_params(TAU, TAU_NH, 1.0, E.COL_EXT_REG)
+            lat.upload(f0)
+            del f0
+            ms = lat.step_timed(n)
+            steps, reason = lat.status()
+            vel = fields.velocity_field(lat)
+        r = {"ms": round(ms, 1), "MLUPS": round(L * L * n / (ms * 1e-3) / 1e6, 1), "completed": steps == n and reason == 0}
+        if ref is None:
+            ref = vel
+        else:
+            ev, em = fields.l2_error(vel, ref)
+            r["l2_vs_fp64"] = {"E_vector": ev, "E_magnitude": em}
+        out[p] = r
+    return out
+
+
 def cpu_maps_baseline():
     """The reference's own map-point runner (roda_ponto of both map drivers,
     compiled from its sources into oracle/_ref, OpenMP over points as the
@@ -276,9 +304,10 @@ def main() -> int:
                 continue
             okp, r = dsl_run(E, p, max(200, a.steps // 2), 20, tune=a.tune)
             prec[p] = r if okp else {"error": "run check failed", **r}
+        prec["full_run_to_t_L_over_U0"] = dsl_full_run_accuracy(E)
         prec["accuracy_vs_fp64"] = ("FP32 storage: L2 velocity error 2.1e-7 (DSL L=256, 1000 steps), "
                                     "TG L=64 E_vector within 0.01%; FP16: 1.7e-3 / +401% "
-                                    "(tests/test_gpu_lowp.py)")
+                                    "(tests/test_gpu_lowp.py); L=2048 to t=L/U0: full_run_to_t_L_over_U0")
         extra["precision"] = prec
         extra["d3q19"] = d3q19_section(E)
         extra["maps"] = maps_section(E)
