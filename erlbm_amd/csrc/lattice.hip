@@ -376,7 +376,10 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
     }
 }
 
-// Three steps per launch, the strip walk with two LDS rings: the step-k
+// Three steps per launch, the strip walk with two LDS rings (the 18-row
+// layout, or with LEAN the 12-row one of walk.hpp, which writes the
+// populations a pull reads from the older two rows one barrier late, from
+// registers, so three workgroups share a CU instead of two): the step-k
 // results of positions x0-2 .. x0+TX+1 (ring 1) and the step-(k+1) results
 // of x0-1 .. x0+TX (ring 2), each keeping a population only while it is
 // read (18 rows: 3 / 2 / 1 row slots for the populations whose source row
@@ -391,8 +394,8 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
 // halo rows neighbouring segments share are read by both at the start (or
 // both at the end) of their walks, while they are still in L2.
 
-template <int COL, int TX, int SEG, int DIR>
-__device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int ya, double *ring1, double *ring2)
+template <int COL, int TX, int DIR, bool LEAN>
+__device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int ya, int rows, double *ring1, double *ring2)
 {
     using V = VSet<9>;
     constexpr int E1 = TX + 4, E2 = TX + 2;
@@ -413,27 +416,44 @@ __device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int ya, d
             f[i] = buf_load<V::cx[i] == 0 ? BUF_NT : 0>(rs, vo[V::cx[i] + 1] | off, (int)(8 * at(a.g, i, ys, 0)));
         });
     };
-    auto rrow = [&](int i, int y) { return ring_base<DIR>(i) + (y - ya + 2) % ring_depth<DIR>(i); };
+    auto rrow = [&](int i, int y) {
+        if constexpr (LEAN) return lean_base<DIR>(i) + (y - ya + 2) % lean_depth<DIR>(i);
+        else return ring_base<DIR>(i) + (y - ya + 2) % ring_depth<DIR>(i);
+    };
+    // lean rings: the populations read from the newest of the three rows a
+    // pull reads (c_y = -DIR) are written at once, the others one barrier
+    // later
+    auto now = [](int i) { return !LEAN || V::cy[i] * DIR < 0; };
     auto collide = [&](double (&f)[9]) {
         double vx, vy, vz;
         collide_exact<9, COL>(f, a.k, vx, vy, vz);
     };
-    // step-k rows ya-2 .. ya+SEG+1 in walking order
-    const int r0 = DIR > 0 ? ya - 2 : ya + SEG + 1;
+    // step-k rows ya-2 .. ya+rows+1 in walking order
+    const int r0 = DIR > 0 ? ya - 2 : ya + rows + 1;
     double fq[9];
+    double h2[9];   // lean: the previous walking step's step-(k+1) row, still to be written to ring 2
     load_row(r0, fq, false);
     if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
     nine_dropped_stores<double>(rd);
-    for (int n = 0; n < SEG + 4; n++) {
+    for (int n = 0; n < rows + 4; n++) {
         const int r = r0 + DIR * n;
         double f[9];
         static_for<9>([&](auto I) { f[(int)I] = fq[(int)I]; });
-        load_row(r + DIR, fq, n + 1 >= SEG + 4);                     // in flight meanwhile
+        load_row(r + DIR, fq, n + 1 >= rows + 4);                    // in flight meanwhile
         if (act1) {                                                   // phase 1: step k, row r
             collide(f);
-            static_for<9>([&](auto I) { ring1[rrow((int)I, r) * E1 + t] = f[(int)I]; });
+            static_for<9>([&](auto I) {
+                if (now(I)) ring1[rrow((int)I, r) * E1 + t] = f[(int)I];
+            });
         }
         lds_barrier();
+        if constexpr (LEAN) {
+            if (act2 && n >= 3) {                                     // the rest of row r-2 DIR
+                static_for<9>([&](auto I) {
+                    if (!now(I)) ring2[rrow((int)I, r - 2 * DIR) * E2 + t] = h2[(int)I];
+                });
+            }
+        }
         if (act2 && n >= 2) {                                         // phase 2: step k+1, row r-DIR
             const int y = r - DIR;
             double g[9];
@@ -442,9 +462,19 @@ __device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int ya, d
                 g[i] = ring1[rrow(i, y - V::cy[i]) * E1 + (t + 1 - V::cx[i])];
             });
             collide(g);
-            static_for<9>([&](auto I) { ring2[rrow((int)I, y) * E2 + t] = g[(int)I]; });
+            static_for<9>([&](auto I) {
+                if (now(I)) ring2[rrow((int)I, y) * E2 + t] = g[(int)I];
+                if constexpr (LEAN) h2[(int)I] = g[(int)I];
+            });
         }
         lds_barrier();
+        if constexpr (LEAN) {
+            if (act1) {                                               // the rest of row r
+                static_for<9>([&](auto I) {
+                    if (!now(I)) ring1[rrow((int)I, r) * E1 + t] = f[(int)I];
+                });
+            }
+        }
         const bool out = n >= 4;                                      // phase 3: step k+2, row r-2 DIR
         const int y = out ? r - 2 * DIR : ya;
         double g[9] = {};
@@ -460,19 +490,24 @@ __device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int ya, d
     }
 }
 
-template <int COL, int TX, int SEG>
-__global__ __launch_bounds__(strip_threads<TX>()) void k_step3_strip(StepArgs a, int remap)
+// One launch: nstrip x nseg workgroups, the rows of a strip split evenly
+// among its segments.
+template <int COL, int TX, bool LEAN>
+__global__ __launch_bounds__(strip_threads<TX>()) void k_step3_strip(StepArgs a, int remap, int nseg)
 {
     constexpr int E1 = TX + 4;
-    extern __shared__ double rings[];   // ring 1 [RING_ROWS][TX+4], ring 2 [RING_ROWS][TX+2]
+    constexpr int ROWS = LEAN ? LEAN_ROWS : RING_ROWS;
+    extern __shared__ double rings[];   // ring 1 [ROWS][TX+4], ring 2 [ROWS][TX+2]
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
     if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
     const int nstrip = a.g.nx / TX;
     const int x0 = (int)(b % (unsigned)nstrip) * TX;
     const int seg = (int)(b / (unsigned)nstrip);
-    if (seg & 1) strip3_walk<COL, TX, SEG, -1>(a, x0, seg * SEG, rings, rings + RING_ROWS * E1);
-    else strip3_walk<COL, TX, SEG, 1>(a, x0, seg * SEG, rings, rings + RING_ROWS * E1);
+    const int ya = (int)((long long)seg * a.g.ny / nseg);
+    const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
+    if (seg & 1) strip3_walk<COL, TX, -1, LEAN>(a, x0, ya, rows, rings, rings + ROWS * E1);
+    else strip3_walk<COL, TX, 1, LEAN>(a, x0, ya, rows, rings, rings + ROWS * E1);
 }
 
 // Lane l of a wavefront receives the value of lane l-1 (WAVE_SHR1) or l+1
@@ -1198,7 +1233,12 @@ constexpr FuseShape FUSE_SHAPES[] = {
     {256, 8, FUSE_STRIP},              {128, 16, FUSE_STRIP},            {128, 8, FUSE_STRIP},
     // three steps per launch
     {256, 32, FUSE_STRIP3},            {256, 16, FUSE_STRIP3},           {128, 32, FUSE_STRIP3},
-    {256, 64, FUSE_STRIP3}};
+    {256, 64, FUSE_STRIP3},
+    // three steps per launch: lean rings, segments sized to the resident
+    // workgroups (ty = 0: any height from 16 up) or of 32 rows; 29: 18-row
+    // rings with sized segments
+    {256, 0, FUSE_STRIP3},             {128, 0, FUSE_STRIP3},            {256, 32, FUSE_STRIP3},
+    {256, 0, FUSE_STRIP3}};
 constexpr int N_FUSE_SHAPES = sizeof(FUSE_SHAPES) / sizeof(FUSE_SHAPES[0]);
 
 // Steps one launch of the shape advances.
@@ -1222,6 +1262,7 @@ bool fuse_geometry_fits(const erlbm_lattice *h, const FuseShape &fs)
 {
     if (fs.kind == FUSE_WAVE) return h->nx >= 64 && h->extent * 8 < (1ll << 31);   // 31-bit buffer offsets
     if ((fs.kind == FUSE_STRIP || fs.kind == FUSE_STRIP3) && h->elem == 8 && h->extent * 8 >= (1ll << 31)) return false;
+    if (fs.ty == 0) return h->nx % fs.tx == 0 && h->ny >= 16;
     return h->nx % fs.tx == 0 && h->ny % fs.ty == 0;
 }
 
@@ -1231,6 +1272,10 @@ long long fuse_workgroups(const erlbm_lattice *h, const FuseShape &fs)
     if (fs.kind == FUSE_WAVE) {   // (as if four wavefronts per SIMD fit)
         const long long waves = (long long)((h->nx + WAVE_STRIP - 1) / WAVE_STRIP) * wave_segments(h, 16, fs.ty);
         return (waves + fs.wpb - 1) / fs.wpb;
+    }
+    if (fs.ty == 0) {   // segments sized to the resident workgroups (as if two fit a CU)
+        const long long nstrip = h->nx / fs.tx;
+        return nstrip * std::max(1ll, std::min<long long>(h->ny / 16, 2ll * h->n_cu / std::max(1ll, nstrip)));
     }
     return (long long)(h->nx / fs.tx) * (h->ny / fs.ty);
 }
@@ -1317,16 +1362,26 @@ void launch_wave_t(erlbm_lattice *h, const StepArgs &a, int rounds)
                        h->tune_fuse_remap);
 }
 
-template <int COL, int TX, int SEG>
+// SEG > 0: segments of SEG rows; SEG = 0: as many segments as make one
+// workgroup per resident slot, rows split evenly, at least 16 per segment
+// (a segment recomputes 4 + 2 halo rows of the first two steps).
+template <int COL, int TX, int SEG, bool LEAN = false>
 void launch_strip3_t(erlbm_lattice *h, const StepArgs &a)
 {
-    constexpr size_t lds = sizeof(double) * RING_ROWS * ((TX + 4) + (TX + 2));
-    if constexpr (lds > 65536)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_step3_strip<COL, TX, SEG>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    const unsigned nb = (unsigned)((h->nx / TX) * (h->ny / SEG));
-    hipLaunchKernelGGL((k_step3_strip<COL, TX, SEG>), dim3(nb), dim3(strip_threads<TX>()), lds, h->stream, a,
-                       h->tune_fuse_remap);
+    constexpr size_t lds = sizeof(double) * (LEAN ? LEAN_ROWS : RING_ROWS) * ((TX + 4) + (TX + 2));
+    const void *k = reinterpret_cast<const void *>(&k_step3_strip<COL, TX, LEAN>);
+    static const int per_cu = [&] {
+        if constexpr (lds > 65536) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int nblk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, k, strip_threads<TX>(), lds) != hipSuccess || nblk < 1)
+            nblk = 1;
+        return nblk;
+    }();
+    const int nstrip = h->nx / TX;
+    const int nseg = SEG > 0 ? h->ny / SEG
+                             : (int)std::max(1ll, std::min<long long>(h->ny / 16, (long long)h->n_cu * per_cu / nstrip));
+    hipLaunchKernelGGL((k_step3_strip<COL, TX, LEAN>), dim3((unsigned)(nstrip * nseg)), dim3(strip_threads<TX>()), lds,
+                       h->stream, a, h->tune_fuse_remap, nseg);
 }
 
 template <int COL>
@@ -1337,6 +1392,10 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
     case 23: launch_strip3_t<COL, 256, 16>(h, a); return;
     case 24: launch_strip3_t<COL, 128, 32>(h, a); return;
     case 25: launch_strip3_t<COL, 256, 64>(h, a); return;
+    case 26: launch_strip3_t<COL, 256, 0, true>(h, a); return;
+    case 27: launch_strip3_t<COL, 128, 0, true>(h, a); return;
+    case 28: launch_strip3_t<COL, 256, 32, true>(h, a); return;
+    case 29: launch_strip3_t<COL, 256, 0, false>(h, a); return;
     case 16: launch_wave_t<COL, 1, 4, 1>(h, a, 2); return;
     case 17: launch_wave_t<COL, 1, 4, 1>(h, a, 1); return;
     case 18: launch_wave_t<COL, 2, 4, 1>(h, a, 2); return;
@@ -1479,7 +1538,8 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
             {1, {"two steps, 64x8 tiles", "two steps, 64x8 tiles, XCD runs"}},
             {7, {"two steps, 64x4 tiles", "two steps, 64x4 tiles, XCD runs"}},
             {16, {"two steps, wavefront strips", "two steps, wavefront strips, XCD runs"}},
-            {22, {"three steps, 256-wide strips x 32 rows", "three steps, 256-wide strips x 32 rows, XCD runs"}}};
+            {22, {"three steps, 256-wide strips x 32 rows", "three steps, 256-wide strips x 32 rows, XCD runs"}},
+            {26, {"three steps, 256-wide strips, sized segments", "three steps, 256-wide strips, sized segments, XCD runs"}}};
         for (const Named &nm : fused) {
             if (!fuse_geometry_fits(h, FUSE_SHAPES[nm.f])) continue;
             for (int rm = 0; rm < 2; rm++) {

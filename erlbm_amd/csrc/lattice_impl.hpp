@@ -174,8 +174,10 @@ struct erlbm_lattice {
     //                    128x4, 16x16, 32x32, 64x4, 128x8, 256x4, 64x16), 11..15 and
     //                    19..21 strip walks (256x32, 256x16, 128x32, 256x64, 128x64;
     //                    256x8, 128x16, 128x8), 16..18 wavefront strips; three steps:
-    //                    22..25 strip walks (256x32, 256x16, 128x32, 256x64)
-    int tune_fuse = 22;
+    //                    22..25 strip walks (256x32, 256x16, 128x32, 256x64), 26..29 with
+    //                    segments sized to the resident workgroups (256- and 128-wide,
+    //                    lean rings; 256x32 lean; 256-wide 18-row rings)
+    int tune_fuse = 26;
     int tune_fuse_remap = 1;  //   ERLBM_FUSE_REMAP 1: XCD-contiguous runs of tiles (0: dispatch order)
     bool checked = false;     // a stop test has run since the upload (no fused steps then)
     bool fuse_requested = false;   // ERLBM_FUSE set explicitly

@@ -79,6 +79,23 @@ __host__ __device__ constexpr int ring_base(int i)
 constexpr int RING_ROWS = ring_base<1>(8) + ring_depth<1>(8);
 static_assert(RING_ROWS == 18 && ring_base<-1>(8) + ring_depth<-1>(8) == RING_ROWS, "D2Q9 ring rows");
 
+// The lean rings of the three-step walk write a population into the ring
+// only when its first reader needs it (the newest-row ones at once, the
+// others one barrier later, kept in registers meanwhile): 2 / 1 / 1 row
+// slots, 12 rows in all.
+template <int DIR>
+__host__ __device__ constexpr int lean_depth(int i)
+{
+    return VSet<9>::cy[i] * DIR > 0 ? 2 : 1;
+}
+template <int DIR>
+__host__ __device__ constexpr int lean_base(int i)
+{
+    return i == 0 ? 0 : lean_base<DIR>(i - 1) + lean_depth<DIR>(i - 1);
+}
+constexpr int LEAN_ROWS = lean_base<1>(8) + lean_depth<1>(8);
+static_assert(LEAN_ROWS == 12 && lean_base<-1>(8) + lean_depth<-1>(8) == LEAN_ROWS, "D2Q9 lean ring rows");
+
 // Threads of a strip-walk workgroup for TX-wide strips (the halo positions
 // of the first steps spill into a fifth wavefront at TX = 256).
 template <int TX>

@@ -269,7 +269,7 @@ def test_mrt_is_d2q9_only(gpu):
             lat.set_params(0.6, 0.65, 1.0, O.COL_MRT)
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29])
 def test_two_step_fused_kernel_bitwise(gpu, orc, monkeypatch, shape):
     """Two or three steps per launch (tiles, strip walks, wavefront strips):
     any mix of call lengths gives the populations, post-collision monitors
@@ -329,7 +329,8 @@ def test_wavefront_strips_ragged_bitwise(gpu, orc, monkeypatch, col, shape, dims
 
 
 @pytest.mark.parametrize("shape,dims", [(13, (384, 96)), (11, (768, 64)), (12, (768, 48)), (1, (192, 40)), (16, (1000, 24)),
-                                        (20, (384, 80)), (19, (768, 40)), (22, (768, 64)), (24, (384, 96)), (25, (1536, 64))])
+                                        (20, (384, 80)), (19, (768, 40)), (22, (768, 64)), (24, (384, 96)), (25, (1536, 64)),
+                                        (26, (768, 83)), (27, (384, 50)), (29, (512, 16))])
 def test_two_step_kernels_non_power_of_two(gpu, orc, monkeypatch, shape, dims):
     """The two-step kernels on widths and heights that are no power of two
     (row and plane offsets then share bits with column offsets): bitwise
