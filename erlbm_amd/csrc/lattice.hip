@@ -510,6 +510,118 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step3_strip(StepArgs a,
     else strip3_walk<COL, TX, 1, LEAN>(a, x0, ya, rows, rings, rings + ROWS * E1);
 }
 
+// D3Q19, two steps per launch: a workgroup owns a TX x TY tile of columns
+// and walks it along z through a segment of planes.  Phase 1 pulls step k
+// of plane r from HBM and collides it on the haloed (TX+2) x (TY+2) tile
+// into an LDS ring of planes (lean layout: the populations a pull reads
+// from the newest of its three planes at once, the others after the
+// barrier that follows the step-(k+1) reads, so a population needs only
+// 2 / 1 / 1 plane slots); phase 2 pulls step k+1 of plane r-DIR from the
+// ring, collides it and stores it.  The loads of plane r+DIR are in flight
+// meanwhile.  Each plane of a pull is addressed through a raw buffer based
+// at that plane (the whole lattice may exceed 32-bit offsets).  Odd
+// segments walk downwards so the halo planes neighbouring segments share
+// are read while in L2.
+template <int TX, int TY>
+constexpr int zwalk_threads() { return ((TX + 2) * (TY + 2) + 63) / 64 * 64; }
+
+template <int COL, int TX, int TY, int DIR>
+__device__ __forceinline__ void zwalk(const StepArgs &a, int x0, int y0, int za, int rows, double *ring)
+{
+    using V = VSet<19>;
+    constexpr int HX = TX + 2, NP = HX * (TY + 2);
+    const int t = (int)threadIdx.x;
+    const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+    const bool act1 = t < NP, act2 = t < TX * TY;
+    const int hx = act1 ? t % HX : 0, hy = act1 ? t / HX : 0;
+    const int x = wrap(x0 - 1 + hx, nx), y = wrap(y0 - 1 + hy, ny);
+    // byte offsets within a plane: column and row of each c_x / c_y
+    const unsigned oob1 = act1 ? 0u : BUF_OOB;
+    const unsigned xo[3] = {8u * wrap(x + 1, nx), 8u * x, 8u * wrap(x - 1, nx)};
+    const unsigned yo[3] = {8u * (unsigned)(wrap(y + 1, ny) * a.g.sr), 8u * (unsigned)(y * a.g.sr),
+                            8u * (unsigned)(wrap(y - 1, ny) * a.g.sr)};
+    const unsigned plane_bytes = (unsigned)(8 * (long long)ny * a.g.sr);
+    auto plane_buf = [&](const double *p, int z) {
+        return __builtin_amdgcn_make_buffer_rsrc((void *)(p + (long long)wrap(z, nz) * ny * a.g.sr), (short)0,
+                                                 (int)plane_bytes, BUF_DWORD3);
+    };
+    auto load_plane = [&](int z, double (&f)[19], bool skip) {   // pulls of step-k plane z (-2 .. nz+1)
+        const unsigned off = skip ? BUF_OOB : 0u;
+        const __amdgpu_buffer_rsrc_t rp[3] = {plane_buf(a.src, z + 1), plane_buf(a.src, z), plane_buf(a.src, z - 1)};
+        static_for<19>([&](auto I) {
+            constexpr int i = I;
+            f[i] = buf_load<V::cx[i] == 0 ? BUF_NT : 0>(rp[V::cz[i] + 1], (yo[V::cy[i] + 1] + xo[V::cx[i] + 1]) | oob1 | off,
+                                                        (int)(8 * i * a.g.si));
+        });
+    };
+    auto slot = [&](int i, int z) { return zring_base<DIR>(i) + (z - za + 2) % zring_depth<DIR>(i); };
+    auto now = [](int i) { return V::cz[i] * DIR < 0; };
+    auto collide = [&](double (&f)[19]) {
+        double vx, vy, vz;
+        collide_exact<19, COL>(f, a.k, vx, vy, vz);
+    };
+    const int ox = act2 ? t % TX : 0, oy = act2 ? t / TX : 0;
+    const unsigned vst = 8u * (unsigned)((y0 + oy) * a.g.sr + x0 + ox);
+    const unsigned sto = act2 ? 0u : BUF_OOB;
+    // step-k planes za-1 .. za+rows in walking order
+    const int r0 = DIR > 0 ? za - 1 : za + rows;
+    double fq[19];
+    load_plane(r0, fq, false);
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+    dropped_stores<double, 19>(plane_buf(a.dst, za));
+    for (int n = 0; n < rows + 2; n++) {
+        const int r = r0 + DIR * n;
+        double f[19];
+        static_for<19>([&](auto I) { f[(int)I] = fq[(int)I]; });
+        load_plane(r + DIR, fq, n + 1 >= rows + 2);                  // in flight meanwhile
+        if (act1) {                                                   // phase 1: step k, plane r
+            collide(f);
+            static_for<19>([&](auto I) {
+                if (now(I)) ring[slot(I, r) * NP + t] = f[(int)I];
+            });
+        }
+        lds_barrier();
+        const bool out = n >= 2;                                      // phase 2: step k+1, plane r-DIR
+        const int zo = out ? r - DIR : za;
+        double g[19] = {};
+        if (act2 && out) {
+            static_for<19>([&](auto I) {
+                constexpr int i = I;
+                g[i] = ring[slot(i, zo - V::cz[i]) * NP + (oy + 1 - V::cy[i]) * HX + (ox + 1 - V::cx[i])];
+            });
+        }
+        lds_barrier();
+        if (act1) {                                                   // the rest of plane r
+            static_for<19>([&](auto I) {
+                if (!now(I)) ring[slot(I, r) * NP + t] = f[(int)I];
+            });
+        }
+        if (act2 && out) collide(g);
+        const __amdgpu_buffer_rsrc_t rd = plane_buf(a.dst, zo);
+        const unsigned so = out ? sto : BUF_OOB;
+        static_for<19>([&](auto I) {
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(buf_u2, g[(int)I]), rd, vst | so,
+                                                  (int)(8 * (int)I * a.g.si), BUF_NT);
+        });
+    }
+}
+
+template <int COL, int TX, int TY>
+__global__ __launch_bounds__((zwalk_threads<TX, TY>())) void k_step2_zwalk(StepArgs a, int remap, int nseg)
+{
+    extern __shared__ double zring[];   // [ZRING_PLANES][(TX+2) (TY+2)]
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    const int tx = a.g.nx / TX, ntile = tx * (a.g.ny / TY);
+    const int tile = (int)(b % (unsigned)ntile), seg = (int)(b / (unsigned)ntile);
+    const int za = (int)((long long)seg * a.g.nz / nseg);
+    const int rows = (int)((long long)(seg + 1) * a.g.nz / nseg) - za;
+    const int x0 = (tile % tx) * TX, y0 = (tile / tx) * TY;
+    if (seg & 1) zwalk<COL, TX, TY, -1>(a, x0, y0, za, rows, zring);
+    else zwalk<COL, TX, TY, 1>(a, x0, y0, za, rows, zring);
+}
+
 // Lane l of a wavefront receives the value of lane l-1 (WAVE_SHR1) or l+1
 // (WAVE_SHL1): one DPP move per dword, no LDS.  Lanes with no source keep 0.
 constexpr int DPP_WAVE_SHL1 = 0x130;
@@ -1217,7 +1329,7 @@ extern "C" int erlbm_upload_aos(erlbm_lattice *h, const double *inif)
 namespace {
 
 // Shapes of the two-step kernel (ERLBM_FUSE selects one; 0 = off).
-enum FuseKind { FUSE_TILE, FUSE_STRIP, FUSE_WAVE, FUSE_STRIP3 };
+enum FuseKind { FUSE_TILE, FUSE_STRIP, FUSE_WAVE, FUSE_STRIP3, FUSE_ZWALK };
 struct FuseShape {
     int tx, ty;       // tile; (strip) strip width and rows walked; (wave) rows walked
     FuseKind kind;
@@ -1238,7 +1350,10 @@ constexpr FuseShape FUSE_SHAPES[] = {
     // workgroups (ty = 0: any height from 16 up) or of 32 rows; 29: 18-row
     // rings with sized segments
     {256, 0, FUSE_STRIP3},             {128, 0, FUSE_STRIP3},            {256, 32, FUSE_STRIP3},
-    {256, 0, FUSE_STRIP3}};
+    {256, 0, FUSE_STRIP3},
+    // D3Q19, two steps per launch: x-y tiles walked along z
+    {32, 8, FUSE_ZWALK},               {64, 4, FUSE_ZWALK},              {32, 4, FUSE_ZWALK},
+    {64, 8, FUSE_ZWALK},               {128, 4, FUSE_ZWALK},             {32, 16, FUSE_ZWALK}};
 constexpr int N_FUSE_SHAPES = sizeof(FUSE_SHAPES) / sizeof(FUSE_SHAPES[0]);
 
 // Steps one launch of the shape advances.
@@ -1260,6 +1375,12 @@ int wave_segments(const erlbm_lattice *h, int waves_per_cu, int rounds)
 // wavefront strips take ragged edges but need 64 distinct columns).
 bool fuse_geometry_fits(const erlbm_lattice *h, const FuseShape &fs)
 {
+    if ((fs.kind == FUSE_ZWALK) != (h->q == 19)) return false;
+    if (fs.kind == FUSE_ZWALK) {   // 31-bit offsets within a plane, segments of at least 16 planes
+        const long long window = 8 * ((long long)(h->q - 1) * h->si + (long long)h->ny * h->sr);
+        return h->elem == 8 && h->nz >= 16 && h->nx % fs.tx == 0 && h->ny % fs.ty == 0 && window < (1ll << 31);
+    }
+    if (h->nz != 1) return false;
     if (fs.kind == FUSE_WAVE) return h->nx >= 64 && h->extent * 8 < (1ll << 31);   // 31-bit buffer offsets
     if ((fs.kind == FUSE_STRIP || fs.kind == FUSE_STRIP3) && h->elem == 8 && h->extent * 8 >= (1ll << 31)) return false;
     if (fs.ty == 0) return h->nx % fs.tx == 0 && h->ny >= 16;
@@ -1272,6 +1393,10 @@ long long fuse_workgroups(const erlbm_lattice *h, const FuseShape &fs)
     if (fs.kind == FUSE_WAVE) {   // (as if four wavefronts per SIMD fit)
         const long long waves = (long long)((h->nx + WAVE_STRIP - 1) / WAVE_STRIP) * wave_segments(h, 16, fs.ty);
         return (waves + fs.wpb - 1) / fs.wpb;
+    }
+    if (fs.kind == FUSE_ZWALK) {   // segments sized to the resident workgroups (as if two fit a CU)
+        const long long ntile = (long long)(h->nx / fs.tx) * (h->ny / fs.ty);
+        return ntile * std::max(1ll, std::min<long long>(h->nz / 16, 2ll * h->n_cu / std::max(1ll, ntile)));
     }
     if (fs.ty == 0) {   // segments sized to the resident workgroups (as if two fit a CU)
         const long long nstrip = h->nx / fs.tx;
@@ -1307,10 +1432,16 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
 // from 64 up); shifted FP32 storage: the two-step strip walk.
 int fuse_shape(const erlbm_lattice *h, int energy_interval)
 {
-    if (!(h->q == 9 && h->nz == 1 && h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0 && h->dens_limit <= 0.0 &&
-          energy_interval <= 0 && !h->checked))
+    if (!(h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0 && h->dens_limit <= 0.0 && energy_interval <= 0 &&
+          !h->checked))
         return 0;
     if (h->fuse_requested) return fuse_shape_fits(h, h->tune_fuse) ? h->tune_fuse : 0;
+    if (h->q == 19) {
+        for (int f : {FUSE2_ZWALK_64x8, FUSE2_ZWALK_64x4, FUSE2_ZWALK_32x8})
+            if (fuse_shape_fits(h, f) && fuse_workgroups(h, FUSE_SHAPES[f]) >= h->n_cu) return f;
+        return 0;
+    }
+    if (!(h->q == 9 && h->nz == 1)) return 0;
     // FP32 storage: the two-step walk unless asked (its three-step walk
     // measured 52.2 vs 50.3 us/step at L = 2048: with half the bytes the
     // extra halo collisions cost more than the traffic saved)
@@ -1384,9 +1515,40 @@ void launch_strip3_t(erlbm_lattice *h, const StepArgs &a)
                        h->stream, a, h->tune_fuse_remap, nseg);
 }
 
+// Segments as many as make one workgroup per resident slot, at least 16
+// planes each.
+template <int COL, int TX, int TY>
+void launch_zwalk_t(erlbm_lattice *h, const StepArgs &a)
+{
+    constexpr size_t lds = sizeof(double) * ZRING_PLANES * (TX + 2) * (TY + 2);
+    const void *k = reinterpret_cast<const void *>(&k_step2_zwalk<COL, TX, TY>);
+    static const int per_cu = [&] {
+        if constexpr (lds > 65536) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int nblk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, k, zwalk_threads<TX, TY>(), lds) != hipSuccess || nblk < 1)
+            nblk = 1;
+        return nblk;
+    }();
+    const int ntile = (h->nx / TX) * (h->ny / TY);
+    const int nseg = (int)std::max(1ll, std::min<long long>(h->nz / 16, (long long)h->n_cu * per_cu / ntile));
+    hipLaunchKernelGGL((k_step2_zwalk<COL, TX, TY>), dim3((unsigned)(ntile * nseg)), dim3(zwalk_threads<TX, TY>()), lds,
+                       h->stream, a, h->tune_fuse_remap, nseg);
+}
+
 template <int COL>
 void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
 {
+    if constexpr (COL != COL_MRT) {   // D3Q19 has no MRT
+        switch (h->fuse_active) {
+        case FUSE2_ZWALK_32x8: launch_zwalk_t<COL, 32, 8>(h, a); return;
+        case FUSE2_ZWALK_64x4: launch_zwalk_t<COL, 64, 4>(h, a); return;
+        case FUSE2_ZWALK_32x4: launch_zwalk_t<COL, 32, 4>(h, a); return;
+        case FUSE2_ZWALK_64x8: launch_zwalk_t<COL, 64, 8>(h, a); return;
+        case FUSE2_ZWALK_128x4: launch_zwalk_t<COL, 128, 4>(h, a); return;
+        case FUSE2_ZWALK_32x16: launch_zwalk_t<COL, 32, 16>(h, a); return;
+        default: break;
+        }
+    }
     switch (h->fuse_active) {
     case 22: launch_strip3_t<COL, 256, 32>(h, a); return;
     case 23: launch_strip3_t<COL, 256, 16>(h, a); return;
@@ -1555,6 +1717,14 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
         c.push_back({1, 256, 2, 0, 0, "nt-loads=all nodes/thread=2"});
         if (h->nx % 2 == 0) c.push_back({1, 256, 1, 1, 0, "nt-loads=all paired 16B"});
     } else if (h->elem == 8) {
+        static const int zs[3] = {FUSE2_ZWALK_64x8, FUSE2_ZWALK_64x4, FUSE2_ZWALK_32x8};
+        static const char *zn[3][2] = {{"two steps, 64x8 tiles walked along z", "two steps, 64x8 tiles walked along z, XCD runs"},
+                                       {"two steps, 64x4 tiles walked along z", "two steps, 64x4 tiles walked along z, XCD runs"},
+                                       {"two steps, 32x8 tiles walked along z", "two steps, 32x8 tiles walked along z, XCD runs"}};
+        for (int k = 0; k < 3; k++) {
+            if (!fuse_geometry_fits(h, FUSE_SHAPES[zs[k]])) continue;
+            for (int rm = 0; rm < 2; rm++) c.push_back({2, 256, 1, 0, 0, zn[k][rm], zs[k], rm});
+        }
         c.push_back({2, 256, 1, 0, 0, "nt-loads=c_x0"});
         c.push_back({1, 256, 1, 0, 0, "nt-loads=all"});
         c.push_back({0, 256, 1, 0, 0, "nt-loads=none"});

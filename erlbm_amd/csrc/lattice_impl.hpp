@@ -123,6 +123,9 @@ __device__ inline void zouhe(double (&f)[Q], double vx, bool west)
 // also has.
 constexpr int FUSE2_STRIP_256x32 = 11, FUSE2_STRIP_128x32 = 13;
 constexpr int FUSE3_STRIP_256x32 = 22, FUSE3_STRIP_128x32 = 24;
+// D3Q19 two-step z-walks (x-y tiles)
+constexpr int FUSE2_ZWALK_32x8 = 30, FUSE2_ZWALK_64x4 = 31, FUSE2_ZWALK_32x4 = 32, FUSE2_ZWALK_64x8 = 33,
+              FUSE2_ZWALK_128x4 = 34, FUSE2_ZWALK_32x16 = 35;
 
 struct erlbm_lattice {
     int q = 0, nx = 0, ny = 0, nz = 0;
@@ -176,7 +179,8 @@ struct erlbm_lattice {
     //                    256x8, 128x16, 128x8), 16..18 wavefront strips; three steps:
     //                    22..25 strip walks (256x32, 256x16, 128x32, 256x64), 26..29 with
     //                    segments sized to the resident workgroups (256- and 128-wide,
-    //                    lean rings; 256x32 lean; 256-wide 18-row rings)
+    //                    lean rings; 256x32 lean; 256-wide 18-row rings); D3Q19 (two steps,
+    //                    x-y tiles walked along z): 30..34 (32x8, 64x4, 32x4, 64x8, 128x4)
     int tune_fuse = 26;
     int tune_fuse_remap = 1;  //   ERLBM_FUSE_REMAP 1: XCD-contiguous runs of tiles (0: dispatch order)
     bool checked = false;     // a stop test has run since the upload (no fused steps then)

@@ -96,6 +96,29 @@ __host__ __device__ constexpr int lean_base(int i)
 constexpr int LEAN_ROWS = lean_base<1>(8) + lean_depth<1>(8);
 static_assert(LEAN_ROWS == 12 && lean_base<-1>(8) + lean_depth<-1>(8) == LEAN_ROWS, "D2Q9 lean ring rows");
 
+// Lean rings of the D3Q19 z-walk, by c_z: 2 / 1 / 1 plane slots for the
+// populations a pull reads from the oldest (c_z = DIR), middle and newest
+// of its three planes; 24 planes in all.
+template <int DIR>
+__host__ __device__ constexpr int zring_depth(int i)
+{
+    return VSet<19>::cz[i] * DIR > 0 ? 2 : 1;
+}
+template <int DIR>
+__host__ __device__ constexpr int zring_base(int i)
+{
+    return i == 0 ? 0 : zring_base<DIR>(i - 1) + zring_depth<DIR>(i - 1);
+}
+constexpr int ZRING_PLANES = zring_base<1>(18) + zring_depth<1>(18);
+static_assert(ZRING_PLANES == 24 && zring_base<-1>(18) + zring_depth<-1>(18) == ZRING_PLANES, "D3Q19 ring planes");
+
+// N dropped stores of T (see nine_dropped_stores).
+template <typename T, int N>
+__device__ __forceinline__ void dropped_stores(__amdgpu_buffer_rsrc_t r)
+{
+    static_for<N>([&](auto I) { buf_store_nt(T(0), r, BUF_OOB | ((unsigned)sizeof(T) * (int)I)); });
+}
+
 // Threads of a strip-walk workgroup for TX-wide strips (the halo positions
 // of the first steps spill into a fifth wavefront at TX = 256).
 template <int TX>

@@ -447,3 +447,45 @@ def test_two_step_kernel_fma_tolerance_1000_steps(gpu, monkeypatch):
     err = _rel(got, want)
     print(f"two-step kernel vs FMA oracle, L={L}, 1000 steps: max rel err {err:.2e}", flush=True)
     assert err <= TOL_FMA
+
+
+@pytest.mark.parametrize("col", [O.COL_BGK, O.COL_REG, O.COL_EXT_REG])
+@pytest.mark.parametrize("shape,dims", [(30, (64, 16, 40)), (31, (64, 8, 37)), (32, (96, 12, 33)), (33, (128, 16, 35)),
+                                        (34, (128, 8, 33)), (35, (64, 32, 34))])
+def test_d3q19_zwalk_bitwise(gpu, orc, monkeypatch, col, shape, dims):
+    """D3Q19 two steps per launch (x-y tiles walked along z, segments of
+    both walking directions, ragged segment lengths): bitwise the oracle
+    for odd and even call lengths."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(nx * ny + nz + col)
+    ini, st, nf = orc.stream_map(19, nx, ny, nz)
+    feq = orc.dist_eq(19, 0.02, -0.01, 0.015, 1.0)
+    f0 = feq[None] * (1 + 1e-2 * rng.uniform(-1, 1, (nf, 19)))
+    tau, tnh = 0.52, {O.COL_BGK: 0.52, O.COL_REG: 1.0, O.COL_EXT_REG: 0.65}[col]
+    monkeypatch.setenv("ERLBM_FUSE", str(shape))
+    print(f"d3q19 z-walk shape={shape} dims={dims} col={col}", flush=True)
+    with gpu.Lattice(19, nx, ny, nz) as lat:
+        lat.set_params(tau, tnh, 1.0, col)
+        lat.upload(f0)
+        done = 0
+        for n in (1, 4, 3):
+            lat.step(n)
+            done += n
+            assert lat.status() == (done, 0)
+            assert np.array_equal(lat.download(), orc.run_steps(19, f0, st, done, col, tau, tnh, nthreads=16)), done
+
+
+def test_d3q19_default_two_step_128(gpu, orc):
+    """128^3 periodic D3Q19 E-RLBM (the default rule takes the z-walk there):
+    1 + 2 x 2 steps bitwise the oracle; mass conserved to rounding."""
+    n = 128
+    rng = np.random.default_rng(5)
+    ini, st, nf = orc.stream_map(19, n, n, n)
+    f0 = orc.dist_eq(19, 0.02, -0.01, 0.015, 1.0)[None] * (1 + 1e-3 * rng.uniform(-1, 1, (nf, 19)))
+    with gpu.Lattice(19, n, n, n) as lat:
+        lat.set_params(0.6, 0.65, 1.0, O.COL_EXT_REG)
+        lat.upload(f0)
+        lat.step(5)
+        got = lat.download()
+    assert np.array_equal(got, orc.run_steps(19, f0, st, 5, O.COL_EXT_REG, 0.6, 0.65, nthreads=16))
+    assert abs(got.sum() - f0.sum()) <= 1e-12 * f0.sum()
