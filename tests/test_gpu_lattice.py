@@ -413,10 +413,10 @@ def test_lattice_errors_leave_the_handle_usable(gpu, orc):
         assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, 3, O.COL_EXT_REG, 0.6, 0.65))
 
 
-def test_headline_two_step_kernel_full_size(gpu, orc):
+def test_headline_default_kernel_full_size(gpu, orc):
     """The headline workload as the bench runs it (D2Q9 double shear,
-    L = 2048, default two-step strip kernel): 25 steps bitwise the strict
-    oracle; total mass conserved to rounding."""
+    L = 2048, the default multi-step kernel: three steps per launch): 25
+    steps bitwise the strict oracle; total mass conserved to rounding."""
     L, n = 2048, 25
     tau = 0.5 + 3. * (1. / 32.) * L / 80000.
     f0 = gpu.init_double_shear(L)
@@ -429,12 +429,13 @@ def test_headline_two_step_kernel_full_size(gpu, orc):
     assert abs(got.sum() - f0.sum()) <= 1e-12 * f0.sum()
 
 
-def test_two_step_kernel_fma_tolerance_1000_steps(gpu, monkeypatch):
-    """The north-star tolerance on the two-step kernel: <= 1e-10 max relative
-    population error against the FMA-contracted oracle after 1000 steps
-    (D2Q9 double shear, L = 512, 256-wide strips)."""
+@pytest.mark.parametrize("shape", [11, 26])
+def test_multi_step_kernel_fma_tolerance_1000_steps(gpu, monkeypatch, shape):
+    """The 1e-10 tolerance on the two- and three-step strip walks: max
+    relative population error against the FMA-contracted oracle after 1000
+    steps (D2Q9 double shear, L = 512, 256-wide strips)."""
     import oracle
-    monkeypatch.setenv("ERLBM_FUSE", "11")
+    monkeypatch.setenv("ERLBM_FUSE", str(shape))
     ofma = oracle.Oracle(fma=True)
     L = 512
     tau = 0.5 + 3. * (1. / 32.) * L / 80000.
@@ -445,7 +446,7 @@ def test_two_step_kernel_fma_tolerance_1000_steps(gpu, monkeypatch):
         lat.step(1000)
         got = lat.download()
     err = _rel(got, want)
-    print(f"two-step kernel vs FMA oracle, L={L}, 1000 steps: max rel err {err:.2e}", flush=True)
+    print(f"shape {shape} vs FMA oracle, L={L}, 1000 steps: max rel err {err:.2e}", flush=True)
     assert err <= TOL_FMA
 
 
