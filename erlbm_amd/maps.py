@@ -132,8 +132,9 @@ def run_obstacle_map(precision=L.FP64, n=400, ma=0.1, batch: Optional[Batch] = N
 
 
 def main(argv=None) -> int:
-    """python -m erlbm_amd.maps [--map tg|tg_reg|obstacle] [--n 400] [--precision fp64|fp32|auto]:
-    one full stability map, timed; prints one JSON line."""
+    """python -m erlbm_amd.maps [--map tg|tg_reg|obstacle] [--n 400] [--precision fp64|fp32|auto]
+    [--out FILE]: one full stability map, timed; prints one JSON line and
+    optionally writes the map as the reference's map drivers do."""
     import argparse
     import json
     import time
@@ -141,6 +142,7 @@ def main(argv=None) -> int:
     ap.add_argument("--map", default="tg", choices=["tg", "tg_reg", "obstacle"])
     ap.add_argument("--n", type=int, default=400)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "auto"])
+    ap.add_argument("--out", default=None, help="write the map file (the reference drivers' VTK format)")
     a = ap.parse_args(argv)
     prec = {"fp64": L.FP64, "fp32": L.FP32_SHIFTED, "auto": L.FP_AUTO}[a.precision]
     if a.map == "obstacle":
@@ -152,6 +154,13 @@ def main(argv=None) -> int:
         t = time.perf_counter()
         res = b.run(jobs)
         dt = time.perf_counter() - t
+    if a.out:
+        from . import fields
+        m = res["unstable"].reshape(a.n, a.n).astype(np.int32)   # (n_j, n_i), i fastest
+        if a.map == "obstacle":
+            fields.write_obstacle_map_vtk(a.out, m, 0.1)
+        else:
+            fields.write_tg_map_vtk(a.out, m, col, tnh)
     print(json.dumps({"map": a.map, "points": int(jobs.size), "seconds": round(dt, 3),
                       "points_per_s": round(jobs.size / dt, 1), "unstable": int(res["unstable"].sum()),
                       "node_steps_per_s": float(res["steps"].astype(float).sum() * nf / dt)}))

@@ -9,6 +9,7 @@ Sources, all of them the reference's own:
     three 400 x 400 stability maps bit-packed).
 
     python tests/golden/make_golden.py [--maps]
+    python tests/golden/make_golden.py --only map_disagreements --gpu-maps DIR
 """
 from __future__ import annotations
 
@@ -269,10 +270,42 @@ def map_subsamples():
     np.savez_compressed(os.path.join(HERE, "ref_map_subsample.npz"), **data)
 
 
+def map_disagreements(gpu_dir):
+    """The points where the GPU's full FP64 maps differ from the published
+    ones, and the reference's roda_ponto classification of each (strict and
+    FMA builds).  gpu_dir holds the three maps as written by
+    `python -m erlbm_amd.maps --map {tg,tg_reg,obstacle} --out <gpu_dir>/<name>.vtk`
+    on the GPU."""
+    d = np.load(os.path.join(HERE, "published_maps.npz"))
+    data = {}
+    r19s, r19f = O.Reference(19), O.Reference(19, fma=True)
+    for name, fn in (("tg_erlbm065", "tg.vtk"), ("tg_reg", "tg_reg.vtk"), ("obstacle", "obstacle.vtk")):
+        gpu = read_vtk_map(os.path.join(gpu_dir, fn)).reshape(-1)
+        pub = np.unpackbits(d[name])[: gpu.size]
+        idx = np.nonzero(gpu != pub)[0].astype(np.int32)
+        jj, ii = (idx // 400).astype(np.int32), (idx % 400).astype(np.int32)
+        data[f"{name}_idx"] = idx
+        data[f"{name}_gpu"] = gpu[idx]
+        for tag, r in (("strict", r19s), ("fma", r19f)):
+            t = time.time()
+            if name == "obstacle":
+                c = r.obstacle_map_points(ii, jj)
+            elif name == "tg_reg":
+                c = r.tg_map_points(O.COL_REG, 1.0, ii, jj)
+            else:
+                c = r.tg_map_points(O.COL_EXT_REG, 0.65, ii, jj)
+            data[f"{name}_{tag}"] = c.astype(np.uint8)
+            print(f"  {name} ({tag}): {idx.size} points, {int((c == gpu[idx]).sum())} as the GPU, "
+                  f"{time.time() - t:.1f}s", flush=True)
+    np.savez_compressed(os.path.join(HERE, "ref_map_disagreements.npz"), **data)
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--maps", action="store_true", help="also rerun the 20x20 map subsamples (minutes)")
-    ap.add_argument("--only", choices=["functions", "outputs", "kappa4"], help="write only this group of fixtures")
+    ap.add_argument("--only", choices=["functions", "outputs", "kappa4", "map_disagreements"],
+                    help="write only this group of fixtures")
+    ap.add_argument("--gpu-maps", default=None, help="directory of the GPU's map files (map_disagreements)")
     a = ap.parse_args()
     if not (O.Reference.available(9) and O.Reference.available(19, fma=True)):
         O.build(ref=True)
@@ -284,6 +317,9 @@ if __name__ == "__main__":
         sys.exit(0)
     if a.only == "kappa4":
         kappa4_strict()
+        sys.exit(0)
+    if a.only == "map_disagreements":
+        map_disagreements(a.gpu_maps)
         sys.exit(0)
     published()
     per_function()

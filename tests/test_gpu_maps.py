@@ -46,6 +46,25 @@ def test_obstacle_map_subsample_fp64_equals_reference(gpu, golden):
     assert agree == jobs.size, f"{agree}/{jobs.size}"
 
 
+@pytest.mark.parametrize("key", ["tg_erlbm065", "tg_reg", "obstacle"])
+def test_points_off_the_published_maps_equal_reference(gpu, golden, key):
+    """The points where the full FP64 maps differ from the published data
+    (185, 190 and 52 of 160,000; tests/golden/ref_map_disagreements.npz):
+    there the GPU classifies every point as the reference's own strict build
+    does (its FMA build splits about evenly: stability-edge points)."""
+    g = golden("ref_map_disagreements.npz")
+    idx = g[f"{key}_idx"]
+    kind = "obstacle" if key == "obstacle" else "tg"
+    col, tnh = (O.COL_REG, 1.0) if key == "tg_reg" else (O.COL_EXT_REG, 0.65)
+    pts = {"i": idx % 400, "j": idx // 400}
+    jobs = _subsample_jobs(gpu, kind, pts, gpu.FP64, col, tnh)
+    print(f"{key}: {jobs.size} points off the published map", flush=True)
+    with (gpu.obstacle_batch() if kind == "obstacle" else gpu.tg_batch()) as b:
+        res = b.run(jobs)
+    assert np.array_equal(res["unstable"].astype(np.uint8), g[f"{key}_strict"])
+    assert np.array_equal(res["unstable"].astype(np.uint8), g[f"{key}_gpu"])
+
+
 def test_stop_steps_and_energies_match_oracle(gpu, orc):
     """Stop step and energies of single points, bitwise, against the oracle's
     restatement of roda_ponto."""
