@@ -9,7 +9,6 @@ from ._lib import (COL_BGK, COL_EXT_REG, COL_MRT, COL_REG, FP16_SHIFTED, FP32_SH
                    POST_COLLISION,
                    STATE, STOP_DENSITY, STOP_ENERGY, ErlbmError, device_count, hbm_copy_bench, lib)
 from ._lib import FP_AUTO, INIT_REST, INIT_TAYLOR_GREEN, JOB_DTYPE, RESULT_DTYPE
-from .maps import Batch, obstacle_batch, obstacle_map_jobs, run_obstacle_map, run_tg_map, tg_batch, tg_map_jobs
 from .lattice import (Lattice, dist_eq, init_double_shear, init_taylor_green, init_tg_map, obstacle_map_point_params,
                       obstacle_mask, taylor_green_error, tg_map_point_params)
 
@@ -20,3 +19,14 @@ __all__ = [
     "obstacle_map_point_params", "FP_AUTO", "INIT_REST", "INIT_TAYLOR_GREEN", "JOB_DTYPE", "RESULT_DTYPE", "Batch", "taylor_green_error",
     "obstacle_batch", "obstacle_map_jobs", "run_obstacle_map", "run_tg_map", "tg_batch", "tg_map_jobs",
 ]
+
+# The batched-map names are imported on first use, so that running the maps
+# module as a program (python -m erlbm_amd.maps) does not import it twice.
+_MAPS = ("Batch", "obstacle_batch", "obstacle_map_jobs", "run_obstacle_map", "run_tg_map", "tg_batch", "tg_map_jobs")
+
+
+def __getattr__(name):
+    if name in _MAPS:
+        from . import maps
+        return getattr(maps, name)
+    raise AttributeError(f"module 'erlbm_amd' has no attribute {name!r}")
