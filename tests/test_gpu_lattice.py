@@ -490,3 +490,28 @@ def test_d3q19_default_two_step_128(gpu, orc):
         got = lat.download()
     assert np.array_equal(got, orc.run_steps(19, f0, st, 5, O.COL_EXT_REG, 0.6, 0.65, nthreads=16))
     assert abs(got.sum() - f0.sum()) <= 1e-12 * f0.sum()
+
+
+def test_d3q19_zwalk_monitors_equal_one_step(gpu, monkeypatch):
+    """After z-walk launches, the post-collision download and the bitwise
+    kinetic-energy monitor equal those of the one-step path."""
+    nx, ny, nz = 128, 16, 34
+    f0 = gpu.dist_eq(19, 0.02, -0.01, 0.015, 1.0)[None] * (
+        1 + 1e-2 * np.random.default_rng(11).uniform(-1, 1, (nx * ny * nz, 19)))
+    lats = []
+    for fuse in ("0", "33"):
+        monkeypatch.setenv("ERLBM_FUSE", fuse)
+        lat = gpu.Lattice(19, nx, ny, nz)
+        lat.set_params(0.55, 0.65, 1.0, O.COL_EXT_REG)
+        lat.upload(f0)
+        lats.append(lat)
+    ref, fused = lats
+    with ref, fused:
+        for n in (1, 6, 5):
+            ref.step(n)
+            fused.step(n)
+            assert fused.status() == ref.status()
+            assert np.array_equal(fused.download(), ref.download())
+            assert np.array_equal(fused.download(gpu.POST_COLLISION), ref.download(gpu.POST_COLLISION))
+            assert fused.kinetic_energy(gpu.POST_COLLISION) == ref.kinetic_energy(gpu.POST_COLLISION)
+            assert fused.kinetic_energy(gpu.STATE) == ref.kinetic_energy(gpu.STATE)
