@@ -349,7 +349,7 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
             double vx, vy, vz;
             collide_exact<9, COL>(g, a.k, vx, vy, vz);
         }
-        static_for<9>([&](auto I) { buf_store_nt(g[(int)I], rd, (vst + (unsigned)(8 * at(a.g, (int)I, y, 0))) | sto); });
+        static_for<9>([&](auto I) { buf_store_nt(g[(int)I], rd, vst | sto, (int)(8 * at(a.g, (int)I, y, 0))); });
     };
     // PD rows of step-k loads in flight: register set J holds row y+1 when
     // output row y = ya + r, r = J (mod PD), is next
@@ -486,7 +486,7 @@ __device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int ya, i
             collide(g);
         }
         const unsigned sto = act3 && out ? 0u : BUF_OOB;
-        static_for<9>([&](auto I) { buf_store_nt(g[(int)I], rd, (vst + (unsigned)(8 * at(a.g, (int)I, y, 0))) | sto); });
+        static_for<9>([&](auto I) { buf_store_nt(g[(int)I], rd, vst | sto, (int)(8 * at(a.g, (int)I, y, 0))); });
     }
 }
 
@@ -709,7 +709,7 @@ __device__ __forceinline__ void wave_walk(const StepArgs &a, int x0, int ya, int
         });
         collide(g);                                                        // step k+1, row y
         static_for<9>([&](auto I) {
-            buf_store_nt(g[(int)I], rd, (vst + (unsigned)(8 * at(a.g, (int)I, y, 0))) | sto);
+            buf_store_nt(g[(int)I], rd, vst | sto, (int)(8 * at(a.g, (int)I, y, 0)));
         });
         static_for<9>([&](auto I) {
             hb[(int)I] = h0[(int)I];

@@ -390,7 +390,7 @@ __device__ __forceinline__ void lp_strip3_walk(const LPArgs &a, int x0, int ya, 
             (void)collide_lp<9, COL, false>(g, a);
         }
         const unsigned sto = act3 && out ? 0u : BUF_OOB;
-        static_for<9>([&](auto I) { buf_store_nt(g[(int)I], rd, (vst + (unsigned)(4 * at(a.g, (int)I, y, 0))) | sto); });
+        static_for<9>([&](auto I) { buf_store_nt(g[(int)I], rd, vst | sto, (int)(4 * at(a.g, (int)I, y, 0))); });
     }
 }
 

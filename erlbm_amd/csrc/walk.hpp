@@ -23,8 +23,9 @@ constexpr unsigned BUF_OOB = 0x80000000u;
 
 // Raw buffer over a population array of E-byte elements (byte size below
 // 2^31, checked by the launchers): loads and stores at a per-lane VGPR
-// offset plus a wave-uniform SGPR offset; an offset with BUF_OOB or-ed in
-// is out of range.
+// offset plus a wave-uniform SGPR offset; a VGPR offset with BUF_OOB or-ed
+// in is out of range whether or not the SGPR offset (below 2^31) is added
+// before the range check.
 template <int E>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t pop_buffer(const void *p, const Geometry &g)
 {
@@ -41,13 +42,13 @@ __device__ __forceinline__ float buf_load_f32(__amdgpu_buffer_rsrc_t r, unsigned
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
 }
-__device__ __forceinline__ void buf_store_nt(double v, __amdgpu_buffer_rsrc_t r, unsigned voff)
+__device__ __forceinline__ void buf_store_nt(double v, __amdgpu_buffer_rsrc_t r, unsigned voff, int soff = 0)
 {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(buf_u2, v), r, voff, 0, BUF_NT);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(buf_u2, v), r, voff, soff, BUF_NT);
 }
-__device__ __forceinline__ void buf_store_nt(float v, __amdgpu_buffer_rsrc_t r, unsigned voff)
+__device__ __forceinline__ void buf_store_nt(float v, __amdgpu_buffer_rsrc_t r, unsigned voff, int soff = 0)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, 0, BUF_NT);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, BUF_NT);
 }
 // Nine dropped stores: issued after the loads a loop waits for at its top,
 // they make the state entering the loop match the one at its back edge
