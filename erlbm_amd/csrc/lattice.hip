@@ -392,16 +392,18 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
 // phase 2 read, and phase 2 ring-2 rows that phase 3 of the previous row
 // read, each across a barrier.  Odd segments walk upwards, so that the two
 // halo rows neighbouring segments share are read by both at the start (or
-// both at the end) of their walks, while they are still in L2.
+// both at the end) of their walks, while they are still in L2.  The strip
+// is w <= TX positions wide (the rings are laid out for TX).
 
 template <int COL, int TX, int DIR, bool LEAN>
-__device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int ya, int rows, double *ring1, double *ring2)
+__device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int w, int ya, int rows, double *ring1,
+                                            double *ring2)
 {
     using V = VSet<9>;
     constexpr int E1 = TX + 4, E2 = TX + 2;
     const int t = (int)threadIdx.x;
     const int nx = a.g.nx, ny = a.g.ny;
-    const bool act1 = t < E1, act2 = t < E2, act3 = t < TX;
+    const bool act1 = t < w + 4, act2 = t < w + 2, act3 = t < w;
     const int x1 = act1 ? wrap(x0 - 2 + t, nx) : 0;
     const __amdgpu_buffer_rsrc_t rs = pop_buffer<8>(a.src, a.g), rd = pop_buffer<8>(a.dst, a.g);
     const unsigned oob1 = act1 ? 0u : BUF_OOB;
@@ -506,8 +508,31 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step3_strip(StepArgs a,
     const int seg = (int)(b / (unsigned)nstrip);
     const int ya = (int)((long long)seg * a.g.ny / nseg);
     const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
-    if (seg & 1) strip3_walk<COL, TX, -1, LEAN>(a, x0, ya, rows, rings, rings + ROWS * E1);
-    else strip3_walk<COL, TX, 1, LEAN>(a, x0, ya, rows, rings, rings + ROWS * E1);
+    if (seg & 1) strip3_walk<COL, TX, -1, LEAN>(a, x0, TX, ya, rows, rings, rings + ROWS * E1);
+    else strip3_walk<COL, TX, 1, LEAN>(a, x0, TX, ya, rows, rings, rings + ROWS * E1);
+}
+
+// The same walk on strips of any width: NW wavefronts per workgroup, nstrip
+// strips of at most 64 NW - 4 positions, widths split evenly (a strip of
+// that width fills every wavefront in all three phases, where a 256-wide
+// strip leaves its fifth wavefront with 4 or 2 active lanes in phases 1
+// and 2), lean rings.
+template <int COL, int NW>
+__global__ __launch_bounds__(64 * NW) void k_step3_gstrip(StepArgs a, int remap, int nstrip, int nseg)
+{
+    constexpr int TX = 64 * NW - 4;
+    extern __shared__ double rings[];   // ring 1 [LEAN_ROWS][TX+4], ring 2 [LEAN_ROWS][TX+2]
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    const int s = (int)(b % (unsigned)nstrip);
+    const int x0 = (int)((long long)s * a.g.nx / nstrip);
+    const int w = (int)((long long)(s + 1) * a.g.nx / nstrip) - x0;
+    const int seg = (int)(b / (unsigned)nstrip);
+    const int ya = (int)((long long)seg * a.g.ny / nseg);
+    const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
+    if (seg & 1) strip3_walk<COL, TX, -1, true>(a, x0, w, ya, rows, rings, rings + LEAN_ROWS * (TX + 4));
+    else strip3_walk<COL, TX, 1, true>(a, x0, w, ya, rows, rings, rings + LEAN_ROWS * (TX + 4));
 }
 
 // D3Q19, two steps per launch: a workgroup owns a TX x TY tile of columns
@@ -1329,7 +1354,7 @@ extern "C" int erlbm_upload_aos(erlbm_lattice *h, const double *inif)
 namespace {
 
 // Shapes of the two-step kernel (ERLBM_FUSE selects one; 0 = off).
-enum FuseKind { FUSE_TILE, FUSE_STRIP, FUSE_WAVE, FUSE_STRIP3, FUSE_ZWALK };
+enum FuseKind { FUSE_TILE, FUSE_STRIP, FUSE_WAVE, FUSE_STRIP3, FUSE_ZWALK, FUSE_GSTRIP3 };
 struct FuseShape {
     int tx, ty;       // tile; (strip) strip width and rows walked; (wave) rows walked
     FuseKind kind;
@@ -1353,13 +1378,19 @@ constexpr FuseShape FUSE_SHAPES[] = {
     {256, 0, FUSE_STRIP3},
     // D3Q19, two steps per launch: x-y tiles walked along z
     {32, 8, FUSE_ZWALK},               {64, 4, FUSE_ZWALK},              {32, 4, FUSE_ZWALK},
-    {64, 8, FUSE_ZWALK},               {128, 4, FUSE_ZWALK},             {32, 16, FUSE_ZWALK}};
+    {64, 8, FUSE_ZWALK},               {128, 4, FUSE_ZWALK},             {32, 16, FUSE_ZWALK},
+    // three steps per launch, lean rings, sized segments: strips of any
+    // width up to tx = 64 NW - 4 for NW = 1 .. 4 wavefronts per workgroup
+    {60, 0, FUSE_GSTRIP3},             {124, 0, FUSE_GSTRIP3},           {188, 0, FUSE_GSTRIP3},
+    {252, 0, FUSE_GSTRIP3}};
 constexpr int N_FUSE_SHAPES = sizeof(FUSE_SHAPES) / sizeof(FUSE_SHAPES[0]);
 
 // Steps one launch of the shape advances.
 int fuse_steps(int f)
 {
-    return f > 0 && f < N_FUSE_SHAPES && FUSE_SHAPES[f].kind == FUSE_STRIP3 ? 3 : 2;
+    return f > 0 && f < N_FUSE_SHAPES && (FUSE_SHAPES[f].kind == FUSE_STRIP3 || FUSE_SHAPES[f].kind == FUSE_GSTRIP3)
+               ? 3
+               : 2;
 }
 
 // Segments per strip for the wavefront-strip kernel: `rounds` times as many
@@ -1382,6 +1413,7 @@ bool fuse_geometry_fits(const erlbm_lattice *h, const FuseShape &fs)
     }
     if (h->nz != 1) return false;
     if (fs.kind == FUSE_WAVE) return h->nx >= 64 && h->extent * 8 < (1ll << 31);   // 31-bit buffer offsets
+    if (fs.kind == FUSE_GSTRIP3) return h->elem == 8 && h->extent * 8 < (1ll << 31) && h->nx >= 16 && h->ny >= 16;
     if ((fs.kind == FUSE_STRIP || fs.kind == FUSE_STRIP3) && h->elem == 8 && h->extent * 8 >= (1ll << 31)) return false;
     if (fs.ty == 0) return h->nx % fs.tx == 0 && h->ny >= 16;
     return h->nx % fs.tx == 0 && h->ny % fs.ty == 0;
@@ -1399,7 +1431,7 @@ long long fuse_workgroups(const erlbm_lattice *h, const FuseShape &fs)
         return ntile * std::max(1ll, std::min<long long>(h->nz / 16, 2ll * h->n_cu / std::max(1ll, ntile)));
     }
     if (fs.ty == 0) {   // segments sized to the resident workgroups (as if two fit a CU)
-        const long long nstrip = h->nx / fs.tx;
+        const long long nstrip = fs.kind == FUSE_GSTRIP3 ? (h->nx + fs.tx - 1) / fs.tx : h->nx / fs.tx;
         return nstrip * std::max(1ll, std::min<long long>(h->ny / 16, 2ll * h->n_cu / std::max(1ll, nstrip)));
     }
     return (long long)(h->nx / fs.tx) * (h->ny / fs.ty);
@@ -1427,9 +1459,10 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
 // The multi-step shape a call of erlbm_step uses (0: one step per launch).
 // Without an explicit ERLBM_FUSE, a shape must give every CU a workgroup
 // (small lattices have too few strips or tiles to fill the chip: there the
-// one-step kernel is faster).  FP64: the three-step strip walk, else the
-// two-step one, else the 64x8 tiles, else the wavefront strips (any width
-// from 64 up); shifted FP32 storage: the two-step strip walk.
+// one-step kernel is faster).  FP64: the three-step strip walk on 256-wide
+// strips, else on strips of any width (four wavefronts), else the two-step
+// walk, else the 64x8 tiles, else the wavefront strips (any width from 64
+// up); shifted FP32 storage: the two-step strip walk.
 int fuse_shape(const erlbm_lattice *h, int energy_interval)
 {
     if (!(h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0 && h->dens_limit <= 0.0 && energy_interval <= 0 &&
@@ -1446,9 +1479,13 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
     // measured 52.2 vs 50.3 us/step at L = 2048: with half the bytes the
     // extra halo collisions cost more than the traffic saved)
     const int first = h->elem == 8 ? h->tune_fuse : FUSE2_STRIP_256x32;
-    for (int f : {first, FUSE2_STRIP_256x32, 1, 16}) {
+    for (int f : {first, FUSE3_GSTRIP_4W, FUSE2_STRIP_256x32, 1, 16}) {
         if (!fuse_shape_fits(h, f)) continue;
-        if (fuse_workgroups(h, FUSE_SHAPES[f]) >= h->n_cu) return f;
+        // strips of any width: from 95% of a workgroup per CU (L = 1000:
+        // 248 workgroups, 61.1k MLUPS against 46.4k for the wavefront
+        // strips; L = 768: 192, 41.8k against 50.9k for the 64x8 tiles)
+        const long long need = f == FUSE3_GSTRIP_4W ? (19ll * h->n_cu + 19) / 20 : h->n_cu;
+        if (fuse_workgroups(h, FUSE_SHAPES[f]) >= need) return f;
     }
     return 0;
 }
@@ -1515,6 +1552,26 @@ void launch_strip3_t(erlbm_lattice *h, const StepArgs &a)
                        h->stream, a, h->tune_fuse_remap, nseg);
 }
 
+// Strips of at most 64 NW - 4 positions, as few as the width allows;
+// segments as for launch_strip3_t with SEG = 0.
+template <int COL, int NW>
+void launch_gstrip3_t(erlbm_lattice *h, const StepArgs &a)
+{
+    constexpr int TX = 64 * NW - 4;
+    constexpr size_t lds = sizeof(double) * LEAN_ROWS * ((TX + 4) + (TX + 2));
+    const void *k = reinterpret_cast<const void *>(&k_step3_gstrip<COL, NW>);
+    static const int per_cu = [&] {
+        if constexpr (lds > 65536) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int nblk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, k, 64 * NW, lds) != hipSuccess || nblk < 1) nblk = 1;
+        return nblk;
+    }();
+    const int nstrip = (h->nx + TX - 1) / TX;
+    const int nseg = (int)std::max(1ll, std::min<long long>(h->ny / 16, (long long)h->n_cu * per_cu / nstrip));
+    hipLaunchKernelGGL((k_step3_gstrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream, a,
+                       h->tune_fuse_remap, nstrip, nseg);
+}
+
 // Segments as many as make one workgroup per resident slot, at least 16
 // planes each.
 template <int COL, int TX, int TY>
@@ -1558,6 +1615,10 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
     case 27: launch_strip3_t<COL, 128, 0, true>(h, a); return;
     case 28: launch_strip3_t<COL, 256, 32, true>(h, a); return;
     case 29: launch_strip3_t<COL, 256, 0, false>(h, a); return;
+    case FUSE3_GSTRIP_1W: launch_gstrip3_t<COL, 1>(h, a); return;
+    case FUSE3_GSTRIP_2W: launch_gstrip3_t<COL, 2>(h, a); return;
+    case FUSE3_GSTRIP_3W: launch_gstrip3_t<COL, 3>(h, a); return;
+    case FUSE3_GSTRIP_4W: launch_gstrip3_t<COL, 4>(h, a); return;
     case 16: launch_wave_t<COL, 1, 4, 1>(h, a, 2); return;
     case 17: launch_wave_t<COL, 1, 4, 1>(h, a, 1); return;
     case 18: launch_wave_t<COL, 2, 4, 1>(h, a, 2); return;
@@ -1701,7 +1762,9 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
             {7, {"two steps, 64x4 tiles", "two steps, 64x4 tiles, XCD runs"}},
             {16, {"two steps, wavefront strips", "two steps, wavefront strips, XCD runs"}},
             {22, {"three steps, 256-wide strips x 32 rows", "three steps, 256-wide strips x 32 rows, XCD runs"}},
-            {26, {"three steps, 256-wide strips, sized segments", "three steps, 256-wide strips, sized segments, XCD runs"}}};
+            {26, {"three steps, 256-wide strips, sized segments", "three steps, 256-wide strips, sized segments, XCD runs"}},
+            {FUSE3_GSTRIP_4W, {"three steps, strips of any width (4 wavefronts)", "three steps, strips of any width (4 wavefronts), XCD runs"}},
+            {FUSE3_GSTRIP_3W, {"three steps, strips of any width (3 wavefronts)", "three steps, strips of any width (3 wavefronts), XCD runs"}}};
         for (const Named &nm : fused) {
             if (!fuse_geometry_fits(h, FUSE_SHAPES[nm.f])) continue;
             for (int rm = 0; rm < 2; rm++) {

@@ -126,6 +126,8 @@ constexpr int FUSE3_STRIP_256x32 = 22, FUSE3_STRIP_128x32 = 24;
 // D3Q19 two-step z-walks (x-y tiles)
 constexpr int FUSE2_ZWALK_32x8 = 30, FUSE2_ZWALK_64x4 = 31, FUSE2_ZWALK_32x4 = 32, FUSE2_ZWALK_64x8 = 33,
               FUSE2_ZWALK_128x4 = 34, FUSE2_ZWALK_32x16 = 35;
+// D2Q9 three-step walks on strips of any width, 1 .. 4 wavefronts per workgroup
+constexpr int FUSE3_GSTRIP_1W = 36, FUSE3_GSTRIP_2W = 37, FUSE3_GSTRIP_3W = 38, FUSE3_GSTRIP_4W = 39;
 
 struct erlbm_lattice {
     int q = 0, nx = 0, ny = 0, nz = 0;

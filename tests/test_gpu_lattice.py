@@ -269,7 +269,8 @@ def test_mrt_is_d2q9_only(gpu):
             lat.set_params(0.6, 0.65, 1.0, O.COL_MRT)
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29,
+                                   36, 37, 38, 39])
 def test_two_step_fused_kernel_bitwise(gpu, orc, monkeypatch, shape):
     """Two or three steps per launch (tiles, strip walks, wavefront strips):
     any mix of call lengths gives the populations, post-collision monitors
@@ -328,6 +329,34 @@ def test_wavefront_strips_ragged_bitwise(gpu, orc, monkeypatch, col, shape, dims
             assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, done, col, tau, tnh, nthreads=4)), done
 
 
+@pytest.mark.parametrize("col", [O.COL_BGK, O.COL_MRT, O.COL_REG, O.COL_EXT_REG])
+@pytest.mark.parametrize("shape,dims", [(36, (300, 77)), (37, (130, 33)), (38, (2048, 40)), (39, (1000, 24)),
+                                        (38, (17, 19)), (36, (16, 16)), (39, (253, 16))])
+def test_any_width_strips_ragged_bitwise(gpu, orc, monkeypatch, col, shape, dims):
+    """The three-step walk on strips of any width (widths split evenly, at
+    most 64 NW - 4 for NW wavefronts) on lattices of ragged widths and
+    heights, down to 16 x 16 (one strip as wide as the lattice): bitwise
+    the oracle for call lengths that are and are not multiples of three,
+    for every collision model."""
+    nx, ny = dims
+    rng = np.random.default_rng(nx * ny + col)
+    ini, st, nf = orc.stream_map(9, nx, ny, 1)
+    feq = orc.dist_eq(9, 0.03, -0.02, 0.0, 1.0)
+    f0 = feq[None] * (1 + 1e-2 * rng.uniform(-1, 1, (nf, 9)))
+    tau, tnh = 0.52, {O.COL_BGK: 0.52, O.COL_MRT: 0.65, O.COL_REG: 1.0, O.COL_EXT_REG: 0.65}[col]
+    monkeypatch.setenv("ERLBM_FUSE", str(shape))
+    print(f"any-width strips shape={shape} dims={dims} col={col}", flush=True)
+    with gpu.Lattice(9, nx, ny, 1) as lat:
+        lat.set_params(tau, tnh, 1.0, col)
+        lat.upload(f0)
+        done = 0
+        for n in (1, 6, 4, 9):
+            lat.step(n)
+            done += n
+            assert lat.status() == (done, 0)
+            assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, done, col, tau, tnh, nthreads=4)), done
+
+
 @pytest.mark.parametrize("shape,dims", [(13, (384, 96)), (11, (768, 64)), (12, (768, 48)), (1, (192, 40)), (16, (1000, 24)),
                                         (20, (384, 80)), (19, (768, 40)), (22, (768, 64)), (24, (384, 96)), (25, (1536, 64)),
                                         (26, (768, 83)), (27, (384, 50)), (29, (512, 16))])
@@ -349,10 +378,11 @@ def test_two_step_kernels_non_power_of_two(gpu, orc, monkeypatch, shape, dims):
         assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, 7, O.COL_EXT_REG, 0.51, 0.65, nthreads=16))
 
 
-def test_default_two_step_on_ragged_width(gpu, orc):
+def test_default_multi_step_on_ragged_width(gpu, orc):
     """A 1000 x 1000 periodic lattice (width no multiple of 64: neither the
-    strips nor the tiles fit, so the default rule takes the wavefront
-    strips): 1 + 2 x 4 steps bitwise the oracle."""
+    256-wide strips nor the tiles fit, so the default rule takes the
+    three-step walk on strips of any width): 1 + 2 x 4 steps bitwise the
+    oracle."""
     nx = ny = 1000
     rng = np.random.default_rng(7)
     ini, st, nf = orc.stream_map(9, nx, ny, 1)
