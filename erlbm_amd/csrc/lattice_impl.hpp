@@ -182,7 +182,8 @@ struct erlbm_lattice {
     //                    22..25 strip walks (256x32, 256x16, 128x32, 256x64), 26..29 with
     //                    segments sized to the resident workgroups (256- and 128-wide,
     //                    lean rings; 256x32 lean; 256-wide 18-row rings); D3Q19 (two steps,
-    //                    x-y tiles walked along z): 30..35 (32x8, 64x4, 32x4, 64x8, 128x4, 32x16)
+    //                    x-y tiles walked along z): 30..35 (32x8, 64x4, 32x4, 64x8, 128x4, 32x16);
+    //                    D2Q9 three steps on strips of any width: 36..39 (1..4 wavefronts)
     int tune_fuse = 26;
     int tune_fuse_remap = 1;  //   ERLBM_FUSE_REMAP 1: XCD-contiguous runs of tiles (0: dispatch order)
     bool checked = false;     // a stop test has run since the upload (no fused steps then)
