@@ -92,8 +92,7 @@ def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True):
         launch = lat.autotune(200)[0]
     else:
         launch = ("16-byte vectors, one step per launch (default)" if prec == E.FP16_SHIFTED
-                  else "two steps, 256-wide strips x 32 rows, XCD runs (default)" if prec == E.FP32_SHIFTED
-                  else "three steps, 256-wide strips, sized segments, XCD runs (default)")
+                  else lat.step_shape()[2] + " (default)")
     ms = lat.step_timed(steps)
     steps_done, reason = lat.status()
     ms_per_step = ms / steps

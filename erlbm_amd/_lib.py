@@ -105,6 +105,7 @@ SIGNATURES = {
     "erlbm_batch_set_launch": (_i, [_vp, _i, _i]),
     "erlbm_batch_set_chunk": (_i, [_vp, _i]),
     "erlbm_lattice_autotune": (_i, [_vp, _i, C.c_char_p, _i, C.POINTER(C.c_float)]),
+    "erlbm_step_shape": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.c_char_p, _i]),
     "erlbm_velocity_field": (_i, [_vp, _i, _vp]),
     "erlbm_vorticity_field": (_i, [_vp, _i, _vp]),
     "erlbm_enstrophy": (_i, [_vp, _i, C.POINTER(C.c_double)]),

@@ -25,6 +25,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <type_traits>
 
 #include "collide.hpp"
 #include "common.hpp"
@@ -1382,7 +1383,11 @@ constexpr FuseShape FUSE_SHAPES[] = {
     // three steps per launch, lean rings, sized segments: strips of any
     // width up to tx = 64 NW - 4 for NW = 1 .. 4 wavefronts per workgroup
     {60, 0, FUSE_GSTRIP3},             {124, 0, FUSE_GSTRIP3},           {188, 0, FUSE_GSTRIP3},
-    {252, 0, FUSE_GSTRIP3}};
+    {252, 0, FUSE_GSTRIP3},
+    // the same with the lanes aligned across the phases (c_x = 0 populations
+    // in registers), 2, 3, 4, 8, 5 and 6 wavefronts per workgroup
+    {124, 0, FUSE_GSTRIP3},            {188, 0, FUSE_GSTRIP3},           {252, 0, FUSE_GSTRIP3},
+    {508, 0, FUSE_GSTRIP3},            {316, 0, FUSE_GSTRIP3},           {380, 0, FUSE_GSTRIP3}};
 constexpr int N_FUSE_SHAPES = sizeof(FUSE_SHAPES) / sizeof(FUSE_SHAPES[0]);
 
 // Steps one launch of the shape advances.
@@ -1459,10 +1464,13 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
 // The multi-step shape a call of erlbm_step uses (0: one step per launch).
 // Without an explicit ERLBM_FUSE, a shape must give every CU a workgroup
 // (small lattices have too few strips or tiles to fill the chip: there the
-// one-step kernel is faster).  FP64: the three-step strip walk on 256-wide
-// strips, else on strips of any width (four wavefronts), else the two-step
+// one-step kernel is faster).  FP64: the three-step walk, lane-aligned on
+// strips of any width or on 256-wide strips (the rule below), else on
+// strips of any width (four wavefronts, lanes shifted), else the two-step
 // walk, else the 64x8 tiles, else the wavefront strips (any width from 64
 // up); shifted FP32 storage: the two-step strip walk.
+long long strip3_workgroups(const erlbm_lattice *h, int f);
+
 int fuse_shape(const erlbm_lattice *h, int energy_interval)
 {
     if (!(h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0 && h->dens_limit <= 0.0 && energy_interval <= 0 &&
@@ -1479,6 +1487,19 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
     // measured 52.2 vs 50.3 us/step at L = 2048: with half the bytes the
     // extra halo collisions cost more than the traffic saved)
     const int first = h->elem == 8 ? h->tune_fuse : FUSE2_STRIP_256x32;
+    // FP64: the lane-aligned walk on strips of any width (four wavefronts,
+    // four workgroups per CU) before the 256-wide walk where it puts no more
+    // workgroups on a CU than that one, or makes three per CU or more
+    // (MLUPS, aligned / 256-wide: L = 1024 49.6 / 57.3 (2 vs 1 per CU); 1280
+    // 70.4 / 60.2; 1536 73.9 / 65.8; 2048 87.8 / 85.4; 4096 95.5 / 96.3)
+    if (h->elem == 8 && first == FUSE3_STRIP_SIZED && fuse_shape_fits(h, FUSE3_ASTRIP_4W)) {
+        const long long wa = strip3_workgroups(h, FUSE3_ASTRIP_4W);
+        const bool wide = fuse_shape_fits(h, first) && fuse_workgroups(h, FUSE_SHAPES[first]) >= h->n_cu;
+        const long long ww = wide ? strip3_workgroups(h, first) : 0;
+        const long long n = h->n_cu;
+        if (wide ? ((wa + n - 1) / n <= (ww + n - 1) / n || wa >= 3 * n) : wa >= (19 * n + 19) / 20)
+            return FUSE3_ASTRIP_4W;
+    }
     for (int f : {first, FUSE3_GSTRIP_4W, FUSE2_STRIP_256x32, 1, 16}) {
         if (!fuse_shape_fits(h, f)) continue;
         // strips of any width: from 95% of a workgroup per CU (L = 1000:
@@ -1533,18 +1554,34 @@ void launch_wave_t(erlbm_lattice *h, const StepArgs &a, int rounds)
 // SEG > 0: segments of SEG rows; SEG = 0: as many segments as make one
 // workgroup per resident slot, rows split evenly, at least 16 per segment
 // (a segment recomputes 4 + 2 halo rows of the first two steps).
-template <int COL, int TX, int SEG, bool LEAN = false>
-void launch_strip3_t(erlbm_lattice *h, const StepArgs &a)
+template <int TX, bool LEAN>
+constexpr size_t strip3_lds()
 {
-    constexpr size_t lds = sizeof(double) * (LEAN ? LEAN_ROWS : RING_ROWS) * ((TX + 4) + (TX + 2));
-    const void *k = reinterpret_cast<const void *>(&k_step3_strip<COL, TX, LEAN>);
-    static const int per_cu = [&] {
+    return sizeof(double) * (LEAN ? LEAN_ROWS : RING_ROWS) * ((TX + 4) + (TX + 2));
+}
+
+// Workgroups of k_step3_strip<COL, TX, LEAN> that fit a CU (occupancy query,
+// once; raises the kernel's dynamic-LDS limit where it needs more than 64 KB).
+template <int COL, int TX, bool LEAN>
+int strip3_per_cu()
+{
+    static const int per_cu = [] {
+        constexpr size_t lds = strip3_lds<TX, LEAN>();
+        const void *k = reinterpret_cast<const void *>(&k_step3_strip<COL, TX, LEAN>);
         if constexpr (lds > 65536) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         int nblk = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, k, strip_threads<TX>(), lds) != hipSuccess || nblk < 1)
             nblk = 1;
         return nblk;
     }();
+    return per_cu;
+}
+
+template <int COL, int TX, int SEG, bool LEAN = false>
+void launch_strip3_t(erlbm_lattice *h, const StepArgs &a)
+{
+    constexpr size_t lds = strip3_lds<TX, LEAN>();
+    const int per_cu = strip3_per_cu<COL, TX, LEAN>();
     const int nstrip = h->nx / TX;
     const int nseg = SEG > 0 ? h->ny / SEG
                              : (int)std::max(1ll, std::min<long long>(h->ny / 16, (long long)h->n_cu * per_cu / nstrip));
@@ -1552,24 +1589,204 @@ void launch_strip3_t(erlbm_lattice *h, const StepArgs &a)
                        h->stream, a, h->tune_fuse_remap, nseg);
 }
 
+// The three-step walk with the lanes aligned across the phases: lane t
+// holds position x0 - 2 + t in all three (phase 1: lanes 0 .. w+3, phase 2:
+// 1 .. w+2, phase 3 and the stores: 2 .. w+1), so a pull of a c_x = 0
+// population reads the lane's own result of the step before, which stays in
+// registers (the source row's age, in walking steps, is 1 + c_y DIR: 0 is
+// the row the phase before has just computed).  The LDS rings keep only the
+// six c_x != 0 populations (lean layout, 8 rows each): 32 KB for NW = 4, so
+// four or five workgroups share a CU instead of three.  Barriers, lean late
+// writes and walking directions as in strip3_walk.
+template <int COL, int NW, int DIR>
+__device__ __forceinline__ void strip3_walk_aligned(const StepArgs &a, int x0, int w, int ya, int rows, double *ring1,
+                                                    double *ring2)
+{
+    using V = VSet<9>;
+    constexpr int E = 64 * NW;
+    const int t = (int)threadIdx.x;
+    const int nx = a.g.nx, ny = a.g.ny;
+    const bool act1 = t < w + 4, act2 = t >= 1 && t < w + 3, act3 = t >= 2 && t < w + 2;
+    const int x1 = act1 ? wrap(x0 - 2 + t, nx) : 0;
+    const __amdgpu_buffer_rsrc_t rs = pop_buffer<8>(a.src, a.g), rd = pop_buffer<8>(a.dst, a.g);
+    const unsigned oob1 = act1 ? 0u : BUF_OOB;
+    const unsigned vo[3] = {8u * wrap(x1 + 1, nx) | oob1, 8u * x1 | oob1, 8u * wrap(x1 - 1, nx) | oob1};
+    const unsigned vst = act3 ? 8u * (x0 - 2 + t) : 0u;
+    auto load_row = [&](int y, double (&f)[9], bool skip) {   // pulls of step-k row y (-3 .. ny+2)
+        const unsigned off = skip ? BUF_OOB : 0u;
+        const int yw = wrap(y, ny);
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            const int ys = V::cy[i] == 0 ? yw : wrap(yw - V::cy[i], ny);
+            f[i] = buf_load<V::cx[i] == 0 ? BUF_NT : 0>(rs, vo[V::cx[i] + 1] | off, (int)(8 * at(a.g, i, ys, 0)));
+        });
+    };
+    auto rrow = [&](int i, int y) { return xring_base<DIR>(i) + (y - ya + 2) % xring_depth<DIR>(i); };
+    auto inring = [](int i) { return V::cx[i] != 0; };
+    auto now = [](int i) { return V::cy[i] * DIR < 0; };
+    auto age = [](int i) { return 1 + V::cy[i] * DIR; };
+    auto collide = [&](double (&f)[9]) {
+        double vx, vy, vz;
+        collide_exact<9, COL>(f, a.k, vx, vy, vz);
+    };
+    const int r0 = DIR > 0 ? ya - 2 : ya + rows + 1;
+    double fq[9];
+    double h2[9];                  // the previous walking step's step-(k+1) row, still to be written to ring 2
+    double k1a[9], k1b[9];         // c_x = 0: step-k results of the rows one and two walking steps back
+    double k2a[9], k2b[9];         // c_x = 0: step-(k+1) results, likewise
+    load_row(r0, fq, false);
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+    nine_dropped_stores<double>(rd);
+    for (int n = 0; n < rows + 4; n++) {
+        const int r = r0 + DIR * n;
+        double f[9];
+        static_for<9>([&](auto I) { f[(int)I] = fq[(int)I]; });
+        load_row(r + DIR, fq, n + 1 >= rows + 4);                    // in flight meanwhile
+        if (act1) {                                                   // phase 1: step k, row r
+            collide(f);
+            static_for<9>([&](auto I) {
+                if (inring(I) && now(I)) ring1[rrow((int)I, r) * E + t] = f[(int)I];
+            });
+        }
+        lds_barrier();
+        if (act2 && n >= 3) {                                         // the rest of row r-2 DIR
+            static_for<9>([&](auto I) {
+                if (inring(I) && !now(I)) ring2[rrow((int)I, r - 2 * DIR) * E + t] = h2[(int)I];
+            });
+        }
+        double g[9];
+        if (act2 && n >= 2) {                                         // phase 2: step k+1, row r-DIR
+            const int y = r - DIR;
+            static_for<9>([&](auto I) {
+                constexpr int i = I;
+                if constexpr (V::cx[i] != 0) g[i] = ring1[rrow(i, y - V::cy[i]) * E + (t - V::cx[i])];
+                else g[i] = age(i) == 0 ? f[i] : (age(i) == 1 ? k1a[i] : k1b[i]);
+            });
+            collide(g);
+            static_for<9>([&](auto I) {
+                if (inring(I) && now(I)) ring2[rrow((int)I, y) * E + t] = g[(int)I];
+                h2[(int)I] = g[(int)I];
+            });
+        }
+        static_for<9>([&](auto I) {
+            if constexpr (V::cx[(int)I] == 0) {
+                k1b[(int)I] = k1a[(int)I];
+                k1a[(int)I] = f[(int)I];
+            }
+        });
+        lds_barrier();
+        if (act1) {                                                   // the rest of row r
+            static_for<9>([&](auto I) {
+                if (inring(I) && !now(I)) ring1[rrow((int)I, r) * E + t] = f[(int)I];
+            });
+        }
+        const bool out = n >= 4;                                      // phase 3: step k+2, row r-2 DIR
+        const int y = out ? r - 2 * DIR : ya;
+        double o[9] = {};
+        if (act3 && out) {
+            static_for<9>([&](auto I) {
+                constexpr int i = I;
+                if constexpr (V::cx[i] != 0) o[i] = ring2[rrow(i, y - V::cy[i]) * E + (t - V::cx[i])];
+                else o[i] = age(i) == 0 ? h2[i] : (age(i) == 1 ? k2a[i] : k2b[i]);
+            });
+            collide(o);
+        }
+        static_for<9>([&](auto I) {
+            if constexpr (V::cx[(int)I] == 0) {
+                k2b[(int)I] = k2a[(int)I];
+                k2a[(int)I] = h2[(int)I];
+            }
+        });
+        const unsigned sto = act3 && out ? 0u : BUF_OOB;
+        static_for<9>([&](auto I) { buf_store_nt(o[(int)I], rd, vst | sto, (int)(8 * at(a.g, (int)I, y, 0))); });
+    }
+}
+
+template <int COL, int NW>
+__global__ __launch_bounds__(64 * NW) void k_step3_astrip(StepArgs a, int remap, int nstrip, int nseg)
+{
+    constexpr int E = 64 * NW;
+    extern __shared__ double rings[];   // ring 1 [XRING_ROWS][E], ring 2 [XRING_ROWS][E]
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    const int s = (int)(b % (unsigned)nstrip);
+    const int x0 = (int)((long long)s * a.g.nx / nstrip);
+    const int w = (int)((long long)(s + 1) * a.g.nx / nstrip) - x0;
+    const int seg = (int)(b / (unsigned)nstrip);
+    const int ya = (int)((long long)seg * a.g.ny / nseg);
+    const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
+    if (seg & 1) strip3_walk_aligned<COL, NW, -1>(a, x0, w, ya, rows, rings, rings + XRING_ROWS * E);
+    else strip3_walk_aligned<COL, NW, 1>(a, x0, w, ya, rows, rings, rings + XRING_ROWS * E);
+}
+
 // Strips of at most 64 NW - 4 positions, as few as the width allows;
 // segments as for launch_strip3_t with SEG = 0.
-template <int COL, int NW>
-void launch_gstrip3_t(erlbm_lattice *h, const StepArgs &a)
+template <int NW, bool ALIGNED>
+constexpr size_t gstrip3_lds()
 {
-    constexpr int TX = 64 * NW - 4;
-    constexpr size_t lds = sizeof(double) * LEAN_ROWS * ((TX + 4) + (TX + 2));
-    const void *k = reinterpret_cast<const void *>(&k_step3_gstrip<COL, NW>);
-    static const int per_cu = [&] {
+    return ALIGNED ? sizeof(double) * XRING_ROWS * 2 * (64 * NW)
+                   : sizeof(double) * LEAN_ROWS * ((64 * NW) + (64 * NW - 2));
+}
+
+// Workgroups of k_step3_gstrip / k_step3_astrip <COL, NW> that fit a CU.
+template <int COL, int NW, bool ALIGNED>
+int gstrip3_per_cu()
+{
+    static const int per_cu = [] {
+        constexpr size_t lds = gstrip3_lds<NW, ALIGNED>();
+        const void *k = ALIGNED ? reinterpret_cast<const void *>(&k_step3_astrip<COL, NW>)
+                                : reinterpret_cast<const void *>(&k_step3_gstrip<COL, NW>);
         if constexpr (lds > 65536) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         int nblk = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, k, 64 * NW, lds) != hipSuccess || nblk < 1) nblk = 1;
         return nblk;
     }();
-    const int nstrip = (h->nx + TX - 1) / TX;
-    const int nseg = (int)std::max(1ll, std::min<long long>(h->ny / 16, (long long)h->n_cu * per_cu / nstrip));
-    hipLaunchKernelGGL((k_step3_gstrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream, a,
-                       h->tune_fuse_remap, nstrip, nseg);
+    return per_cu;
+}
+
+// Strips and segments of one three-step launch on strips of at most tx
+// positions (any width: as few strips as the width allows; else tx-wide
+// strips) with per_cu workgroups fitting a CU: as many segments as make one
+// workgroup per resident slot, at least 16 rows each.
+void strip3_grid(const erlbm_lattice *h, int tx, bool any_width, int per_cu, int &nstrip, int &nseg)
+{
+    nstrip = any_width ? (h->nx + tx - 1) / tx : h->nx / tx;
+    nseg = (int)std::max(1ll, std::min<long long>(h->ny / 16, (long long)h->n_cu * per_cu / nstrip));
+}
+
+// Workgroups one launch of the 256-wide lean walk with sized segments (26)
+// or of the lane-aligned walk with four wavefronts (42) makes here.
+long long strip3_workgroups(const erlbm_lattice *h, int f)
+{
+    auto per_cu = [&](auto col) -> int {
+        constexpr int C = decltype(col)::value;
+        return f == FUSE3_ASTRIP_4W ? gstrip3_per_cu<C, 4, true>() : strip3_per_cu<C, 256, true>();
+    };
+    int pc = 1;
+    switch (h->col) {
+    case ERLBM_COL_BGK: pc = per_cu(std::integral_constant<int, COL_BGK>{}); break;
+    case ERLBM_COL_MRT: pc = per_cu(std::integral_constant<int, COL_MRT>{}); break;
+    case ERLBM_COL_REG: pc = per_cu(std::integral_constant<int, COL_REG>{}); break;
+    default: pc = per_cu(std::integral_constant<int, COL_EXT_REG>{}); break;
+    }
+    int nstrip, nseg;
+    strip3_grid(h, FUSE_SHAPES[f].tx, f == FUSE3_ASTRIP_4W, pc, nstrip, nseg);
+    return (long long)nstrip * nseg;
+}
+
+template <int COL, int NW, bool ALIGNED = false>
+void launch_gstrip3_t(erlbm_lattice *h, const StepArgs &a)
+{
+    constexpr size_t lds = gstrip3_lds<NW, ALIGNED>();
+    int nstrip, nseg;
+    strip3_grid(h, 64 * NW - 4, true, gstrip3_per_cu<COL, NW, ALIGNED>(), nstrip, nseg);
+    if constexpr (ALIGNED)
+        hipLaunchKernelGGL((k_step3_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream,
+                           a, h->tune_fuse_remap, nstrip, nseg);
+    else
+        hipLaunchKernelGGL((k_step3_gstrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream,
+                           a, h->tune_fuse_remap, nstrip, nseg);
 }
 
 // Segments as many as make one workgroup per resident slot, at least 16
@@ -1619,6 +1836,12 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
     case FUSE3_GSTRIP_2W: launch_gstrip3_t<COL, 2>(h, a); return;
     case FUSE3_GSTRIP_3W: launch_gstrip3_t<COL, 3>(h, a); return;
     case FUSE3_GSTRIP_4W: launch_gstrip3_t<COL, 4>(h, a); return;
+    case FUSE3_ASTRIP_2W: launch_gstrip3_t<COL, 2, true>(h, a); return;
+    case FUSE3_ASTRIP_3W: launch_gstrip3_t<COL, 3, true>(h, a); return;
+    case FUSE3_ASTRIP_4W: launch_gstrip3_t<COL, 4, true>(h, a); return;
+    case FUSE3_ASTRIP_8W: launch_gstrip3_t<COL, 8, true>(h, a); return;
+    case FUSE3_ASTRIP_5W: launch_gstrip3_t<COL, 5, true>(h, a); return;
+    case FUSE3_ASTRIP_6W: launch_gstrip3_t<COL, 6, true>(h, a); return;
     case 16: launch_wave_t<COL, 1, 4, 1>(h, a, 2); return;
     case 17: launch_wave_t<COL, 1, 4, 1>(h, a, 1); return;
     case 18: launch_wave_t<COL, 2, 4, 1>(h, a, 2); return;
@@ -1764,7 +1987,9 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
             {22, {"three steps, 256-wide strips x 32 rows", "three steps, 256-wide strips x 32 rows, XCD runs"}},
             {26, {"three steps, 256-wide strips, sized segments", "three steps, 256-wide strips, sized segments, XCD runs"}},
             {FUSE3_GSTRIP_4W, {"three steps, strips of any width (4 wavefronts)", "three steps, strips of any width (4 wavefronts), XCD runs"}},
-            {FUSE3_GSTRIP_3W, {"three steps, strips of any width (3 wavefronts)", "three steps, strips of any width (3 wavefronts), XCD runs"}}};
+            {FUSE3_GSTRIP_3W, {"three steps, strips of any width (3 wavefronts)", "three steps, strips of any width (3 wavefronts), XCD runs"}},
+            {FUSE3_ASTRIP_4W, {"three steps, lane-aligned strips of any width (4 wavefronts)",
+                               "three steps, lane-aligned strips of any width (4 wavefronts), XCD runs"}}};
         for (const Named &nm : fused) {
             if (!fuse_geometry_fits(h, FUSE_SHAPES[nm.f])) continue;
             for (int rm = 0; rm < 2; rm++) {
@@ -1825,6 +2050,46 @@ void apply_shape(erlbm_lattice *h, const LaunchShape &s)
 }
 
 }  // namespace
+
+namespace {
+
+// Description of a multi-step shape, in the autotuner's words.
+std::string fuse_shape_desc(int f, int remap)
+{
+    if (f <= 0 || f >= N_FUSE_SHAPES) return "one step per launch";
+    const FuseShape fs = FUSE_SHAPES[f];
+    const std::string tx = std::to_string(fs.tx), ty = std::to_string(fs.ty);
+    std::string d;
+    switch (fs.kind) {
+    case FUSE_TILE: d = "two steps, " + tx + "x" + ty + " tiles"; break;
+    case FUSE_STRIP: d = "two steps, " + tx + "-wide strips x " + ty + " rows"; break;
+    case FUSE_WAVE: d = "two steps, wavefront strips"; break;
+    case FUSE_STRIP3:
+        d = "three steps, " + tx + "-wide strips" + (fs.ty ? " x " + ty + " rows" : std::string(", sized segments"));
+        break;
+    case FUSE_GSTRIP3:
+        d = std::string("three steps, ") + (f >= FUSE3_ASTRIP_2W ? "lane-aligned " : "") + "strips of any width (" +
+            std::to_string((fs.tx + 4) / 64) + " wavefronts)";
+        break;
+    case FUSE_ZWALK: d = "two steps, " + tx + "x" + ty + " tiles walked along z"; break;
+    }
+    return remap ? d + ", XCD runs" : d;
+}
+
+}  // namespace
+
+extern "C" int erlbm_step_shape(erlbm_lattice *h, int *shape, int *steps_per_launch, char *desc, int desc_len)
+{
+    if (!h) return ERLBM_ERR_ARG;
+    const int f = fuse_shape(h, 0);
+    if (shape) *shape = f;
+    if (steps_per_launch) *steps_per_launch = f > 0 ? fuse_steps(f) : 1;
+    if (desc && desc_len > 0) {
+        const std::string d = fuse_shape_desc(f, h->tune_fuse_remap);
+        std::snprintf(desc, (size_t)desc_len, "%s", d.c_str());
+    }
+    return ERLBM_OK;
+}
 
 extern "C" int erlbm_lattice_autotune(erlbm_lattice *h, int n_steps, char *choice, int choice_len, float *us_per_step)
 {

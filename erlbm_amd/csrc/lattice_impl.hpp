@@ -122,12 +122,15 @@ __device__ inline void zouhe(double (&f)[Q], double vx, bool west)
 // Multi-step shapes (ERLBM_FUSE values) that the FP32 shifted-storage path
 // also has.
 constexpr int FUSE2_STRIP_256x32 = 11, FUSE2_STRIP_128x32 = 13;
-constexpr int FUSE3_STRIP_256x32 = 22, FUSE3_STRIP_128x32 = 24;
+constexpr int FUSE3_STRIP_256x32 = 22, FUSE3_STRIP_128x32 = 24, FUSE3_STRIP_SIZED = 26;
 // D3Q19 two-step z-walks (x-y tiles)
 constexpr int FUSE2_ZWALK_32x8 = 30, FUSE2_ZWALK_64x4 = 31, FUSE2_ZWALK_32x4 = 32, FUSE2_ZWALK_64x8 = 33,
               FUSE2_ZWALK_128x4 = 34, FUSE2_ZWALK_32x16 = 35;
 // D2Q9 three-step walks on strips of any width, 1 .. 4 wavefronts per workgroup
 constexpr int FUSE3_GSTRIP_1W = 36, FUSE3_GSTRIP_2W = 37, FUSE3_GSTRIP_3W = 38, FUSE3_GSTRIP_4W = 39;
+// the same with the lanes aligned across the walk's phases, 2, 3, 4, 8 wavefronts
+constexpr int FUSE3_ASTRIP_2W = 40, FUSE3_ASTRIP_3W = 41, FUSE3_ASTRIP_4W = 42, FUSE3_ASTRIP_8W = 43,
+              FUSE3_ASTRIP_5W = 44, FUSE3_ASTRIP_6W = 45;
 
 struct erlbm_lattice {
     int q = 0, nx = 0, ny = 0, nz = 0;

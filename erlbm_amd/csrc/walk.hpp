@@ -97,6 +97,22 @@ __host__ __device__ constexpr int lean_base(int i)
 constexpr int LEAN_ROWS = lean_base<1>(8) + lean_depth<1>(8);
 static_assert(LEAN_ROWS == 12 && lean_base<-1>(8) + lean_depth<-1>(8) == LEAN_ROWS, "D2Q9 lean ring rows");
 
+// Lean rings of the lane-aligned three-step walk: the c_x != 0 populations
+// only (a c_x = 0 population is pulled by the lane that computed it, from
+// registers): 8 rows.
+template <int DIR>
+__host__ __device__ constexpr int xring_depth(int i)
+{
+    return VSet<9>::cx[i] == 0 ? 0 : lean_depth<DIR>(i);
+}
+template <int DIR>
+__host__ __device__ constexpr int xring_base(int i)
+{
+    return i == 0 ? 0 : xring_base<DIR>(i - 1) + xring_depth<DIR>(i - 1);
+}
+constexpr int XRING_ROWS = xring_base<1>(8) + xring_depth<1>(8);
+static_assert(XRING_ROWS == 8 && xring_base<-1>(8) + xring_depth<-1>(8) == XRING_ROWS, "D2Q9 aligned ring rows");
+
 // Lean rings of the D3Q19 z-walk, by c_z: 2 / 1 / 1 plane slots for the
 // populations a pull reads from the oldest (c_z = DIR), middle and newest
 // of its three planes; 24 planes in all.

@@ -104,6 +104,14 @@ class Lattice:
         check(lib().erlbm_lattice_autotune(self._h, n_steps, buf, 128, C.byref(us)), "erlbm_lattice_autotune")
         return buf.value.decode(), us.value
 
+    def step_shape(self):
+        """The multi-step kernel shape a plain step() call uses now: (shape,
+        steps per launch, description); shape 0 = one step per launch."""
+        f, spl = C.c_int(0), C.c_int(0)
+        buf = C.create_string_buffer(128)
+        check(lib().erlbm_step_shape(self._h, C.byref(f), C.byref(spl), buf, 128), "erlbm_step_shape")
+        return f.value, spl.value, buf.value.decode()
+
     def status(self):
         s, r = C.c_longlong(0), C.c_int(0)
         check(lib().erlbm_status(self._h, C.byref(s), C.byref(r)), "erlbm_status")

@@ -122,6 +122,12 @@ int erlbm_step_timed(erlbm_lattice *h, int n_steps, float *ms);
  * advanced.  Geometries with solids or boundary columns have one kernel
  * and nothing to choose.  Needs uploaded populations; synchronises.      */
 int erlbm_lattice_autotune(erlbm_lattice *h, int n_steps, char *choice, int choice_len, float *us_per_step);
+/* Diagnostics (no counterpart in the reference): the multi-step kernel
+ * shape (the ERLBM_FUSE value; 0 = one step per launch), the steps one of
+ * its launches advances and its description (`desc`, may be NULL) that
+ * erlbm_step(h, n, 0, 0) would use now for all but a first step after an
+ * upload and the remainder.                                              */
+int erlbm_step_shape(erlbm_lattice *h, int *shape, int *steps_per_launch, char *desc, int desc_len);
 /* Global steps run so far and the stop reason (0 if none); synchronises. */
 int erlbm_status(erlbm_lattice *h, long long *steps_done, int *stop_reason);
 int erlbm_synchronize(erlbm_lattice *h);

@@ -270,7 +270,7 @@ def test_mrt_is_d2q9_only(gpu):
 
 
 @pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29,
-                                   36, 37, 38, 39])
+                                   36, 37, 38, 39, 40, 41, 42, 43, 44, 45])
 def test_two_step_fused_kernel_bitwise(gpu, orc, monkeypatch, shape):
     """Two or three steps per launch (tiles, strip walks, wavefront strips):
     any mix of call lengths gives the populations, post-collision monitors
@@ -331,11 +331,15 @@ def test_wavefront_strips_ragged_bitwise(gpu, orc, monkeypatch, col, shape, dims
 
 @pytest.mark.parametrize("col", [O.COL_BGK, O.COL_MRT, O.COL_REG, O.COL_EXT_REG])
 @pytest.mark.parametrize("shape,dims", [(36, (300, 77)), (37, (130, 33)), (38, (2048, 40)), (39, (1000, 24)),
-                                        (38, (17, 19)), (36, (16, 16)), (39, (253, 16))])
+                                        (38, (17, 19)), (36, (16, 16)), (39, (253, 16)), (40, (130, 33)),
+                                        (41, (300, 77)), (42, (1000, 24)), (42, (16, 16)), (42, (253, 16)),
+                                        (43, (1500, 40)), (43, (509, 17)), (44, (2048, 40)), (44, (317, 20)),
+                                        (45, (1000, 33))])
 def test_any_width_strips_ragged_bitwise(gpu, orc, monkeypatch, col, shape, dims):
     """The three-step walk on strips of any width (widths split evenly, at
-    most 64 NW - 4 for NW wavefronts) on lattices of ragged widths and
-    heights, down to 16 x 16 (one strip as wide as the lattice): bitwise
+    most 64 NW - 4 for NW wavefronts; 40-43: lanes aligned across the
+    phases, c_x = 0 populations in registers) on lattices of ragged widths
+    and heights, down to 16 x 16 (one strip as wide as the lattice): bitwise
     the oracle for call lengths that are and are not multiples of three,
     for every collision model."""
     nx, ny = dims
@@ -453,10 +457,29 @@ def test_headline_default_kernel_full_size(gpu, orc):
     ini, st, nf = orc.stream_map(9, L, L, 1)
     want = orc.run_steps(9, f0, st, n, O.COL_EXT_REG, tau, 0.65, nthreads=16)
     with _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0) as lat:
+        shape, spl, desc = lat.step_shape()
+        print(f"headline default: shape {shape}, {desc}", flush=True)
+        assert spl == 3
         lat.step(n)
         got = lat.download()
     assert np.array_equal(got, want)
     assert abs(got.sum() - f0.sum()) <= 1e-12 * f0.sum()
+
+
+def test_default_shape_rule(gpu):
+    """The default multi-step shape per lattice size (periodic FP64 D2Q9
+    E-RLBM; gfx950 parts have 256 CUs): the lane-aligned any-width walk
+    where it loads the CUs no more unevenly than the 256-wide walk or makes
+    three workgroups per CU, the 256-wide walk at L = 1024, the 64x8 two-step
+    tiles at L = 768, and steps_per_launch / descriptions that match."""
+    want = {768: (1, 2), 1000: (42, 3), 1024: (26, 3), 1280: (42, 3), 2048: (42, 3), 3000: (42, 3)}
+    for L, (shape, spl) in want.items():
+        with gpu.Lattice(9, L, L, 1) as lat:
+            lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
+            got = lat.step_shape()
+            print(f"L={L}: {got}", flush=True)
+            assert got[:2] == (shape, spl), L
+            assert got[2].startswith("three steps" if spl == 3 else "two steps")
 
 
 @pytest.mark.parametrize("shape", [11, 26])
