@@ -1702,8 +1702,9 @@ __device__ __forceinline__ void strip3_walk_aligned(const StepArgs &a, int x0, i
     }
 }
 
+// Strip edges on multiples of `unit` positions (16: 128-byte lines).
 template <int COL, int NW>
-__global__ __launch_bounds__(64 * NW) void k_step3_astrip(StepArgs a, int remap, int nstrip, int nseg)
+__global__ __launch_bounds__(64 * NW) void k_step3_astrip(StepArgs a, int remap, int nstrip, int nseg, int unit)
 {
     constexpr int E = 64 * NW;
     extern __shared__ double rings[];   // ring 1 [XRING_ROWS][E], ring 2 [XRING_ROWS][E]
@@ -1711,8 +1712,9 @@ __global__ __launch_bounds__(64 * NW) void k_step3_astrip(StepArgs a, int remap,
     unsigned b = blockIdx.x;
     if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
     const int s = (int)(b % (unsigned)nstrip);
-    const int x0 = (int)((long long)s * a.g.nx / nstrip);
-    const int w = (int)((long long)(s + 1) * a.g.nx / nstrip) - x0;
+    const int units = a.g.nx / unit;
+    const int x0 = (int)((long long)s * units / nstrip) * unit;
+    const int w = (int)((long long)(s + 1) * units / nstrip) * unit - x0;
     const int seg = (int)(b / (unsigned)nstrip);
     const int ya = (int)((long long)seg * a.g.ny / nseg);
     const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
@@ -1749,10 +1751,19 @@ int gstrip3_per_cu()
 // positions (any width: as few strips as the width allows; else tx-wide
 // strips) with per_cu workgroups fitting a CU: as many segments as make one
 // workgroup per resident slot, at least 16 rows each.
-void strip3_grid(const erlbm_lattice *h, int tx, bool any_width, int per_cu, int &nstrip, int &nseg)
+void strip3_grid(const erlbm_lattice *h, int tx, bool any_width, int per_cu, int &nstrip, int &nseg, int unit = 1)
 {
-    nstrip = any_width ? (h->nx + tx - 1) / tx : h->nx / tx;
+    nstrip = any_width ? (h->nx / unit + tx / unit - 1) / (tx / unit) : h->nx / tx;
     nseg = (int)std::max(1ll, std::min<long long>(h->ny / 16, (long long)h->n_cu * per_cu / nstrip));
+}
+
+// Strip edges of the lane-aligned walk on 128-byte lines where the width
+// is a multiple of 16 positions (same-box MLUPS, edges anywhere / on lines:
+// L = 2048 87.9k / 88.9k, 4096 96.5k / 99.4k, 1536 71.6k / 72.3k: no line
+// at a strip edge is read twice or written in two parts).
+int astrip_unit(const erlbm_lattice *h)
+{
+    return h->nx % 16 == 0 ? 16 : 1;
 }
 
 // Workgroups one launch of the 256-wide lean walk with sized segments (26)
@@ -1771,7 +1782,7 @@ long long strip3_workgroups(const erlbm_lattice *h, int f)
     default: pc = per_cu(std::integral_constant<int, COL_EXT_REG>{}); break;
     }
     int nstrip, nseg;
-    strip3_grid(h, FUSE_SHAPES[f].tx, f == FUSE3_ASTRIP_4W, pc, nstrip, nseg);
+    strip3_grid(h, FUSE_SHAPES[f].tx, f == FUSE3_ASTRIP_4W, pc, nstrip, nseg, f == FUSE3_ASTRIP_4W ? astrip_unit(h) : 1);
     return (long long)nstrip * nseg;
 }
 
@@ -1779,11 +1790,12 @@ template <int COL, int NW, bool ALIGNED = false>
 void launch_gstrip3_t(erlbm_lattice *h, const StepArgs &a)
 {
     constexpr size_t lds = gstrip3_lds<NW, ALIGNED>();
+    const int unit = ALIGNED ? astrip_unit(h) : 1;
     int nstrip, nseg;
-    strip3_grid(h, 64 * NW - 4, true, gstrip3_per_cu<COL, NW, ALIGNED>(), nstrip, nseg);
+    strip3_grid(h, 64 * NW - 4, true, gstrip3_per_cu<COL, NW, ALIGNED>(), nstrip, nseg, unit);
     if constexpr (ALIGNED)
         hipLaunchKernelGGL((k_step3_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream,
-                           a, h->tune_fuse_remap, nstrip, nseg);
+                           a, h->tune_fuse_remap, nstrip, nseg, unit);
     else
         hipLaunchKernelGGL((k_step3_gstrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream,
                            a, h->tune_fuse_remap, nstrip, nseg);

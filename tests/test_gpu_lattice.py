@@ -334,7 +334,8 @@ def test_wavefront_strips_ragged_bitwise(gpu, orc, monkeypatch, col, shape, dims
                                         (38, (17, 19)), (36, (16, 16)), (39, (253, 16)), (40, (130, 33)),
                                         (41, (300, 77)), (42, (1000, 24)), (42, (16, 16)), (42, (253, 16)),
                                         (43, (1500, 40)), (43, (509, 17)), (44, (2048, 40)), (44, (317, 20)),
-                                        (45, (1000, 33))])
+                                        (45, (1000, 33)), (42, (2048, 40)), (42, (4096, 16)), (42, (4000, 16)),
+                                        (42, (272, 17))])
 def test_any_width_strips_ragged_bitwise(gpu, orc, monkeypatch, col, shape, dims):
     """The three-step walk on strips of any width (widths split evenly, at
     most 64 NW - 4 for NW wavefronts; 40-43: lanes aligned across the
@@ -472,7 +473,7 @@ def test_default_shape_rule(gpu):
     where it loads the CUs no more unevenly than the 256-wide walk or makes
     three workgroups per CU, the 256-wide walk at L = 1024, the 64x8 two-step
     tiles at L = 768, and steps_per_launch / descriptions that match."""
-    want = {768: (1, 2), 1000: (42, 3), 1024: (26, 3), 1280: (42, 3), 2048: (42, 3), 3000: (42, 3)}
+    want = {768: (1, 2), 1000: (42, 3), 1024: (26, 3), 1280: (42, 3), 2048: (42, 3), 3000: (42, 3), 4096: (42, 3)}
     for L, (shape, spl) in want.items():
         with gpu.Lattice(9, L, L, 1) as lat:
             lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
