@@ -1418,7 +1418,7 @@ bool fuse_geometry_fits(const erlbm_lattice *h, const FuseShape &fs)
     }
     if (h->nz != 1) return false;
     if (fs.kind == FUSE_WAVE) return h->nx >= 64 && h->extent * 8 < (1ll << 31);   // 31-bit buffer offsets
-    if (fs.kind == FUSE_GSTRIP3) return h->elem == 8 && h->extent * 8 < (1ll << 31) && h->nx >= 16 && h->ny >= 16;
+    if (fs.kind == FUSE_GSTRIP3) return h->extent * h->elem < (1ll << 31) && h->nx >= 16 && h->ny >= 16;
     if ((fs.kind == FUSE_STRIP || fs.kind == FUSE_STRIP3) && h->elem == 8 && h->extent * 8 >= (1ll << 31)) return false;
     if (fs.ty == 0) return h->nx % fs.tx == 0 && h->ny >= 16;
     return h->nx % fs.tx == 0 && h->ny % fs.ty == 0;
@@ -1453,7 +1453,7 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
     // VALU-bound once the bytes are that few)
     if (h->elem != 8) {
         const bool two = f == FUSE2_STRIP_256x32 || f == FUSE2_STRIP_128x32;
-        const bool three = f == FUSE3_STRIP_256x32 || f == FUSE3_STRIP_128x32;
+        const bool three = f == FUSE3_STRIP_256x32 || f == FUSE3_STRIP_128x32 || f == FUSE3_ASTRIP_4W;
         if (h->f32_arith || !(two || (three && h->elem == 4))) return false;
         if (h->elem == 2 && !h->fuse_requested) return false;
         if (h->extent * h->elem >= (1ll << 31)) return false;   // 31-bit buffer offsets
@@ -1468,7 +1468,8 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
 // strips of any width or on 256-wide strips (the rule below), else on
 // strips of any width (four wavefronts, lanes shifted), else the two-step
 // walk, else the 64x8 tiles, else the wavefront strips (any width from 64
-// up); shifted FP32 storage: the two-step strip walk.
+// up); shifted FP32 storage: the lane-aligned three-step walk, else the
+// two-step strip walk.
 long long strip3_workgroups(const erlbm_lattice *h, int f);
 
 int fuse_shape(const erlbm_lattice *h, int energy_interval)
@@ -1483,10 +1484,10 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
         return 0;
     }
     if (!(h->q == 9 && h->nz == 1)) return 0;
-    // FP32 storage: the two-step walk unless asked (its three-step walk
-    // measured 52.2 vs 50.3 us/step at L = 2048: with half the bytes the
-    // extra halo collisions cost more than the traffic saved)
-    const int first = h->elem == 8 ? h->tune_fuse : FUSE2_STRIP_256x32;
+    // FP32 storage: the lane-aligned three-step walk (L = 2048: 44.9 us/step
+    // against 49.8 for the two-step walk and 50.0 for the 256-wide
+    // three-step one), else the two-step walk
+    const int first = h->elem == 8 ? h->tune_fuse : FUSE3_ASTRIP_4W;
     // FP64: the lane-aligned walk on strips of any width (four wavefronts,
     // four workgroups per CU) before the 256-wide walk where it puts no more
     // workgroups on a CU than that one, or makes three per CU or more
@@ -1505,7 +1506,8 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
         // strips of any width: from 95% of a workgroup per CU (L = 1000:
         // 248 workgroups, 61.1k MLUPS against 46.4k for the wavefront
         // strips; L = 768: 192, 41.8k against 50.9k for the 64x8 tiles)
-        const long long need = f == FUSE3_GSTRIP_4W ? (19ll * h->n_cu + 19) / 20 : h->n_cu;
+        const bool any = f == FUSE3_GSTRIP_4W || f == FUSE3_ASTRIP_4W;
+        const long long need = any ? (19ll * h->n_cu + 19) / 20 : h->n_cu;
         if (fuse_workgroups(h, FUSE_SHAPES[f]) >= need) return f;
     }
     return 0;
@@ -2039,6 +2041,11 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
                 if (h->nx % FUSE_SHAPES[fs[k]].tx || h->ny % FUSE_SHAPES[fs[k]].ty) continue;
                 for (int rm = 0; rm < 2; rm++) c.push_back({h->tune_ntl, h->tune_block, 1, 0, 0, names[k][rm], fs[k], rm});
             }
+            static const char *aligned[2] = {"three steps, lane-aligned strips of any width (4 wavefronts)",
+                                             "three steps, lane-aligned strips of any width (4 wavefronts), XCD runs"};
+            if (fuse_geometry_fits(h, FUSE_SHAPES[FUSE3_ASTRIP_4W]))
+                for (int rm = 0; rm < 2; rm++)
+                    c.push_back({h->tune_ntl, h->tune_block, 1, 0, 0, aligned[rm], FUSE3_ASTRIP_4W, rm});
         }
         c.push_back({h->tune_ntl, h->tune_block, 1, 0, 0, "one node/thread"});
     }

@@ -409,6 +409,154 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_lp_step3_strip(LPArgs a
     else lp_strip3_walk<COL, TX, SEG, 1>(a, x0, seg * SEG, lp_rings, lp_rings + RING_ROWS * E1);
 }
 
+// The lane-aligned three-step walk of lattice.hip's k_step3_astrip for FP32
+// shifted storage (FP64 arithmetic): lane t holds position x0-2+t in all
+// three phases, the c_x = 0 populations stay in registers, the 8-row lean
+// FP32 rings hold the other six (16 KB for four wavefronts); the values
+// kept, in LDS or registers, are the FP32 ones one-step launches would
+// store and read back.
+template <int COL, int NW, int DIR>
+__device__ __forceinline__ void lp_strip3_walk_aligned(const LPArgs &a, int x0, int w, int ya, int rows, float *ring1,
+                                                       float *ring2)
+{
+    using V = VSet<9>;
+    constexpr int E = 64 * NW;
+    const int t = (int)threadIdx.x;
+    const int nx = a.g.nx, ny = a.g.ny;
+    const bool act1 = t < w + 4, act2 = t >= 1 && t < w + 3, act3 = t >= 2 && t < w + 2;
+    const int x1 = act1 ? wrap(x0 - 2 + t, nx) : 0;
+    const __amdgpu_buffer_rsrc_t rs = pop_buffer<4>(a.src, a.g), rd = pop_buffer<4>(a.dst, a.g);
+    const unsigned oob1 = act1 ? 0u : BUF_OOB;
+    const unsigned vo[3] = {4u * wrap(x1 + 1, nx) | oob1, 4u * x1 | oob1, 4u * wrap(x1 - 1, nx) | oob1};
+    const unsigned vst = act3 ? 4u * (x0 - 2 + t) : 0u;
+    auto load_row = [&](int y, float (&d)[9], bool skip) {   // pulls of step-k row y (-3 .. ny+2)
+        const unsigned off = skip ? BUF_OOB : 0u;
+        const int yw = wrap(y, ny);
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            const int ys = V::cy[i] == 0 ? yw : wrap(yw - V::cy[i], ny);
+            d[i] = buf_load_f32<V::cx[i] == 0 ? BUF_NT : 0>(rs, vo[V::cx[i] + 1] | off, (int)(4 * at(a.g, i, ys, 0)));
+        });
+    };
+    auto rrow = [&](int i, int y) { return xring_base<DIR>(i) + (y - ya + 2) % xring_depth<DIR>(i); };
+    auto inring = [](int i) { return V::cx[i] != 0; };
+    auto now = [](int i) { return V::cy[i] * DIR < 0; };
+    auto age = [](int i) { return 1 + V::cy[i] * DIR; };
+    const int r0 = DIR > 0 ? ya - 2 : ya + rows + 1;
+    float fq[9];
+    float h2[9];
+    float k1a[9], k1b[9];
+    float k2a[9], k2b[9];
+    load_row(r0, fq, false);
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+    nine_dropped_stores<float>(rd);
+    for (int n = 0; n < rows + 4; n++) {
+        const int r = r0 + DIR * n;
+        float f[9];
+        static_for<9>([&](auto I) { f[(int)I] = fq[(int)I]; });
+        load_row(r + DIR, fq, n + 1 >= rows + 4);                    // in flight meanwhile
+        if (act1) {                                                   // phase 1: step k, row r
+            (void)collide_lp<9, COL, false>(f, a);
+            static_for<9>([&](auto I) {
+                if (inring(I) && now(I)) ring1[rrow((int)I, r) * E + t] = f[(int)I];
+            });
+        }
+        lds_barrier();
+        if (act2 && n >= 3) {                                         // the rest of row r-2 DIR
+            static_for<9>([&](auto I) {
+                if (inring(I) && !now(I)) ring2[rrow((int)I, r - 2 * DIR) * E + t] = h2[(int)I];
+            });
+        }
+        if (act2 && n >= 2) {                                         // phase 2: step k+1, row r-DIR
+            const int y = r - DIR;
+            float g[9];
+            static_for<9>([&](auto I) {
+                constexpr int i = I;
+                if constexpr (V::cx[i] != 0) g[i] = ring1[rrow(i, y - V::cy[i]) * E + (t - V::cx[i])];
+                else g[i] = age(i) == 0 ? f[i] : (age(i) == 1 ? k1a[i] : k1b[i]);
+            });
+            (void)collide_lp<9, COL, false>(g, a);
+            static_for<9>([&](auto I) {
+                if (inring(I) && now(I)) ring2[rrow((int)I, y) * E + t] = g[(int)I];
+                h2[(int)I] = g[(int)I];
+            });
+        }
+        static_for<9>([&](auto I) {
+            if constexpr (V::cx[(int)I] == 0) {
+                k1b[(int)I] = k1a[(int)I];
+                k1a[(int)I] = f[(int)I];
+            }
+        });
+        lds_barrier();
+        if (act1) {                                                   // the rest of row r
+            static_for<9>([&](auto I) {
+                if (inring(I) && !now(I)) ring1[rrow((int)I, r) * E + t] = f[(int)I];
+            });
+        }
+        const bool out = n >= 4;                                      // phase 3: step k+2, row r-2 DIR
+        const int y = out ? r - 2 * DIR : ya;
+        float o[9] = {};
+        if (act3 && out) {
+            static_for<9>([&](auto I) {
+                constexpr int i = I;
+                if constexpr (V::cx[i] != 0) o[i] = ring2[rrow(i, y - V::cy[i]) * E + (t - V::cx[i])];
+                else o[i] = age(i) == 0 ? h2[i] : (age(i) == 1 ? k2a[i] : k2b[i]);
+            });
+            (void)collide_lp<9, COL, false>(o, a);
+        }
+        static_for<9>([&](auto I) {
+            if constexpr (V::cx[(int)I] == 0) {
+                k2b[(int)I] = k2a[(int)I];
+                k2a[(int)I] = h2[(int)I];
+            }
+        });
+        const unsigned sto = act3 && out ? 0u : BUF_OOB;
+        static_for<9>([&](auto I) { buf_store_nt(o[(int)I], rd, vst | sto, (int)(4 * at(a.g, (int)I, y, 0))); });
+    }
+}
+
+template <int COL, int NW>
+__global__ __launch_bounds__(64 * NW) void k_lp_step3_astrip(LPArgs a, int remap, int nstrip, int nseg, int unit)
+{
+    constexpr int E = 64 * NW;
+    extern __shared__ float lp_rings[];   // ring 1 [XRING_ROWS][E], ring 2 [XRING_ROWS][E]
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    const int s = (int)(b % (unsigned)nstrip);
+    const int units = a.g.nx / unit;
+    const int x0 = (int)((long long)s * units / nstrip) * unit;
+    const int w = (int)((long long)(s + 1) * units / nstrip) * unit - x0;
+    const int seg = (int)(b / (unsigned)nstrip);
+    const int ya = (int)((long long)seg * a.g.ny / nseg);
+    const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
+    if (seg & 1) lp_strip3_walk_aligned<COL, NW, -1>(a, x0, w, ya, rows, lp_rings, lp_rings + XRING_ROWS * E);
+    else lp_strip3_walk_aligned<COL, NW, 1>(a, x0, w, ya, rows, lp_rings, lp_rings + XRING_ROWS * E);
+}
+
+// Strips of at most 252 positions, edges on 16-position units where the
+// width allows; segments as many as make one workgroup per resident slot,
+// at least 16 rows each.
+template <int COL>
+void launch_lp_astrip(erlbm_lattice *h, const LPArgs &l)
+{
+    constexpr int NW = 4;
+    constexpr size_t lds = sizeof(float) * XRING_ROWS * 2 * (64 * NW);
+    static const int per_cu = [] {
+        int nblk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, reinterpret_cast<const void *>(&k_lp_step3_astrip<COL, NW>),
+                                                         64 * NW, lds) != hipSuccess ||
+            nblk < 1)
+            nblk = 1;
+        return nblk;
+    }();
+    const int unit = h->nx % 16 == 0 ? 16 : 1;
+    const int nstrip = (h->nx / unit + (64 * NW - 4) / unit - 1) / ((64 * NW - 4) / unit);
+    const int nseg = (int)std::max(1ll, std::min<long long>(h->ny / 16, (long long)h->n_cu * per_cu / nstrip));
+    hipLaunchKernelGGL((k_lp_step3_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream, l,
+                       h->tune_fuse_remap, nstrip, nseg, unit);
+}
+
 template <int COL, int TX, int SEG>
 void launch_lp_strip3(erlbm_lattice *h, const LPArgs &l)
 {
@@ -433,6 +581,7 @@ void launch_lp_step2_ct(erlbm_lattice *h, const LPArgs &l)
     if constexpr (sizeof(T) == 4) {
         if (h->fuse_active == FUSE3_STRIP_256x32) return launch_lp_strip3<COL, 256, 32>(h, l);
         if (h->fuse_active == FUSE3_STRIP_128x32) return launch_lp_strip3<COL, 128, 32>(h, l);
+        if (h->fuse_active == FUSE3_ASTRIP_4W) return launch_lp_astrip<COL>(h, l);
     }
     if (h->fuse_active == FUSE2_STRIP_128x32) launch_lp_strip<COL, T, 128, 32>(h, l);
     else launch_lp_strip<COL, T, 256, 32>(h, l);

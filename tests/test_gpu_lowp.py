@@ -107,7 +107,8 @@ def test_shifted_energy_monitor_fp64(gpu, orc):
 
 @pytest.mark.parametrize("prec,shape,remap", [("FP32_SHIFTED", 11, 1), ("FP32_SHIFTED", 13, 0),
                                               ("FP16_SHIFTED", 11, 1), ("FP16_SHIFTED", 13, 0),
-                                              ("FP32_SHIFTED", 22, 1), ("FP32_SHIFTED", 24, 0)])
+                                              ("FP32_SHIFTED", 22, 1), ("FP32_SHIFTED", 24, 0),
+                                              ("FP32_SHIFTED", 42, 1), ("FP32_SHIFTED", 42, 0)])
 def test_two_step_strip_equals_one_step(gpu, monkeypatch, prec, shape, remap):
     """The two- and three-step strip walks keep the intermediate populations
     in LDS in the storage type, so their results equal the one-step shifted
@@ -135,10 +136,12 @@ def test_two_step_strip_equals_one_step(gpu, monkeypatch, prec, shape, remap):
             assert fused.kinetic_energy(gpu.POST_COLLISION) == ref.kinetic_energy(gpu.POST_COLLISION)
 
 
-@pytest.mark.parametrize("shape,dims", [(22, (768, 64)), (24, (384, 96))])
+@pytest.mark.parametrize("shape,dims", [(22, (768, 64)), (24, (384, 96)), (42, (1000, 40)), (42, (2048, 33)),
+                                        (42, (16, 16)), (42, (253, 17))])
 def test_three_step_fp32_non_power_of_two(gpu, orc, monkeypatch, shape, dims):
-    """The FP32 three-step walk on sizes that are no power of two: bitwise
-    the one-step FP32 path."""
+    """The FP32 three-step walks on sizes that are no power of two (42: the
+    lane-aligned walk on strips of any width, down to 16 x 16): bitwise the
+    one-step FP32 path."""
     nx, ny = dims
     rng = np.random.default_rng(nx)
     feq = orc.dist_eq(9, 0.03, -0.02, 0.0, 1.0)
