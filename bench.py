@@ -40,6 +40,13 @@ TAU = 0.5 + 3.0 * U0 * L / RE
 TAU_NH = 0.65
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_NODE = {"fp64": 144, "fp32": 72, "fp16": 36}
+# FP64 vector instructions of one D2Q9 E-RLBM collision in the walk kernels'
+# code (gfx950 ISA of the walking loop, per three collisions: 318 v_add_f64,
+# 177 v_mul_f64, and 39 for three IEEE divisions and the rho == 0 tests),
+# and the FP64 vector issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+# (the 78.6 TFLOP/s FP64 vector figure counts an FMA as two)
+FP64_INSTR_PER_NODE_STEP = 178
+FP64_VALU_PEAK_T = 39.3
 # rocprofv3 PMC traffic per launch (FETCH_SIZE + WRITE_SIZE, calibrated against
 # a plain copy kernel), measured on the FP64 step kernel
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "dsl2048_d2q9_fp64_traffic.json")
@@ -364,6 +371,13 @@ def main() -> int:
                      # the multi-step kernels trade HBM bytes for FP64 VALU work: the
                      # profiled VALU busy fraction of the same kernel (rocprofv3)
                      "valu_busy_profiled": valu_busy},
+        # the compute side of the same kernel: the collisions' FP64 vector
+        # instructions per second (algorithmic, halo recomputation excluded)
+        # against the FP64 vector issue peak
+        "fp64_valu": {"instr_per_node_step": FP64_INSTR_PER_NODE_STEP,
+                      "achieved_T_per_s": round(head["MLUPS"] * 1e6 * FP64_INSTR_PER_NODE_STEP / 1e12, 2),
+                      "peak_T_per_s": FP64_VALU_PEAK_T,
+                      "frac": round(head["MLUPS"] * 1e6 * FP64_INSTR_PER_NODE_STEP / 1e12 / FP64_VALU_PEAK_T, 4)},
         "cpu_baseline": cpu,
         "copy_kernel_GBs": round(copy_gbs, 1),
         "mass_rel_error": head["mass_rel_error"],

@@ -483,11 +483,12 @@ def test_default_shape_rule(gpu):
             assert got[2].startswith("three steps" if spl == 3 else "two steps")
 
 
-@pytest.mark.parametrize("shape", [11, 26])
+@pytest.mark.parametrize("shape", [11, 26, 42])
 def test_multi_step_kernel_fma_tolerance_1000_steps(gpu, monkeypatch, shape):
     """The 1e-10 tolerance on the two- and three-step strip walks: max
     relative population error against the FMA-contracted oracle after 1000
-    steps (D2Q9 double shear, L = 512, 256-wide strips)."""
+    steps (D2Q9 double shear, L = 512; 256-wide strips, 42: lane-aligned
+    strips of any width)."""
     import oracle
     monkeypatch.setenv("ERLBM_FUSE", str(shape))
     ofma = oracle.Oracle(fma=True)
