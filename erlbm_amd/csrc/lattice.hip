@@ -1712,7 +1712,7 @@ __global__ __launch_bounds__(64 * NW) void k_step3_astrip(StepArgs a, int remap,
     extern __shared__ double rings[];   // ring 1 [XRING_ROWS][E], ring 2 [XRING_ROWS][E]
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
-    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (remap) b = xcd_runs(b, nb);   // (L = 2048: 1,017 workgroups; 88.6k vs 87.9k MLUPS without)
     const int s = (int)(b % (unsigned)nstrip);
     const int units = a.g.nx / unit;
     const int x0 = (int)((long long)s * units / nstrip) * unit;

@@ -522,7 +522,7 @@ __global__ __launch_bounds__(64 * NW) void k_lp_step3_astrip(LPArgs a, int remap
     extern __shared__ float lp_rings[];   // ring 1 [XRING_ROWS][E], ring 2 [XRING_ROWS][E]
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
-    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (remap) b = xcd_runs(b, nb);
     const int s = (int)(b % (unsigned)nstrip);
     const int units = a.g.nx / unit;
     const int x0 = (int)((long long)s * units / nstrip) * unit;

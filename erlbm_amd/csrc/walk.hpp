@@ -129,6 +129,16 @@ __host__ __device__ constexpr int zring_base(int i)
 constexpr int ZRING_PLANES = zring_base<1>(18) + zring_depth<1>(18);
 static_assert(ZRING_PLANES == 24 && zring_base<-1>(18) + zring_depth<-1>(18) == ZRING_PLANES, "D3Q19 ring planes");
 
+// Workgroup b of nb, dispatched to XCD b mod 8, renumbered so that each
+// XCD's workgroups take a contiguous run of the logical numbers (the first
+// nb mod 8 XCDs one more than the others): neighbouring strips and segments
+// then share an L2.  A bijection of 0 .. nb-1 for any nb.
+__device__ __forceinline__ unsigned xcd_runs(unsigned b, unsigned nb)
+{
+    const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // N dropped stores of T (see nine_dropped_stores).
 template <typename T, int N>
 __device__ __forceinline__ void dropped_stores(__amdgpu_buffer_rsrc_t r)
