@@ -1139,6 +1139,7 @@ extern "C" int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *ma
     h->fuse_requested = getenv("ERLBM_FUSE") != nullptr;
     h->tune_fuse = (int)env_ll("ERLBM_FUSE", h->tune_fuse);
     h->tune_fuse_remap = (int)env_ll("ERLBM_FUSE_REMAP", h->tune_fuse_remap);
+    h->tune_fuse_slots = (int)env_ll("ERLBM_FUSE_SLOTS", h->tune_fuse_slots);
     if (h->tune_pad < 0 || h->tune_pad > (1 << 24)) h->tune_pad = 0;
     h->layout = (int)env_ll("ERLBM_LAYOUT", h->layout);
     const long long pad = (h->tune_pad + 63) / 64 * 64;
@@ -1755,6 +1756,7 @@ int gstrip3_per_cu()
 // workgroup per resident slot, at least 16 rows each.
 void strip3_grid(const erlbm_lattice *h, int tx, bool any_width, int per_cu, int &nstrip, int &nseg, int unit = 1)
 {
+    if (h->tune_fuse_slots > 0) per_cu = std::min(per_cu, h->tune_fuse_slots);
     nstrip = any_width ? (h->nx / unit + tx / unit - 1) / (tx / unit) : h->nx / tx;
     nseg = (int)std::max(1ll, std::min<long long>(h->ny / 16, (long long)h->n_cu * per_cu / nstrip));
 }
