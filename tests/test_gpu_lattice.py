@@ -481,6 +481,16 @@ def test_default_shape_rule(gpu):
             print(f"L={L}: {got}", flush=True)
             assert got[:2] == (shape, spl), L
             assert got[2].startswith("three steps" if spl == 3 else "two steps")
+    # shifted storage: FP32 takes the lane-aligned walk, FP16 one step per launch
+    for prec, want_shape in ((gpu.FP32_SHIFTED, (42, 3)), (gpu.FP16_SHIFTED, (0, 1))):
+        with gpu.Lattice(9, 2048, 2048, 1, precision=prec) as lat:
+            lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
+            assert lat.step_shape()[:2] == want_shape, prec
+    # a stop test since the upload: one step per launch
+    with gpu.Lattice(9, 2048, 2048, 1) as lat:
+        lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
+        lat.set_density_check(10.0)
+        assert lat.step_shape()[:2] == (0, 1)
 
 
 @pytest.mark.parametrize("shape", [11, 26, 42])
