@@ -156,7 +156,7 @@ __global__ __launch_bounds__(BS) void k_step_periodic(StepArgs a, int remap)
     using V = VSet<Q>;
     const unsigned nbx = gridDim.x, nb = gridDim.x * gridDim.y;
     unsigned b = blockIdx.y * nbx + blockIdx.x;
-    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (remap) b = xcd_runs(b, nb);
     const int bx = (int)(b % nbx);
     const int row = (int)(b / nbx);
     const int y = row % a.g.ny;
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(BLOCK) void k_step2_d2q9(StepArgs a, int remap)
     // takes a contiguous run of tiles, so tiles sharing halo lines share an L2
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
-    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (remap) b = xcd_runs(b, nb);
     const int ntx = a.g.nx / TX;
     const int x0 = (int)(b % (unsigned)ntx) * TX;
     const int y0 = (int)(b / (unsigned)ntx) * TY;
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
     extern __shared__ double ring[];   // [RING_ROWS][EX]
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
-    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (remap) b = xcd_runs(b, nb);
     const int nstrip = a.g.nx / TX;
     const int x0 = (int)(b % (unsigned)nstrip) * TX;
     const int ya = (int)(b / (unsigned)nstrip) * SEG;
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step3_strip(StepArgs a,
     extern __shared__ double rings[];   // ring 1 [ROWS][TX+4], ring 2 [ROWS][TX+2]
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
-    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (remap) b = xcd_runs(b, nb);
     const int nstrip = a.g.nx / TX;
     const int x0 = (int)(b % (unsigned)nstrip) * TX;
     const int seg = (int)(b / (unsigned)nstrip);
@@ -525,7 +525,7 @@ __global__ __launch_bounds__(64 * NW) void k_step3_gstrip(StepArgs a, int remap,
     extern __shared__ double rings[];   // ring 1 [LEAN_ROWS][TX+4], ring 2 [LEAN_ROWS][TX+2]
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
-    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (remap) b = xcd_runs(b, nb);
     const int s = (int)(b % (unsigned)nstrip);
     const int x0 = (int)((long long)s * a.g.nx / nstrip);
     const int w = (int)((long long)(s + 1) * a.g.nx / nstrip) - x0;
@@ -638,7 +638,7 @@ __global__ __launch_bounds__((zwalk_threads<TX, TY>())) void k_step2_zwalk(StepA
     extern __shared__ double zring[];   // [ZRING_PLANES][(TX+2) (TY+2)]
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
-    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (remap) b = xcd_runs(b, nb);
     const int tx = a.g.nx / TX, ntile = tx * (a.g.ny / TY);
     const int tile = (int)(b % (unsigned)ntile), seg = (int)(b / (unsigned)ntile);
     const int za = (int)((long long)seg * a.g.nz / nseg);
@@ -760,7 +760,7 @@ void k_step2_wave(StepArgs a, int nstrip, int nseg, int remap)
     const int ny = a.g.ny;
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
-    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (remap) b = xcd_runs(b, nb);
     // the wavefront index is uniform across the wavefront; readfirstlane
     // lets the compiler keep everything derived from it in SGPRs
     const int wv = (int)b * WPB + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -787,7 +787,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_d2q9_pair(StepArgs a, int remap)
     if (stopped(a.status, a.step)) return;
     const unsigned nbx = gridDim.x, nb = gridDim.x * gridDim.y;
     unsigned b = blockIdx.y * nbx + blockIdx.x;
-    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (remap) b = xcd_runs(b, nb);
     const int bx = (int)(b % nbx);
     const int y = (int)(b / nbx);
     const int x0 = 2 * (bx * BLOCK + (int)threadIdx.x);

@@ -272,7 +272,7 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_lp_step2_strip(LPArgs a
     T *__restrict__ dst = static_cast<T *>(a.dst);
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
-    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (remap) b = xcd_runs(b, nb);
     const int nstrip = a.g.nx / TX;
     const int x0 = (int)(b % (unsigned)nstrip) * TX;
     const int ya = (int)(b / (unsigned)nstrip) * SEG;
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_lp_step3_strip(LPArgs a
     extern __shared__ float lp_rings[];   // ring 1 [RING_ROWS][TX+4], ring 2 [RING_ROWS][TX+2]
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
-    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (remap) b = xcd_runs(b, nb);
     const int nstrip = a.g.nx / TX;
     const int x0 = (int)(b % (unsigned)nstrip) * TX;
     const int seg = (int)(b / (unsigned)nstrip);
