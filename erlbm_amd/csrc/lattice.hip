@@ -1140,6 +1140,7 @@ extern "C" int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *ma
     h->tune_fuse = (int)env_ll("ERLBM_FUSE", h->tune_fuse);
     h->tune_fuse_remap = (int)env_ll("ERLBM_FUSE_REMAP", h->tune_fuse_remap);
     h->tune_fuse_slots = (int)env_ll("ERLBM_FUSE_SLOTS", h->tune_fuse_slots);
+    h->tune_fuse_prio = (int)env_ll("ERLBM_FUSE_PRIO", h->tune_fuse_prio);
     if (h->tune_pad < 0 || h->tune_pad > (1 << 24)) h->tune_pad = 0;
     h->layout = (int)env_ll("ERLBM_LAYOUT", h->layout);
     const long long pad = (h->tune_pad + 63) / 64 * 64;
@@ -1603,7 +1604,7 @@ void launch_strip3_t(erlbm_lattice *h, const StepArgs &a)
 // writes and walking directions as in strip3_walk.
 template <int COL, int NW, int DIR>
 __device__ __forceinline__ void strip3_walk_aligned(const StepArgs &a, int x0, int w, int ya, int rows, double *ring1,
-                                                    double *ring2)
+                                                    double *ring2, bool prio)
 {
     using V = VSet<9>;
     constexpr int E = 64 * NW;
@@ -1641,6 +1642,7 @@ __device__ __forceinline__ void strip3_walk_aligned(const StepArgs &a, int x0, i
     if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
     nine_dropped_stores<double>(rd);
     for (int n = 0; n < rows + 4; n++) {
+        if (prio) progress_prio(n, rows + 4);
         const int r = r0 + DIR * n;
         double f[9];
         static_for<9>([&](auto I) { f[(int)I] = fq[(int)I]; });
@@ -1707,7 +1709,8 @@ __device__ __forceinline__ void strip3_walk_aligned(const StepArgs &a, int x0, i
 
 // Strip edges on multiples of `unit` positions (16: 128-byte lines).
 template <int COL, int NW>
-__global__ __launch_bounds__(64 * NW) void k_step3_astrip(StepArgs a, int remap, int nstrip, int nseg, int unit)
+__global__ __launch_bounds__(64 * NW) void k_step3_astrip(StepArgs a, int remap, int nstrip, int nseg, int unit,
+                                                         int prio)
 {
     constexpr int E = 64 * NW;
     extern __shared__ double rings[];   // ring 1 [XRING_ROWS][E], ring 2 [XRING_ROWS][E]
@@ -1721,8 +1724,8 @@ __global__ __launch_bounds__(64 * NW) void k_step3_astrip(StepArgs a, int remap,
     const int seg = (int)(b / (unsigned)nstrip);
     const int ya = (int)((long long)seg * a.g.ny / nseg);
     const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
-    if (seg & 1) strip3_walk_aligned<COL, NW, -1>(a, x0, w, ya, rows, rings, rings + XRING_ROWS * E);
-    else strip3_walk_aligned<COL, NW, 1>(a, x0, w, ya, rows, rings, rings + XRING_ROWS * E);
+    if (seg & 1) strip3_walk_aligned<COL, NW, -1>(a, x0, w, ya, rows, rings, rings + XRING_ROWS * E, prio != 0);
+    else strip3_walk_aligned<COL, NW, 1>(a, x0, w, ya, rows, rings, rings + XRING_ROWS * E, prio != 0);
 }
 
 // Strips of at most 64 NW - 4 positions, as few as the width allows;
@@ -1754,6 +1757,14 @@ int gstrip3_per_cu()
 // positions (any width: as few strips as the width allows; else tx-wide
 // strips) with per_cu workgroups fitting a CU: as many segments as make one
 // workgroup per resident slot, at least 16 rows each.
+// Whether the aligned walks set their issue priority by progress: where
+// two or more workgroups share a CU (ERLBM_FUSE_PRIO: -1 so, 0 never, 1
+// always).
+int walk_prio(const erlbm_lattice *h, long long workgroups)
+{
+    return h->tune_fuse_prio >= 0 ? h->tune_fuse_prio : (workgroups >= 2ll * h->n_cu ? 1 : 0);
+}
+
 void strip3_grid(const erlbm_lattice *h, int tx, bool any_width, int per_cu, int &nstrip, int &nseg, int unit = 1)
 {
     if (h->tune_fuse_slots > 0) per_cu = std::min(per_cu, h->tune_fuse_slots);
@@ -1798,8 +1809,8 @@ void launch_gstrip3_t(erlbm_lattice *h, const StepArgs &a)
     int nstrip, nseg;
     strip3_grid(h, 64 * NW - 4, true, gstrip3_per_cu<COL, NW, ALIGNED>(), nstrip, nseg, unit);
     if constexpr (ALIGNED)
-        hipLaunchKernelGGL((k_step3_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream,
-                           a, h->tune_fuse_remap, nstrip, nseg, unit);
+        hipLaunchKernelGGL((k_step3_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds,
+                           h->stream, a, h->tune_fuse_remap, nstrip, nseg, unit, walk_prio(h, nstrip * nseg));
     else
         hipLaunchKernelGGL((k_step3_gstrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream,
                            a, h->tune_fuse_remap, nstrip, nseg);

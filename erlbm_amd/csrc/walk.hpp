@@ -139,6 +139,20 @@ __device__ __forceinline__ unsigned xcd_runs(unsigned b, unsigned nb)
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// Issue priority by progress (walking step n of steps): the SIMD arbiter
+// prefers the oldest ready wave, so of the workgroups sharing a CU the first
+// started would run ahead and the last finish alone, latency-bound;
+// lowering a walk's priority a level per quarter of its steps lets the ones
+// behind catch up.
+__device__ __forceinline__ void progress_prio(int n, int steps)
+{
+    const int q = 4 * n / steps;
+    if (q == 0) __builtin_amdgcn_s_setprio(3);
+    else if (q == 1) __builtin_amdgcn_s_setprio(2);
+    else if (q == 2) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 // N dropped stores of T (see nine_dropped_stores).
 template <typename T, int N>
 __device__ __forceinline__ void dropped_stores(__amdgpu_buffer_rsrc_t r)

@@ -467,6 +467,25 @@ def test_headline_default_kernel_full_size(gpu, orc):
     assert abs(got.sum() - f0.sum()) <= 1e-12 * f0.sum()
 
 
+@pytest.mark.parametrize("prio", ["0", "1"])
+def test_aligned_walk_issue_priority_bitwise(gpu, orc, monkeypatch, prio):
+    """The lane-aligned walk with its issue priority by progress forced on
+    and off (by default it is on where two or more workgroups share a CU):
+    scheduling only, so bitwise the oracle either way."""
+    nx, ny = 1000, 40
+    rng = np.random.default_rng(5)
+    ini, st, nf = orc.stream_map(9, nx, ny, 1)
+    feq = orc.dist_eq(9, 0.03, -0.02, 0.0, 1.0)
+    f0 = feq[None] * (1 + 1e-2 * rng.uniform(-1, 1, (nf, 9)))
+    monkeypatch.setenv("ERLBM_FUSE", "42")
+    monkeypatch.setenv("ERLBM_FUSE_PRIO", prio)
+    with gpu.Lattice(9, nx, ny, 1) as lat:
+        lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
+        lat.upload(f0)
+        lat.step(10)
+        assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, 10, O.COL_EXT_REG, 0.51, 0.65, nthreads=16))
+
+
 def test_default_shape_rule(gpu):
     """The default multi-step shape per lattice size (periodic FP64 D2Q9
     E-RLBM; gfx950 parts have 256 CUs): the lane-aligned any-width walk

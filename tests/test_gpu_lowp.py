@@ -137,7 +137,7 @@ def test_two_step_strip_equals_one_step(gpu, monkeypatch, prec, shape, remap):
 
 
 @pytest.mark.parametrize("shape,dims", [(22, (768, 64)), (24, (384, 96)), (42, (1000, 40)), (42, (2048, 33)),
-                                        (42, (16, 16)), (42, (253, 17))])
+                                        (42, (16, 16)), (42, (253, 17)), ("42p", (1000, 40))])
 def test_three_step_fp32_non_power_of_two(gpu, orc, monkeypatch, shape, dims):
     """The FP32 three-step walks on sizes that are no power of two (42: the
     lane-aligned walk on strips of any width, down to 16 x 16): bitwise the
@@ -147,6 +147,9 @@ def test_three_step_fp32_non_power_of_two(gpu, orc, monkeypatch, shape, dims):
     feq = orc.dist_eq(9, 0.03, -0.02, 0.0, 1.0)
     f0 = feq[None] * (1 + 1e-2 * rng.uniform(-1, 1, (nx * ny, 9)))
     out = []
+    if shape == "42p":   # the aligned walk with its issue priority forced on
+        shape = 42
+        monkeypatch.setenv("ERLBM_FUSE_PRIO", "1")
     for fuse in ("0", str(shape)):
         monkeypatch.setenv("ERLBM_FUSE", fuse)
         with gpu.Lattice(9, nx, ny, 1, precision=gpu.FP32_SHIFTED) as lat:
