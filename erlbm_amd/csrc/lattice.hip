@@ -2016,7 +2016,11 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
             {FUSE3_GSTRIP_4W, {"three steps, strips of any width (4 wavefronts)", "three steps, strips of any width (4 wavefronts), XCD runs"}},
             {FUSE3_GSTRIP_3W, {"three steps, strips of any width (3 wavefronts)", "three steps, strips of any width (3 wavefronts), XCD runs"}},
             {FUSE3_ASTRIP_4W, {"three steps, lane-aligned strips of any width (4 wavefronts)",
-                               "three steps, lane-aligned strips of any width (4 wavefronts), XCD runs"}}};
+                               "three steps, lane-aligned strips of any width (4 wavefronts), XCD runs"}},
+            {FUSE3_ASTRIP_3W, {"three steps, lane-aligned strips of any width (3 wavefronts)",
+                               "three steps, lane-aligned strips of any width (3 wavefronts), XCD runs"}},
+            {FUSE3_ASTRIP_2W, {"three steps, lane-aligned strips of any width (2 wavefronts)",
+                               "three steps, lane-aligned strips of any width (2 wavefronts), XCD runs"}}};
         for (const Named &nm : fused) {
             if (!fuse_geometry_fits(h, FUSE_SHAPES[nm.f])) continue;
             for (int rm = 0; rm < 2; rm++) {
