@@ -136,6 +136,30 @@ def test_two_step_strip_equals_one_step(gpu, monkeypatch, prec, shape, remap):
             assert fused.kinetic_energy(gpu.POST_COLLISION) == ref.kinetic_energy(gpu.POST_COLLISION)
 
 
+@pytest.mark.parametrize("col", [O.COL_BGK, O.COL_MRT, O.COL_REG, O.COL_EXT_REG])
+def test_aligned_walk_fp32_every_collision(gpu, orc, monkeypatch, col):
+    """The FP32 lane-aligned three-step walk for every collision model on a
+    ragged lattice (1000 x 40, issue priority forced on): bitwise the
+    one-step FP32 path after 1 + 3 + 7 steps."""
+    nx, ny = 1000, 40
+    rng = np.random.default_rng(col)
+    feq = orc.dist_eq(9, 0.03, -0.02, 0.0, 1.0)
+    f0 = feq[None] * (1 + 1e-2 * rng.uniform(-1, 1, (nx * ny, 9)))
+    tnh = {O.COL_BGK: 0.52, O.COL_MRT: 0.65, O.COL_REG: 1.0, O.COL_EXT_REG: 0.65}[col]
+    monkeypatch.setenv("ERLBM_FUSE_PRIO", "1")
+    out = []
+    for fuse in ("0", "42"):
+        monkeypatch.setenv("ERLBM_FUSE", fuse)
+        with gpu.Lattice(9, nx, ny, 1, precision=gpu.FP32_SHIFTED) as lat:
+            lat.set_params(0.52, tnh, 1.0, col)
+            lat.upload(f0)
+            for n in (1, 3, 7):
+                lat.step(n)
+            assert lat.status() == (11, 0)
+            out.append(lat.download())
+    assert np.array_equal(out[0], out[1])
+
+
 @pytest.mark.parametrize("shape,dims", [(22, (768, 64)), (24, (384, 96)), (42, (1000, 40)), (42, (2048, 33)),
                                         (42, (16, 16)), (42, (253, 17)), ("42p", (1000, 40))])
 def test_three_step_fp32_non_power_of_two(gpu, orc, monkeypatch, shape, dims):
