@@ -1604,7 +1604,7 @@ void launch_strip3_t(erlbm_lattice *h, const StepArgs &a)
 // writes and walking directions as in strip3_walk.
 template <int COL, int NW, int DIR>
 __device__ __forceinline__ void strip3_walk_aligned(const StepArgs &a, int x0, int w, int ya, int rows, double *ring1,
-                                                    double *ring2, bool prio)
+                                                    double *ring2, int prio)
 {
     using V = VSet<9>;
     constexpr int E = 64 * NW;
@@ -1642,7 +1642,7 @@ __device__ __forceinline__ void strip3_walk_aligned(const StepArgs &a, int x0, i
     if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
     nine_dropped_stores<double>(rd);
     for (int n = 0; n < rows + 4; n++) {
-        if (prio) progress_prio(n, rows + 4);
+        if (prio) progress_prio(n, rows + 4, prio);
         const int r = r0 + DIR * n;
         double f[9];
         static_for<9>([&](auto I) { f[(int)I] = fq[(int)I]; });
@@ -1724,8 +1724,8 @@ __global__ __launch_bounds__(64 * NW) void k_step3_astrip(StepArgs a, int remap,
     const int seg = (int)(b / (unsigned)nstrip);
     const int ya = (int)((long long)seg * a.g.ny / nseg);
     const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
-    if (seg & 1) strip3_walk_aligned<COL, NW, -1>(a, x0, w, ya, rows, rings, rings + XRING_ROWS * E, prio != 0);
-    else strip3_walk_aligned<COL, NW, 1>(a, x0, w, ya, rows, rings, rings + XRING_ROWS * E, prio != 0);
+    if (seg & 1) strip3_walk_aligned<COL, NW, -1>(a, x0, w, ya, rows, rings, rings + XRING_ROWS * E, prio);
+    else strip3_walk_aligned<COL, NW, 1>(a, x0, w, ya, rows, rings, rings + XRING_ROWS * E, prio);
 }
 
 // Strips of at most 64 NW - 4 positions, as few as the width allows;
@@ -1757,12 +1757,14 @@ int gstrip3_per_cu()
 // positions (any width: as few strips as the width allows; else tx-wide
 // strips) with per_cu workgroups fitting a CU: as many segments as make one
 // workgroup per resident slot, at least 16 rows each.
-// Whether the aligned walks set their issue priority by progress: where
-// two or more workgroups share a CU (ERLBM_FUSE_PRIO: -1 so, 0 never, 1
-// always).
+// Whether and how the aligned walks set their issue priority by progress
+// (progress_prio's scheme, 0 = not at all): where two or more workgroups
+// share a CU (ERLBM_FUSE_PRIO: -1 so, 0 never, 1 equal quarters, 2 the
+// 50 / 75 / 90% bands, the default; L = 2048 FP64 91.8-92.2k MLUPS against
+// 90.8-91.0k with quarters, FP32 +0.9%).
 int walk_prio(const erlbm_lattice *h, long long workgroups)
 {
-    return h->tune_fuse_prio >= 0 ? h->tune_fuse_prio : (workgroups >= 2ll * h->n_cu ? 1 : 0);
+    return h->tune_fuse_prio >= 0 ? h->tune_fuse_prio : (workgroups >= 2ll * h->n_cu ? 2 : 0);
 }
 
 void strip3_grid(const erlbm_lattice *h, int tx, bool any_width, int per_cu, int &nstrip, int &nseg, int unit = 1)

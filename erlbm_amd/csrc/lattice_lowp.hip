@@ -417,7 +417,7 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_lp_step3_strip(LPArgs a
 // store and read back.
 template <int COL, int NW, int DIR>
 __device__ __forceinline__ void lp_strip3_walk_aligned(const LPArgs &a, int x0, int w, int ya, int rows, float *ring1,
-                                                       float *ring2, bool prio)
+                                                       float *ring2, int prio)
 {
     using V = VSet<9>;
     constexpr int E = 64 * NW;
@@ -451,7 +451,7 @@ __device__ __forceinline__ void lp_strip3_walk_aligned(const LPArgs &a, int x0, 
     if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
     nine_dropped_stores<float>(rd);
     for (int n = 0; n < rows + 4; n++) {
-        if (prio) progress_prio(n, rows + 4);
+        if (prio) progress_prio(n, rows + 4, prio);
         const int r = r0 + DIR * n;
         float f[9];
         static_for<9>([&](auto I) { f[(int)I] = fq[(int)I]; });
@@ -532,8 +532,8 @@ __global__ __launch_bounds__(64 * NW) void k_lp_step3_astrip(LPArgs a, int remap
     const int seg = (int)(b / (unsigned)nstrip);
     const int ya = (int)((long long)seg * a.g.ny / nseg);
     const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
-    if (seg & 1) lp_strip3_walk_aligned<COL, NW, -1>(a, x0, w, ya, rows, lp_rings, lp_rings + XRING_ROWS * E, prio != 0);
-    else lp_strip3_walk_aligned<COL, NW, 1>(a, x0, w, ya, rows, lp_rings, lp_rings + XRING_ROWS * E, prio != 0);
+    if (seg & 1) lp_strip3_walk_aligned<COL, NW, -1>(a, x0, w, ya, rows, lp_rings, lp_rings + XRING_ROWS * E, prio);
+    else lp_strip3_walk_aligned<COL, NW, 1>(a, x0, w, ya, rows, lp_rings, lp_rings + XRING_ROWS * E, prio);
 }
 
 // Strips of at most 252 positions, edges on 16-position units where the
@@ -556,7 +556,7 @@ void launch_lp_astrip(erlbm_lattice *h, const LPArgs &l)
     const int nstrip = (h->nx / unit + (64 * NW - 4) / unit - 1) / ((64 * NW - 4) / unit);
     const int nseg = (int)std::max(1ll, std::min<long long>(h->ny / 16, (long long)h->n_cu * per_cu / nstrip));
     // issue priority by progress where two or more workgroups share a CU
-    const int prio = h->tune_fuse_prio >= 0 ? h->tune_fuse_prio : (nstrip * nseg >= 2 * h->n_cu ? 1 : 0);
+    const int prio = h->tune_fuse_prio >= 0 ? h->tune_fuse_prio : (nstrip * nseg >= 2 * h->n_cu ? 2 : 0);
     hipLaunchKernelGGL((k_lp_step3_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream, l,
                        h->tune_fuse_remap, nstrip, nseg, unit, prio);
 }

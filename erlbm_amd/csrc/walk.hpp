@@ -142,11 +142,14 @@ __device__ __forceinline__ unsigned xcd_runs(unsigned b, unsigned nb)
 // Issue priority by progress (walking step n of steps): the SIMD arbiter
 // prefers the oldest ready wave, so of the workgroups sharing a CU the first
 // started would run ahead and the last finish alone, latency-bound;
-// lowering a walk's priority a level per quarter of its steps lets the ones
-// behind catch up.
-__device__ __forceinline__ void progress_prio(int n, int steps)
+// lowering a walk's priority a level per band of its steps lets the ones
+// behind catch up.  Scheme 2: bands ending at 50, 75 and 90% of the walk
+// (what matters is how far apart the workgroups are when the first ends);
+// 1: equal quarters.
+__device__ __forceinline__ void progress_prio(int n, int steps, int scheme)
 {
-    const int q = 4 * n / steps;
+    const int q = scheme == 2 ? (20 * n < 10 * steps ? 0 : (20 * n < 15 * steps ? 1 : (20 * n < 18 * steps ? 2 : 3)))
+                              : 4 * n / steps;
     if (q == 0) __builtin_amdgcn_s_setprio(3);
     else if (q == 1) __builtin_amdgcn_s_setprio(2);
     else if (q == 2) __builtin_amdgcn_s_setprio(1);

@@ -467,7 +467,7 @@ def test_headline_default_kernel_full_size(gpu, orc):
     assert abs(got.sum() - f0.sum()) <= 1e-12 * f0.sum()
 
 
-@pytest.mark.parametrize("prio", ["0", "1"])
+@pytest.mark.parametrize("prio", ["0", "1", "2"])
 def test_aligned_walk_issue_priority_bitwise(gpu, orc, monkeypatch, prio):
     """The lane-aligned walk with its issue priority by progress forced on
     and off (by default it is on where two or more workgroups share a CU):
