@@ -48,6 +48,19 @@ def test_device_count_without_gpu_is_not_an_error():
     assert erlbm_amd.device_count() >= 0
 
 
+def test_null_handles_are_argument_errors():
+    """Calls on a null handle fail with ERLBM_ERR_ARG before any HIP call
+    (so this runs without a GPU)."""
+    import ctypes as C
+    import erlbm_amd
+    from erlbm_amd._lib import ERLBM_ERR_ARG
+    lib = erlbm_amd.lib()
+    f, spl = C.c_int(-1), C.c_int(-1)
+    assert lib.erlbm_step_shape(None, C.byref(f), C.byref(spl), None, 0) == ERLBM_ERR_ARG
+    assert (f.value, spl.value) == (-1, -1)
+    assert lib.erlbm_step(None, 1, 0, 0.0) == ERLBM_ERR_ARG
+
+
 def test_host_initial_conditions_bitwise(orc):
     import erlbm_amd as E
     for L, u0, tau, poisson, neq in ((64, 2 / 64, 0.62, True, True), (32, 1e-3, 0.8, True, False),
