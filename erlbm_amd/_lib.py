@@ -82,6 +82,8 @@ SIGNATURES = {
     "erlbm_set_zouhe_inlet": (_i, [_vp, _i, _d]),
     "erlbm_set_outlet_rescale": (_i, [_vp, _i, _d]),
     "erlbm_set_density_check": (_i, [_vp, _d]),
+    "erlbm_set_body_force": (_i, [_vp, _d, _d, _d]),
+    "erlbm_autotune_candidates": (_i, [_i, _i, _i, _i, _i, _i, _ip, _i, C.POINTER(_i)]),
     "erlbm_upload_aos": (_i, [_vp, _dp]),
     "erlbm_download_aos": (_i, [_vp, _i, _dp]),
     "erlbm_step": (_i, [_vp, _i, _i, _d]),

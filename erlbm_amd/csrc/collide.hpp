@@ -13,7 +13,11 @@
 //  * opposite directions share cv*3 and ((0.5 cv) cv 3) 3, which are odd and
 //    even in cv under IEEE round-to-nearest;
 //  * the forcing term S (source(), LBM_functions.cpp:5645-5716) is +-0 at the
-//    zero body force every driver passes, so it is not added.
+//    zero body force every driver passes, so the unforced operator (FORCE =
+//    false) does not add it; with FORCE the equilibrium takes u + a/2, F = a rho
+//    and S is added last, as coll_BGK / coll_Reg / coll_Ext_Reg do
+//    (Collision_operators.cpp:24-49, 866-921, 1196-1254), and coll_MRT's own
+//    forcing moments (:647-690) are formed.
 // Non-finite states stay non-finite (possibly inf where the reference has
 // NaN), which every stability test treats alike.
 #pragma once
@@ -28,6 +32,14 @@ struct CollideConsts {
     double omega1;   // 1 - 1/tau   (coll_Ext_Reg :1192; coll_Reg's (1 - 1/tau))
     double omega2;   // 1 - 1/tau_nh (coll_Ext_Reg :1194)
     double oot;      // 1/tau        (coll_BGK :24)
+};
+
+// Body-force acceleration of colisao_sitio's acc_x / acc_y / acc_z arguments
+// (src/erlbm_common.h:68-81) and source()'s factor 1 - 1/(2 tau)
+// (LBM_functions.cpp:5664), computed on the host in that order.
+struct BodyForce {
+    double ax, ay, az;
+    double factor;
 };
 
 // rho, u as calcula() computes them (sums in velocity order from 0.0,
@@ -104,6 +116,34 @@ __device__ __forceinline__ void feq_exact(double rho, double vx, double vy, doub
     });
 }
 
+// source() (LBM_functions.cpp:5645-5716) at the velocity u the caller gives:
+// F = a rho, S_i = ((PRE w_i) 3) ((cF + (3 cF) cv) - vF) with vF = F.u summed
+// from 0.0 (dot_product, :2864-2876), cF = c_i.F from 0.0 without the zero-c
+// terms (adding +-0 to a sum that starts at +0.0 leaves it unchanged) and
+// cv = c_i.u as the equilibrium forms it.  PRE = 1 - 1/(2 tau) for the
+// BGK / regularized operators; coll_MRT's own term (Collision_operators.cpp:
+// 647-671) has no factor (MRT = true).
+template <int Q, bool MRT>
+__device__ __forceinline__ void source_exact(double rho, double ux, double uy, double uz, const BodyForce &bf,
+                                             double (&S)[Q])
+{
+    using V = VSet<Q>;
+    const double Fx = bf.ax * rho, Fy = bf.ay * rho, Fz = bf.az * rho;
+    double vF = 0.0 + Fx * ux;
+    vF = vF + Fy * uy;
+    if constexpr (V::D == 3) vF = vF + Fz * uz;
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        double cF = 0.0;
+        if constexpr (V::cx[i] != 0) cF = V::cx[i] > 0 ? cF + Fx : cF - Fx;
+        if constexpr (V::cy[i] != 0) cF = V::cy[i] > 0 ? cF + Fy : cF - Fy;
+        if constexpr (V::D == 3 && V::cz[i] != 0) cF = V::cz[i] > 0 ? cF + Fz : cF - Fz;
+        const double cv = cdotu<Q, i>(ux, uy, uz);
+        const double pre = MRT ? V::w[i] * 3.0 : (bf.factor * V::w[i]) * 3.0;
+        S[i] = pre * ((cF + (3.0 * cF) * cv) - vF);
+    });
+}
+
 // coll_MRT, D2Q9 (Collision_operators.cpp:611-701): moments m = M f and
 // m_eq = M f_eq, m <- m - s (m - m_eq) + (1 - s/2) F_m with F_m = +0 at zero
 // force, f = M^-1 m; M and M^-1 of :449-471, rates of :614-622.  Entries
@@ -128,29 +168,35 @@ struct MrtD2Q9 {
     static constexpr double S_FIXED[9] = {0.0, 1.6, 1.8, 0.0, 1.9, 0.0, 1.9, -1.0, -1.0};   // -1: 1/tau
 };
 
-__device__ __forceinline__ void collide_mrt_d2q9(double (&g)[9], const double (&feq)[9], double oot)
+// FORCE: the forcing moments F_m = M S of coll_MRT's S (:680-690) instead of +0.
+template <bool FORCE = false>
+__device__ __forceinline__ void collide_mrt_d2q9(double (&g)[9], const double (&feq)[9], double oot,
+                                                 const double (&S)[9])
 {
     using T = MrtD2Q9;
     double m[9];
     static_for<9>([&](auto R) {
         constexpr int r = R;
-        double a = 0.0, e = 0.0;
+        double a = 0.0, e = 0.0, fm = 0.0;
         static_for<9>([&](auto Cc) {
             constexpr int c = Cc;
             constexpr int k = T::M[r][c];
             if constexpr (k == 1) {
                 a = a + g[c];
                 e = e + feq[c];
+                if constexpr (FORCE) fm = fm + S[c];
             } else if constexpr (k == -1) {
                 a = a - g[c];
                 e = e - feq[c];
+                if constexpr (FORCE) fm = fm - S[c];
             } else if constexpr (k != 0) {
                 a = a + (double)k * g[c];
                 e = e + (double)k * feq[c];
+                if constexpr (FORCE) fm = fm + (double)k * S[c];
             }
         });
         const double s = T::S_FIXED[r] < 0.0 ? oot : T::S_FIXED[r];
-        m[r] = (a - s * (a - e)) + (1.0 - 0.5 * s) * 0.0;
+        m[r] = (a - s * (a - e)) + (1.0 - 0.5 * s) * fm;
     });
     static_for<9>([&](auto R) {
         constexpr int r = R;
@@ -164,26 +210,40 @@ __device__ __forceinline__ void collide_mrt_d2q9(double (&g)[9], const double (&
 }
 
 // One collision in place.  Returns nothing; g holds f* afterwards.
-// vx, vy, vz receive the pre-collision velocity (for monitors).
-template <int Q, int COL>
+// vx, vy, vz receive the pre-collision velocity (for monitors).  FORCE adds
+// the body force bf (the operators' acc_x, acc_y, acc_z arguments).
+template <int Q, int COL, bool FORCE = false>
 __device__ __forceinline__ void collide_exact(double (&g)[Q], const CollideConsts &k, double &vx,
-                                              double &vy, double &vz)
+                                              double &vy, double &vz, const BodyForce &bf = BodyForce{})
 {
     using V = VSet<Q>;
     double rho;
     moments_exact<Q>(g, rho, vx, vy, vz);
+    // velocity of the equilibrium: u + a/2 under a body force (coll_Ext_Reg
+    // :1198-1200; coll_MRT keeps u, :637-640)
+    double ux = vx, uy = vy, uz = vz;
+    double S[Q];
+    if constexpr (FORCE) {
+        if constexpr (COL != COL_MRT) {
+            ux = vx + 0.5 * bf.ax;
+            uy = vy + 0.5 * bf.ay;
+            uz = vz + 0.5 * bf.az;
+        }
+        source_exact<Q, COL == COL_MRT>(rho, ux, uy, uz, bf, S);
+    }
     double feq[Q];
-    feq_exact<Q>(rho, vx, vy, vz, feq);
+    feq_exact<Q>(rho, ux, uy, uz, feq);
 
     if constexpr (COL == COL_BGK) {
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             g[i] = g[i] + (feq[i] - g[i]) * k.oot;
+            if constexpr (FORCE) g[i] = g[i] + S[i];
         });
         return;
     } else if constexpr (COL == COL_MRT) {
         static_assert(Q == 9, "coll_MRT exists for D2Q9 only");
-        collide_mrt_d2q9(g, feq, k.oot);
+        collide_mrt_d2q9<FORCE>(g, feq, k.oot, S);
         return;
     } else {
         double d[Q];
@@ -237,12 +297,14 @@ __device__ __forceinline__ void collide_exact(double (&g)[Q], const CollideConst
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
                 g[i] = feq[i] + k.omega1 * f1[i];
+                if constexpr (FORCE) g[i] = g[i] + S[i];
             });
         } else {
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
                 const double fgt1 = d[i] - f1[i];
                 g[i] = (feq[i] + k.omega1 * f1[i]) + k.omega2 * fgt1;
+                if constexpr (FORCE) g[i] = g[i] + S[i];
             });
         }
     }

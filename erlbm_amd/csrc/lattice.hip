@@ -110,8 +110,9 @@ __device__ __forceinline__ void apply_bc(const StepArgs &a, int x, int y, int z,
     }
 }
 
-// General kernel: any geometry, boundary columns, 1-D voxel mapping.
-template <int Q, int COL, bool STREAM>
+// General kernel: any geometry, boundary columns, body force, 1-D voxel
+// mapping.
+template <int Q, int COL, bool STREAM, bool FORCE = false>
 __global__ __launch_bounds__(BLOCK) void k_step_general(StepArgs a)
 {
     if (stopped(a.status, a.step)) return;
@@ -129,7 +130,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_general(StepArgs a)
             gather<Q, STREAM, true>(a.src, a.g, nb, x, y, z, f);
             apply_bc<Q, STREAM>(a, x, y, z, f);
             double vx, vy, vz;
-            collide_exact<Q, COL>(f, a.k, vx, vy, vz);
+            collide_exact<Q, COL, FORCE>(f, a.k, vx, vy, vz, a.bf);
             if (a.dens_limit > 0.0) {
                 const double d = density_exact<Q>(f);
                 bad = !(fabs(d) <= a.dens_limit);
@@ -1068,7 +1069,11 @@ void launch_periodic_tuned(erlbm_lattice *h, const StepArgs &a)
 template <int Q, int COL>
 int launch_step_q(erlbm_lattice *h, const StepArgs &a, bool stream)
 {
-    if (h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0) {
+    if (h->forced) {
+        const long long nb = (h->nvox + BLOCK - 1) / BLOCK;
+        if (stream) hipLaunchKernelGGL((k_step_general<Q, COL, true, true>), dim3((unsigned)nb), dim3(BLOCK), 0, h->stream, a);
+        else hipLaunchKernelGGL((k_step_general<Q, COL, false, true>), dim3((unsigned)nb), dim3(BLOCK), 0, h->stream, a);
+    } else if (h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0) {
         if (stream) launch_periodic_tuned<Q, COL, true>(h, a);
         else launch_periodic_tuned<Q, COL, false>(h, a);
     } else {
@@ -1092,6 +1097,48 @@ int launch_step(erlbm_lattice *h, const StepArgs &a, bool stream)
         [[fallthrough]];
     default: set_last_error_msg("collision operator not supported on this path"); return ERLBM_ERR_UNSUPPORTED;
     }
+}
+
+// Size, storage layout and tuning knobs of a handle (environment overrides
+// read here); no device call.
+void set_layout(erlbm_lattice *h, int q, int nx, int ny, int nz, int precision)
+{
+    h->q = q;
+    h->nx = nx;
+    h->ny = ny;
+    h->nz = nz;
+    h->nvox = (long long)nx * ny * nz;
+    auto env_ll = [](const char *name, long long dflt) {
+        const char *v = getenv(name);
+        return (v && *v) ? atoll(v) : dflt;
+    };
+    h->tune_pad = env_ll("ERLBM_PLANE_PAD", h->tune_pad);
+    h->tune_npt = (int)env_ll("ERLBM_NPT", h->tune_npt);
+    h->tune_ntl = (int)env_ll("ERLBM_NT_LOADS", h->tune_ntl);
+    h->tune_remap = (int)env_ll("ERLBM_XCD_REMAP", h->tune_remap);
+    h->tune_pair = (int)env_ll("ERLBM_PAIR", h->tune_pair);
+    h->tune_block = (int)env_ll("ERLBM_BLOCK", h->tune_block);
+    h->tune_vec = (int)env_ll("ERLBM_VEC", h->tune_vec);
+    h->fuse_requested = getenv("ERLBM_FUSE") != nullptr;
+    h->tune_fuse = (int)env_ll("ERLBM_FUSE", h->tune_fuse);
+    h->tune_fuse_remap = (int)env_ll("ERLBM_FUSE_REMAP", h->tune_fuse_remap);
+    h->tune_fuse_slots = (int)env_ll("ERLBM_FUSE_SLOTS", h->tune_fuse_slots);
+    h->tune_fuse_prio = (int)env_ll("ERLBM_FUSE_PRIO", h->tune_fuse_prio);
+    if (h->tune_pad < 0 || h->tune_pad > (1 << 24)) h->tune_pad = 0;
+    h->layout = (int)env_ll("ERLBM_LAYOUT", h->layout);
+    const long long pad = (h->tune_pad + 63) / 64 * 64;
+    const long long nrows = (long long)ny * nz;
+    if (h->layout == 0) {   // population planes
+        h->si = (h->nvox + 63) / 64 * 64 + pad;
+        h->sr = nx;
+    } else {                // row blocks
+        h->si = nx;
+        h->sr = (long long)q * nx + pad;
+    }
+    h->extent = (q - 1) * h->si + (nrows - 1) * h->sr + nx;
+    h->precision = precision;
+    h->elem = precision == ERLBM_FP64 ? 8 : (precision == ERLBM_FP16_SHIFTED ? 2 : 4);
+    h->f32_arith = precision == ERLBM_FP32_SHIFTED_F32;
 }
 
 }  // namespace
@@ -1120,42 +1167,7 @@ extern "C" int erlbm_lattice_create(int q, int nx, int ny, int nz, const int *ma
         return ERLBM_ERR_ARG;
     }
     auto *h = new erlbm_lattice();
-    h->q = q;
-    h->nx = nx;
-    h->ny = ny;
-    h->nz = nz;
-    h->nvox = nvox;
-    auto env_ll = [](const char *name, long long dflt) {
-        const char *v = getenv(name);
-        return (v && *v) ? atoll(v) : dflt;
-    };
-    h->tune_pad = env_ll("ERLBM_PLANE_PAD", h->tune_pad);
-    h->tune_npt = (int)env_ll("ERLBM_NPT", h->tune_npt);
-    h->tune_ntl = (int)env_ll("ERLBM_NT_LOADS", h->tune_ntl);
-    h->tune_remap = (int)env_ll("ERLBM_XCD_REMAP", h->tune_remap);
-    h->tune_pair = (int)env_ll("ERLBM_PAIR", h->tune_pair);
-    h->tune_block = (int)env_ll("ERLBM_BLOCK", h->tune_block);
-    h->tune_vec = (int)env_ll("ERLBM_VEC", h->tune_vec);
-    h->fuse_requested = getenv("ERLBM_FUSE") != nullptr;
-    h->tune_fuse = (int)env_ll("ERLBM_FUSE", h->tune_fuse);
-    h->tune_fuse_remap = (int)env_ll("ERLBM_FUSE_REMAP", h->tune_fuse_remap);
-    h->tune_fuse_slots = (int)env_ll("ERLBM_FUSE_SLOTS", h->tune_fuse_slots);
-    h->tune_fuse_prio = (int)env_ll("ERLBM_FUSE_PRIO", h->tune_fuse_prio);
-    if (h->tune_pad < 0 || h->tune_pad > (1 << 24)) h->tune_pad = 0;
-    h->layout = (int)env_ll("ERLBM_LAYOUT", h->layout);
-    const long long pad = (h->tune_pad + 63) / 64 * 64;
-    const long long nrows = (long long)ny * nz;
-    if (h->layout == 0) {   // population planes
-        h->si = (nvox + 63) / 64 * 64 + pad;
-        h->sr = nx;
-    } else {                // row blocks
-        h->si = nx;
-        h->sr = (long long)q * nx + pad;
-    }
-    h->extent = (q - 1) * h->si + (nrows - 1) * h->sr + nx;
-    h->precision = precision;
-    h->elem = precision == ERLBM_FP64 ? 8 : (precision == ERLBM_FP16_SHIFTED ? 2 : 4);
-    h->f32_arith = precision == ERLBM_FP32_SHIFTED_F32;
+    set_layout(h, q, nx, ny, nz, precision);
     h->stream = (hipStream_t)stream;
     hipGetDevice(&h->device);
     if (hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || h->n_cu < 1)
@@ -1298,6 +1310,26 @@ extern "C" int erlbm_set_outlet_rescale(erlbm_lattice *h, int pos_x, double rho)
     }
     h->outlet_x = pos_x < 0 ? -1 : pos_x;
     h->rho_out = rho;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_set_body_force(erlbm_lattice *h, double ax, double ay, double az)
+{
+    if (!h) return ERLBM_ERR_ARG;
+    if (!std::isfinite(ax) || !std::isfinite(ay) || !std::isfinite(az)) {
+        set_last_error_msg("erlbm_set_body_force: accelerations must be finite");
+        return ERLBM_ERR_ARG;
+    }
+    if (h->q == 9) az = 0.0;   // the 2-D operators read x and y only
+    const bool forced = ax != 0.0 || ay != 0.0 || az != 0.0;
+    if (forced && h->elem != 8) {
+        set_last_error_msg("erlbm_set_body_force: a body force needs FP64 storage");
+        return ERLBM_ERR_UNSUPPORTED;
+    }
+    h->acc[0] = ax;
+    h->acc[1] = ay;
+    h->acc[2] = az;
+    h->forced = forced;
     return ERLBM_OK;
 }
 
@@ -1476,8 +1508,8 @@ long long strip3_workgroups(const erlbm_lattice *h, int f);
 
 int fuse_shape(const erlbm_lattice *h, int energy_interval)
 {
-    if (!(h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0 && h->dens_limit <= 0.0 && energy_interval <= 0 &&
-          !h->checked))
+    if (!(h->all_fluid && !h->forced && h->inlet_x < 0 && h->outlet_x < 0 && h->dens_limit <= 0.0 &&
+          energy_interval <= 0 && !h->checked))
         return 0;
     if (h->fuse_requested) return fuse_shape_fits(h, h->tune_fuse) ? h->tune_fuse : 0;
     if (h->q == 19) {
@@ -1924,6 +1956,10 @@ StepArgs step_args(const erlbm_lattice *h)
     a.k.omega1 = 1.0 - 1.0 / h->tau;
     a.k.omega2 = 1.0 - 1.0 / h->tau_nh;
     a.k.oot = 1.0 / h->tau;
+    a.bf.ax = h->acc[0];
+    a.bf.ay = h->acc[1];
+    a.bf.az = h->acc[2];
+    a.bf.factor = 1. - 1. / (2. * h->tau);   // source(), LBM_functions.cpp:5664
     a.bc.inlet_x = h->inlet_x;
     a.bc.inlet_west = h->inlet_x >= 0 && h->inlet_x < h->nx / 2;
     a.bc.u_in = h->u_in;
@@ -2000,7 +2036,7 @@ struct LaunchShape {
 std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
 {
     std::vector<LaunchShape> c;
-    const bool periodic = h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0;
+    const bool periodic = h->all_fluid && !h->forced && h->inlet_x < 0 && h->outlet_x < 0;
     if (!periodic) return c;
     if (h->elem == 8 && h->q == 9) {
         struct Named {
@@ -2024,7 +2060,7 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
             {FUSE3_ASTRIP_2W, {"three steps, lane-aligned strips of any width (2 wavefronts)",
                                "three steps, lane-aligned strips of any width (2 wavefronts), XCD runs"}}};
         for (const Named &nm : fused) {
-            if (!fuse_geometry_fits(h, FUSE_SHAPES[nm.f])) continue;
+            if (!fuse_shape_fits(h, nm.f)) continue;
             for (int rm = 0; rm < 2; rm++) {
                 LaunchShape ls{2, 256, 1, 0, 0, nm.name[rm], nm.f, rm};
                 c.push_back(ls);
@@ -2043,7 +2079,7 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
                                        {"two steps, 64x4 tiles walked along z", "two steps, 64x4 tiles walked along z, XCD runs"},
                                        {"two steps, 32x8 tiles walked along z", "two steps, 32x8 tiles walked along z, XCD runs"}};
         for (int k = 0; k < 3; k++) {
-            if (!fuse_geometry_fits(h, FUSE_SHAPES[zs[k]])) continue;
+            if (!fuse_shape_fits(h, zs[k])) continue;
             for (int rm = 0; rm < 2; rm++) c.push_back({2, 256, 1, 0, 0, zn[k][rm], zs[k], rm});
         }
         c.push_back({2, 256, 1, 0, 0, "nt-loads=c_x0"});
@@ -2056,13 +2092,15 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
                                               {"three steps, 256-wide strips x 32 rows", "three steps, 256-wide strips x 32 rows, XCD runs"},
                                               {"three steps, 128-wide strips x 32 rows", "three steps, 128-wide strips x 32 rows, XCD runs"}};
             const int fs[4] = {FUSE2_STRIP_256x32, FUSE2_STRIP_128x32, FUSE3_STRIP_256x32, FUSE3_STRIP_128x32};
+            // the same fit test the step path applies (whole strips, and
+            // buffer offsets below 2^31 bytes for the raw-buffer walks)
             for (int k = 0; k < 4; k++) {
-                if (h->nx % FUSE_SHAPES[fs[k]].tx || h->ny % FUSE_SHAPES[fs[k]].ty) continue;
+                if (!fuse_shape_fits(h, fs[k])) continue;
                 for (int rm = 0; rm < 2; rm++) c.push_back({h->tune_ntl, h->tune_block, 1, 0, 0, names[k][rm], fs[k], rm});
             }
             static const char *aligned[2] = {"three steps, lane-aligned strips of any width (4 wavefronts)",
                                              "three steps, lane-aligned strips of any width (4 wavefronts), XCD runs"};
-            if (fuse_geometry_fits(h, FUSE_SHAPES[FUSE3_ASTRIP_4W]))
+            if (fuse_shape_fits(h, FUSE3_ASTRIP_4W))
                 for (int rm = 0; rm < 2; rm++)
                     c.push_back({h->tune_ntl, h->tune_block, 1, 0, 0, aligned[rm], FUSE3_ASTRIP_4W, rm});
         }
@@ -2137,8 +2175,12 @@ extern "C" int erlbm_lattice_autotune(erlbm_lattice *h, int n_steps, char *choic
         return ERLBM_ERR_STATE;
     }
     const std::vector<LaunchShape> shapes = candidate_shapes(h);
-    const LaunchShape keep{h->tune_ntl, h->tune_block, h->tune_npt, h->tune_pair, h->tune_vec, "unchanged",
-                           h->tune_fuse, h->tune_fuse_remap};
+    // the handle's launch choice as it was, restored as such if the tuning fails
+    struct Kept {
+        int ntl, block, npt, pair, vec, fuse, fuse_remap, fuse_active;
+        bool fuse_requested;
+    } const kept{h->tune_ntl, h->tune_block, h->tune_npt,  h->tune_pair,     h->tune_vec,
+                 h->tune_fuse, h->tune_fuse_remap, h->fuse_active, h->fuse_requested};
     const int steps = (n_steps + 5) / 6 * 6;   // whole launches for the two- and three-step candidates
     // scratch copy of the current state, a private stop word: the handle's
     // run is not touched
@@ -2199,15 +2241,48 @@ extern "C" int erlbm_lattice_autotune(erlbm_lattice *h, int n_steps, char *choic
     (void)hipFree(tmp[0]);
     (void)hipFree(tmp[1]);
     (void)hipFree(st);
-    const LaunchShape &pick = (rc == ERLBM_OK && best >= 0) ? shapes[best] : keep;
-    apply_shape(h, pick);
+    if (rc != ERLBM_OK || best < 0) {
+        h->tune_ntl = kept.ntl;
+        h->tune_block = kept.block;
+        h->tune_npt = kept.npt;
+        h->tune_pair = kept.pair;
+        h->tune_vec = kept.vec;
+        h->tune_fuse = kept.fuse;
+        h->tune_fuse_remap = kept.fuse_remap;
+        h->fuse_active = kept.fuse_active;
+        h->fuse_requested = kept.fuse_requested;
+    } else {
+        apply_shape(h, shapes[best]);
+    }
     if (rc) return rc;
     if (choice && choice_len > 0) {
-        const char *name = best >= 0 ? pick.name : "fixed (no launch choice for this geometry)";
+        const char *name = best >= 0 ? shapes[best].name : "fixed (no launch choice for this geometry)";
         std::strncpy(choice, name, (size_t)choice_len - 1);
         choice[choice_len - 1] = '\0';
     }
     if (us_per_step) *us_per_step = best >= 0 ? best_ms * 1000.f / steps : 0.f;
+    return ERLBM_OK;
+}
+
+extern "C" int erlbm_autotune_candidates(int q, int nx, int ny, int nz, int precision, int n_cu, int *shapes,
+                                         int cap, int *n)
+{
+    if (!n || (q != 9 && q != 19) || nx < 1 || ny < 1 || nz < 1 || (q == 9 && nz != 1) || n_cu < 1 ||
+        (cap > 0 && !shapes) || (long long)nx * ny * nz > (1ll << 31) - 1 ||
+        (precision != ERLBM_FP64 && precision != ERLBM_FP32_SHIFTED && precision != ERLBM_FP16_SHIFTED &&
+         precision != ERLBM_FP32_SHIFTED_F32))
+        return ERLBM_ERR_ARG;
+    erlbm_lattice h;   // host-side description only: no device memory
+    set_layout(&h, q, nx, ny, nz, precision);
+    h.n_cu = n_cu;
+    const std::vector<LaunchShape> c = candidate_shapes(&h);
+    int k = 0;
+    for (const LaunchShape &ls : c) {
+        if (ls.fuse <= 0) continue;
+        if (k < cap) shapes[k] = ls.fuse;
+        k++;
+    }
+    *n = k;
     return ERLBM_OK;
 }
 

@@ -43,6 +43,7 @@ struct StepArgs {
     Geometry g;
     const uint32_t *__restrict__ nbr;  // general kernel only
     CollideConsts k;
+    BodyForce bf;                      // forced lattices (general kernel) only
     BCArgs bc;
     double dens_limit;                 // <= 0: no test
     unsigned long long *status;        // written (atomics) by this launch
@@ -165,6 +166,10 @@ struct erlbm_lattice {
     double rho_out = 1.0;
     double dens_limit = 0.0;
     bool e0_valid = false;
+    // body force (erlbm_set_body_force): forced lattices step on the general
+    // one-step kernel
+    bool forced = false;
+    double acc[3] = {0.0, 0.0, 0.0};
 
     // launch tuning (environment overrides read at creation):
     //   ERLBM_PLANE_PAD  doubles added to each population plane's stride

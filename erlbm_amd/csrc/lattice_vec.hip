@@ -189,7 +189,7 @@ bool vec_step_applies(const erlbm_lattice *h)
     // for FP16 storage only (FP64 and FP32 are as fast or faster per node
     // with one element per thread)
     if (h->tune_vec < 0 && h->elem != 2) return false;
-    if (!(h->all_fluid && h->inlet_x < 0 && h->outlet_x < 0)) return false;
+    if (!(h->all_fluid && !h->forced && h->inlet_x < 0 && h->outlet_x < 0)) return false;
     const int V = 16 / h->elem;
     // whole 16-byte vectors per row and 16-byte aligned row starts
     return (h->nx % V) == 0 && (h->si % V) == 0 && (h->sr % V) == 0;

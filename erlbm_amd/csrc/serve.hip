@@ -99,9 +99,12 @@ struct erlbm_server {
 
 namespace {
 
+size_t pool_bytes(const erlbm_server *s, const erlbm_batch *geom, int precision);
+int pool_key(int geom_kind, int precision, int collision);
+
 int make_pool(erlbm_server *s, int geom_kind, int precision, int collision, Pool **out)
 {
-    const int key = geom_kind * 64 + precision * 8 + collision;
+    const int key = pool_key(geom_kind, precision, collision);
     auto it = s->pools.find(key);
     if (it != s->pools.end()) {
         *out = it->second.get();
@@ -113,7 +116,7 @@ int make_pool(erlbm_server *s, int geom_kind, int precision, int collision, Pool
     p->collision = collision;
     p->cap = s->tiny_slots;
     const size_t sb = tiny::state_bytes(p->geom, precision);
-    p->bytes = (size_t)p->cap * (sb + sizeof(MapJob) + sizeof(MapResult) + sizeof(int));
+    p->bytes = pool_bytes(s, p->geom, precision);
     if (s->used + p->bytes > s->budget) return ERLBM_ERR_NOMEM;
     hipError_t e = hipSuccess;
     auto ok = [&](hipError_t r) {
@@ -191,18 +194,68 @@ int resolve_precision(int precision, double tau)
     return precision;
 }
 
+bool is_map(const erlbm_request &r)
+{
+    return r.kind == ERLBM_REQ_TG_MAP_POINT || r.kind == ERLBM_REQ_OBSTACLE_MAP_POINT;
+}
+
+// The map driver's job for a map-point request, precision resolved.
+int map_job(const erlbm_request &r, MapJob *J)
+{
+    int rc = r.kind == ERLBM_REQ_TG_MAP_POINT
+                 ? erlbm_tg_map_jobs(r.n_i, r.n_j, r.collision, r.tau_nh, r.precision, r.i, r.i + 1, r.j, r.j + 1, J)
+                 : erlbm_obstacle_map_jobs(r.n_i, r.n_j, 0.1, r.precision, r.i, r.i + 1, r.j, r.j + 1, J);
+    if (rc) return rc;
+    J->precision = resolve_precision(J->precision, J->tau);
+    return ERLBM_OK;
+}
+
+int pool_key(int geom_kind, int precision, int collision) { return geom_kind * 64 + precision * 8 + collision; }
+
+size_t pool_bytes(const erlbm_server *s, const erlbm_batch *geom, int precision)
+{
+    const size_t sb = tiny::state_bytes(geom, precision);
+    return (size_t)s->tiny_slots * (sb + sizeof(MapJob) + sizeof(MapResult) + sizeof(int));
+}
+
+size_t lattice_bytes(const erlbm_request &r)
+{
+    const int prec = r.precision == ERLBM_FP_AUTO ? ERLBM_FP64 : r.precision;
+    const size_t elem = prec == ERLBM_FP64 ? 8 : (prec == ERLBM_FP16_SHIFTED ? 2 : 4);
+    return 2 * elem * 9 * (size_t)r.L * r.L + 8 * 3 * (size_t)r.L * r.L;   // populations + monitors
+}
+
+// Bytes of budget a request would newly reserve: its pool's slot array if
+// that pool does not exist yet (map points), its lattice (D2Q9 runs); 0 when
+// it only needs a free slot of an existing pool or is malformed (admit()
+// then answers it).  *pool: the existing pool it would use, if any.
+size_t request_bytes(const erlbm_server *s, const erlbm_request &r, const Pool **pool)
+{
+    *pool = nullptr;
+    if (is_map(r)) {
+        MapJob J;
+        if (map_job(r, &J)) return 0;
+        const int kind = r.kind == ERLBM_REQ_TG_MAP_POINT ? 0 : 1;
+        auto it = s->pools.find(pool_key(kind, J.precision, J.collision));
+        if (it != s->pools.end()) {
+            *pool = it->second.get();
+            return 0;
+        }
+        return pool_bytes(s, kind == 0 ? s->tg_geom : s->obst_geom, J.precision);
+    }
+    if ((r.kind != ERLBM_REQ_TAYLOR_GREEN && r.kind != ERLBM_REQ_DOUBLE_SHEAR) || r.L < 4 || r.L > 65536) return 0;
+    return lattice_bytes(r);
+}
+
 // Try to admit one request; ERLBM_ERR_NOMEM = does not fit yet.
 int admit(erlbm_server *s, const Pending &pd)
 {
     const erlbm_request &r = pd.req;
-    if (r.kind == ERLBM_REQ_TG_MAP_POINT || r.kind == ERLBM_REQ_OBSTACLE_MAP_POINT) {
+    if (is_map(r)) {
         MapJob J;
         const int kind = r.kind == ERLBM_REQ_TG_MAP_POINT ? 0 : 1;
-        int rc = kind == 0 ? erlbm_tg_map_jobs(r.n_i, r.n_j, r.collision, r.tau_nh, r.precision, r.i, r.i + 1, r.j,
-                                               r.j + 1, &J)
-                           : erlbm_obstacle_map_jobs(r.n_i, r.n_j, 0.1, r.precision, r.i, r.i + 1, r.j, r.j + 1, &J);
+        int rc = map_job(r, &J);
         if (rc) return rc;
-        J.precision = resolve_precision(J.precision, J.tau);
         Pool *p = nullptr;
         rc = make_pool(s, kind, J.precision, J.collision, &p);
         if (rc) return rc;
@@ -223,7 +276,7 @@ int admit(erlbm_server *s, const Pending &pd)
     if (L < 4 || L > 65536) return ERLBM_ERR_ARG;
     const int prec = r.precision == ERLBM_FP_AUTO ? ERLBM_FP64 : r.precision;
     const size_t elem = prec == ERLBM_FP64 ? 8 : (prec == ERLBM_FP16_SHIFTED ? 2 : 4);
-    const size_t bytes = 2 * elem * 9 * (size_t)L * L + 8 * 3 * (size_t)L * L;   // populations + monitors
+    const size_t bytes = lattice_bytes(r);
     if (s->used + bytes > s->budget) return ERLBM_ERR_NOMEM;
     auto b = std::make_unique<Big>();
     b->id = pd.id;
@@ -266,6 +319,31 @@ int admit(erlbm_server *s, const Pending &pd)
     s->used += bytes;
     s->bigs.push_back(std::move(b));
     return ERLBM_OK;
+}
+
+// Bytes held by pools with no live request other than `keep`.
+size_t idle_pool_bytes(const erlbm_server *s, const Pool *keep)
+{
+    size_t b = 0;
+    for (const auto &kv : s->pools)
+        if (kv.second.get() != keep && kv.second->active.empty()) b += kv.second->bytes;
+    return b;
+}
+
+// Frees the pools that hold no live request, `keep` aside (a pool's slot
+// array counts against the budget for as long as the pool exists).
+void release_idle_pools(erlbm_server *s, const Pool *keep)
+{
+    for (auto it = s->pools.begin(); it != s->pools.end();) {
+        Pool *p = it->second.get();
+        if (p == keep || !p->active.empty()) {
+            ++it;
+            continue;
+        }
+        free_pool(p);
+        s->used -= p->bytes;
+        it = s->pools.erase(it);
+    }
 }
 
 void respond(erlbm_server *s, long long id, int kind, int unstable, long long steps, const double *v, double t_done)
@@ -350,14 +428,31 @@ extern "C" int erlbm_server_clock(erlbm_server *s, double *t)
 extern "C" int erlbm_server_pump(erlbm_server *s, int *in_flight, int *queued)
 {
     if (!s) return ERLBM_ERR_ARG;
-    // 1. admission, first fit in arrival order
+    // 1. admission, first fit in arrival order.  A request that needs more
+    // than the whole budget never fits and is answered ERLBM_ERR_NOMEM; one
+    // that would fit without the idle pools' reservations frees them; any
+    // other waits for in-flight requests to retire (they all do: every
+    // request has a step budget)
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         int rc = admit(s, *it);
+        if (rc == ERLBM_ERR_NOMEM) {
+            const Pool *keep = nullptr;
+            const size_t need = request_bytes(s, it->req, &keep);
+            const size_t idle = idle_pool_bytes(s, keep);
+            if (need > s->budget) {
+                set_last_error_msg("erlbm_server: request needs more than the HBM budget");
+            } else if (idle > 0 && s->used - idle + need <= s->budget) {
+                release_idle_pools(s, keep);
+                rc = admit(s, *it);
+            }
+            if (rc == ERLBM_ERR_NOMEM && need <= s->budget) {
+                ++it;
+                continue;
+            }
+        }
         if (rc == ERLBM_OK) {
             s->admit_time[it->id] = now_s(s->t0);
             it = s->queue.erase(it);
-        } else if (rc == ERLBM_ERR_NOMEM) {
-            ++it;
         } else {
             erlbm_response o{};
             o.id = it->id;

@@ -74,6 +74,11 @@ class Lattice:
     def set_density_check(self, limit: float) -> None:
         check(lib().erlbm_set_density_check(self._h, limit), "erlbm_set_density_check")
 
+    def set_body_force(self, ax: float, ay: float, az: float = 0.0) -> None:
+        """Body-force accelerations of the collision operators (the acc_x,
+        acc_y, acc_z arguments of coll_* in Collision_operators.cpp)."""
+        check(lib().erlbm_set_body_force(self._h, ax, ay, az), "erlbm_set_body_force")
+
     # -- transfer ----------------------------------------------------------------
     def upload(self, inif: np.ndarray) -> None:
         a = np.ascontiguousarray(inif, dtype=np.float64).reshape(-1)
@@ -133,6 +138,17 @@ class Lattice:
 
 
 # ---- host-side driver helpers (C ABI, libm on the host) ------------------------
+def autotune_candidates(q: int, nx: int, ny: int, nz: int = 1, precision: int = L.FP64, n_cu: int = 256) -> list:
+    """The multi-step shapes (ERLBM_FUSE values) the autotuner would time for
+    an all-fluid periodic lattice of this size, computed on the host."""
+    cap = 256
+    out = np.zeros(cap, np.int32)
+    n = C.c_int(0)
+    check(lib().erlbm_autotune_candidates(q, nx, ny, nz, precision, n_cu, out, cap, C.byref(n)),
+          "erlbm_autotune_candidates")
+    return [int(v) for v in out[: min(n.value, cap)]]
+
+
 def dist_eq(q: int, vx: float, vy: float, vz: float, rho: float) -> np.ndarray:
     out = np.empty(q)
     check(lib().erlbm_dist_eq(q, vx, vy, vz, rho, out), "erlbm_dist_eq")

@@ -84,6 +84,12 @@ class Server:
         check(self.lb.erlbm_server_clock(self._h, C.byref(t)), "erlbm_server_clock")
         return t.value
 
+    def memory(self):
+        """(bytes of budget in use, budget)."""
+        used, budget = C.c_longlong(), C.c_longlong()
+        check(self.lb.erlbm_server_memory(self._h, C.byref(used), C.byref(budget)), "erlbm_server_memory")
+        return used.value, budget.value
+
 
 def poisson_mix(n: int, rate: float, seed: int = 20261203, map_precision: int = L.FP64, dsl_t_final: float = 0.25,
                 tg_sizes=(128, 256, 512)) -> List[Solve]:

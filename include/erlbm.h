@@ -89,6 +89,17 @@ int erlbm_set_zouhe_inlet(erlbm_lattice *h, int pos_x, double vx);
 /* devnull_rho_x (Boundary_conditions.cpp:68-141) at column pos_x after the
  * inlet and before every collision; pos_x < 0 disables it.                */
 int erlbm_set_outlet_rescale(erlbm_lattice *h, int pos_x, double rho);
+/* Body force: the accelerations acc_x, acc_y, acc_z that colisao_sitio
+ * (src/erlbm_common.h:68-81) passes to every collision operator
+ * (coll_Ext_Reg, Collision_operators.cpp:1182; coll_Reg :859; coll_BGK :11;
+ * coll_MRT :487).  F = a rho, the equilibrium at u + a/2 and the source term
+ * of source() (SimBoltz_Functions/LBM_functions.cpp:5645-5716) added last
+ * (Collision_operators.cpp:1196-1254); coll_MRT forms its own forcing
+ * moments (:647-690).  D2Q9 ignores az, as the 2-D operators do.  All zero
+ * (the default, what every reference driver passes) is the unforced
+ * operator.  A forced lattice needs FP64 storage (ERLBM_ERR_UNSUPPORTED
+ * otherwise) and advances one step per launch.                             */
+int erlbm_set_body_force(erlbm_lattice *h, double ax, double ay, double az);
 /* Per-step post-collision density test of main_stability-map_obstacle.cpp:
  * 104-106: a step whose some site has |rho| > limit or non-finite rho stops
  * the run after that step completes.  limit <= 0 disables it.             */
@@ -122,6 +133,15 @@ int erlbm_step_timed(erlbm_lattice *h, int n_steps, float *ms);
  * advanced.  Geometries with solids or boundary columns have one kernel
  * and nothing to choose.  Needs uploaded populations; synchronises.      */
 int erlbm_lattice_autotune(erlbm_lattice *h, int n_steps, char *choice, int choice_len, float *us_per_step);
+/* Diagnostics (no counterpart in the reference): the multi-step shapes
+ * (ERLBM_FUSE values, each once per XCD-run setting) that
+ * erlbm_lattice_autotune would time for an all-fluid periodic lattice of
+ * this size and precision on a GPU of n_cu compute units -- the same fit
+ * tests as erlbm_step (whole strips or tiles, buffer offsets below 2^31
+ * bytes) -- computed on the host without any device call.  *n = the count;
+ * at most cap are written to shapes.                                      */
+int erlbm_autotune_candidates(int q, int nx, int ny, int nz, int precision, int n_cu, int *shapes, int cap,
+                              int *n);
 /* Diagnostics (no counterpart in the reference): the multi-step kernel
  * shape (the ERLBM_FUSE value; 0 = one step per launch), the steps one of
  * its launches advances and its description (`desc`, may be NULL) that

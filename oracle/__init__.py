@@ -66,7 +66,8 @@ class Oracle(_Lib):
             _sig(L, f"orc_kinetic_energy_{s}", _d, [_dp, _i])
             _sig(L, f"orc_devnull_rho_x_{s}", None, [_i, _d, _i, _i, _i, _ip, _dp])
             _sig(L, f"orc_collide_stream_{s}", _i, [_dp, _dp, _ip, _i, _i, _d, _d, _d, _i])
-            _sig(L, f"orc_run_steps_{s}", None, [_dp, _i, _ip, _i, _i, _d, _d, _i])
+            _sig(L, f"orc_collide_stream_acc_{s}", _i, [_dp, _dp, _ip, _i, _i, _d, _d, _d, _d, _d, _d, _i])
+            _sig(L, f"orc_run_steps_{s}", None, [_dp, _i, _ip, _i, _i, _d, _d, _d, _d, _d, _i])
         _sig(L, "orc_zouhe_d3q19_vx_x", None, [_i, _d, _i, _i, _i, _ip, _dp])
         _sig(L, "orc_zouhe_d2q9_vx_x", None, [_i, _d, _i, _i, _ip, _dp])
         _sig(L, "orc_init_taylor_green_d2q9", None, [_i, _d, _d, _i, _i, _dp])
@@ -81,8 +82,10 @@ class Oracle(_Lib):
         _sig(L, "orc_run_map", None, [_i, _i, _i, _d, _d, _i, _i, _ip, _ip, _i, _ip, _ip, _i])
         _sig(L, "orc_tg_error_d2q9", None, [_dp, _i, _d, _d, _i, _dp])
         _sig(L, "orc_run_taylor_green_d2q9", None, [_i, _i, _d, _d, _d, _d, _i, _i, _i, _dp])
-        _sig(L, "orc_run_bc_steps_d3q19", _i, [_dp, _i, _i, _i, _ip, _ip, _i, _i, _i, _d, _d, _i, _d, _i, _d, _d, C.POINTER(_i)])
-        _sig(L, "orc_run_bc_steps_d2q9", _i, [_dp, _i, _i, _ip, _ip, _i, _i, _i, _d, _d, _i, _d, _i, _d, _d, C.POINTER(_i)])
+        _sig(L, "orc_run_bc_steps_d3q19", _i, [_dp, _i, _i, _i, _ip, _ip, _i, _i, _i, _d, _d, _i, _d, _i, _d, _d,
+                                               _d, _d, _d, C.POINTER(_i)])
+        _sig(L, "orc_run_bc_steps_d2q9", _i, [_dp, _i, _i, _ip, _ip, _i, _i, _i, _d, _d, _i, _d, _i, _d, _d,
+                                              _d, _d, _d, C.POINTER(_i)])
         _sig(L, "orc_max_threads", _i, [])
 
     # ---- lattice constants -------------------------------------------------
@@ -149,23 +152,27 @@ class Oracle(_Lib):
         return getattr(self.lib, f"orc_collide_stream_{_s(q)}")(a.reshape(-1), b.reshape(-1), stream, a.size // q,
                                                                col, tau, tau_nh, dens_limit, nthreads)
 
-    def run_steps(self, q, inif, stream, n_steps, col, tau, tau_nh, nthreads=1) -> np.ndarray:
+    def run_steps(self, q, inif, stream, n_steps, col, tau, tau_nh, nthreads=1, acc=(0.0, 0.0, 0.0)) -> np.ndarray:
+        """n_steps collide-stream sweeps; acc = the collisions' body-force
+        accelerations (ax, ay, az)."""
         a = np.ascontiguousarray(inif, np.float64).copy()
         getattr(self.lib, f"orc_run_steps_{_s(q)}")(a.reshape(-1), a.size // q, stream, n_steps, col, tau, tau_nh,
-                                                    nthreads)
+                                                    float(acc[0]), float(acc[1]), float(acc[2]), nthreads)
         return a
 
     def run_bc_steps(self, q, inif, nx, ny, nz, ini, stream, n_steps, col, tau, tau_nh, inlet_x=-1, u_in=0.0,
-                     outlet_x=-1, rho_out=1.0, dens_limit=0.0):
+                     outlet_x=-1, rho_out=1.0, dens_limit=0.0, acc=(0.0, 0.0, 0.0)):
         a = np.ascontiguousarray(inif, np.float64).copy()
         viol = C.c_int(0)
+        ax, ay, az = (float(v) for v in acc)
         if q == 19:
             n = self.lib.orc_run_bc_steps_d3q19(a.reshape(-1), nx, ny, nz, ini, stream, a.size // q, n_steps, col,
                                                 tau, tau_nh, inlet_x, u_in, outlet_x, rho_out, dens_limit,
-                                                C.byref(viol))
+                                                ax, ay, az, C.byref(viol))
         else:
             n = self.lib.orc_run_bc_steps_d2q9(a.reshape(-1), nx, ny, ini, stream, a.size // q, n_steps, col, tau,
-                                               tau_nh, inlet_x, u_in, outlet_x, rho_out, dens_limit, C.byref(viol))
+                                               tau_nh, inlet_x, u_in, outlet_x, rho_out, dens_limit, ax, ay, az,
+                                               C.byref(viol))
         return a, n, viol.value
 
     # ---- driver inits and runs ---------------------------------------------
@@ -256,7 +263,8 @@ class Reference(_Lib):
         _sig(L, f"ref_density_{s}", _d, [_dp])
         _sig(L, f"ref_kinetic_energy_{s}", _d, [_i, _i, _i, _ip, _dp])
         _sig(L, f"ref_boundary_{s}", None, [_i, _i, _i, _i, _ip, _dp, _i, _d])
-        _sig(L, f"ref_run_steps_{s}", _i, [_i, _i, _i, _ip, _dp, _i, _i, _d, _d, _i, _d, _i, _d, _d, C.POINTER(_i)])
+        _sig(L, f"ref_run_steps_{s}", _i, [_i, _i, _i, _ip, _dp, _i, _i, _d, _d, _i, _d, _i, _d, _d, _d, _d, _d,
+                                           C.POINTER(_i)])
         _sig(L, f"ref_run_steps_omp_{s}", _d, [_i, _i, _i, _dp, _i, _i, _d, _d, _i])
         _sig(L, f"ref_rec_velocity_{s}", None, [_i, _i, _i, _ip, _dp, _d])
         _sig(L, f"ref_rec_vorticity_{s}", None, [_i, _i, _i, _ip, _dp, C.c_uint])
@@ -318,12 +326,14 @@ class Reference(_Lib):
         return g
 
     def run_steps(self, nx, ny, nz, mask, inif, n_steps, col, tau, tau_nh, inlet_x=-1, u_in=0.0, outlet_x=-1,
-                  rho_out=1.0, dens_limit=0.0):
+                  rho_out=1.0, dens_limit=0.0, acc=(0.0, 0.0, 0.0)):
         m = np.ones(nx * ny * nz, np.int32) if mask is None else np.ascontiguousarray(mask, np.int32)
         g = np.ascontiguousarray(inif, np.float64).copy()
         viol = C.c_int(0)
+        ax, ay, az = (float(v) for v in acc)
         n = getattr(self.lib, f"ref_run_steps_{self.s}")(nx, ny, nz, m, g.reshape(-1), n_steps, col, tau, tau_nh,
-                                                         inlet_x, u_in, outlet_x, rho_out, dens_limit, C.byref(viol))
+                                                         inlet_x, u_in, outlet_x, rho_out, dens_limit, ax, ay, az,
+                                                         C.byref(viol))
         return g, n, viol.value
 
     def run_steps_omp(self, nx, ny, nz, inif, n_steps, col, tau, tau_nh, nthreads=0) -> float:

@@ -389,15 +389,16 @@ void orc_run_taylor_green_d2q9(int Li, int col, double tau_nh_in, double Re,
     free(mask); free(ini); free(st); free(a); free(b);
 }
 
-/* Run n_steps collide-stream sweeps of a whole lattice (no BCs), ping-pong. */
+/* Run n_steps collide-stream sweeps of a whole lattice (no BCs), ping-pong,
+ * under the body force (ax, ay, az). */
 void orc_run_steps_d2q9(double *inif, int n_fluid, const int *ini_stream,
                         int n_steps, int col, double tau, double tau_nh,
-                        int nthreads)
+                        double ax, double ay, double az, int nthreads)
 {
     double *b = (double *)malloc(sizeof(double) * (size_t)n_fluid * 9);
     double *a = inif;
     for (int s = 0; s < n_steps; s++) {
-        orc_collide_stream_d2q9(a, b, ini_stream, n_fluid, col, tau, tau_nh, 0.0, nthreads);
+        orc_collide_stream_acc_d2q9(a, b, ini_stream, n_fluid, col, tau, tau_nh, ax, ay, az, 0.0, nthreads);
         double *t = a; a = b; b = t;
     }
     if (a != inif) {
@@ -410,12 +411,12 @@ void orc_run_steps_d2q9(double *inif, int n_fluid, const int *ini_stream,
 
 void orc_run_steps_d3q19(double *inif, int n_fluid, const int *ini_stream,
                          int n_steps, int col, double tau, double tau_nh,
-                         int nthreads)
+                         double ax, double ay, double az, int nthreads)
 {
     double *b = (double *)malloc(sizeof(double) * (size_t)n_fluid * 19);
     double *a = inif;
     for (int s = 0; s < n_steps; s++) {
-        orc_collide_stream_d3q19(a, b, ini_stream, n_fluid, col, tau, tau_nh, 0.0, nthreads);
+        orc_collide_stream_acc_d3q19(a, b, ini_stream, n_fluid, col, tau, tau_nh, ax, ay, az, 0.0, nthreads);
         double *t = a; a = b; b = t;
     }
     if (a != inif) {
@@ -428,13 +429,15 @@ void orc_run_steps_d3q19(double *inif, int n_fluid, const int *ini_stream,
 
 /* General D3Q19 step loop with the obstacle driver's boundary conditions:
  * per step ZouHe at inlet_x (if >= 0), devnull at outlet_x (if >= 0), then
- * collide (+density check when dens_limit > 0) and push.  Returns the number
- * of steps run; stops after the first step whose check fails.               */
+ * collide (+density check when dens_limit > 0; body force ax, ay, az) and
+ * push.  Returns the number of steps run; stops after the first step whose
+ * check fails.                                                             */
 int orc_run_bc_steps_d3q19(double *inif, int nx, int ny, int nz, const int *ini,
                            const int *ini_stream, int n_fluid, int n_steps,
                            int col, double tau, double tau_nh, int inlet_x,
                            double u_in, int outlet_x, double rho_out,
-                           double dens_limit, int *violated)
+                           double dens_limit, double ax, double ay, double az,
+                           int *violated)
 {
     double *b = (double *)malloc(sizeof(double) * (size_t)n_fluid * 19);
     double *a = inif;
@@ -442,7 +445,7 @@ int orc_run_bc_steps_d3q19(double *inif, int nx, int ny, int nz, const int *ini,
     for (s = 0; s < n_steps && !bad; s++) {
         if (inlet_x >= 0) orc_zouhe_d3q19_vx_x(inlet_x, u_in, nx, ny, nz, ini, a);
         if (outlet_x >= 0) orc_devnull_rho_x_d3q19(outlet_x, rho_out, nx, ny, nz, ini, a);
-        bad = orc_collide_stream_d3q19(a, b, ini_stream, n_fluid, col, tau, tau_nh, dens_limit, 1);
+        bad = orc_collide_stream_acc_d3q19(a, b, ini_stream, n_fluid, col, tau, tau_nh, ax, ay, az, dens_limit, 1);
         double *t = a; a = b; b = t;
     }
     if (a != inif) {
@@ -460,7 +463,8 @@ int orc_run_bc_steps_d2q9(double *inif, int nx, int ny, const int *ini,
                           const int *ini_stream, int n_fluid, int n_steps,
                           int col, double tau, double tau_nh, int inlet_x,
                           double u_in, int outlet_x, double rho_out,
-                          double dens_limit, int *violated)
+                          double dens_limit, double ax, double ay, double az,
+                          int *violated)
 {
     double *b = (double *)malloc(sizeof(double) * (size_t)n_fluid * 9);
     double *a = inif;
@@ -468,7 +472,7 @@ int orc_run_bc_steps_d2q9(double *inif, int nx, int ny, const int *ini,
     for (s = 0; s < n_steps && !bad; s++) {
         if (inlet_x >= 0) orc_zouhe_d2q9_vx_x(inlet_x, u_in, nx, ny, ini, a);
         if (outlet_x >= 0) orc_devnull_rho_x_d2q9(outlet_x, rho_out, nx, ny, 1, ini, a);
-        bad = orc_collide_stream_d2q9(a, b, ini_stream, n_fluid, col, tau, tau_nh, dens_limit, 1);
+        bad = orc_collide_stream_acc_d2q9(a, b, ini_stream, n_fluid, col, tau, tau_nh, ax, ay, az, dens_limit, 1);
         double *t = a; a = b; b = t;
     }
     if (a != inif) {

@@ -504,13 +504,16 @@ void SFX(orc_devnull_rho_x)(int pos_x, double rho, int nx, int ny, int nz,
 /* ---- one driver step: [BCs] -> per site {collide; [density check]; push} ---
  * Mirrors the step loops of src/main_double-shear-layer.cpp:167-180,
  * src/main_stability-map_taylor-green.cpp:83-92 and
- * src/main_stability-map_obstacle.cpp:98-111.  Returns 1 when some site's
+ * src/main_stability-map_obstacle.cpp:98-111, with the body-force
+ * accelerations ax, ay, az passed to every collision (colisao_sitio's
+ * acc_x, acc_y, acc_z, src/erlbm_common.h:68-81).  Returns 1 when some site's
  * post-collision density is non-finite or |rho| > dens_limit (checked only
  * when dens_limit > 0).  The caller swaps the buffers.                       */
-int SFX(orc_collide_stream)(double *inif, double *inif_new,
-                            const int *ini_stream, int n_fluid, int col,
-                            double tau, double tau_nh, double dens_limit,
-                            int nthreads)
+int SFX(orc_collide_stream_acc)(double *inif, double *inif_new,
+                                const int *ini_stream, int n_fluid, int col,
+                                double tau, double tau_nh, double ax,
+                                double ay, double az, double dens_limit,
+                                int nthreads)
 {
     SFX(setup)();
     int bad = 0;
@@ -519,7 +522,7 @@ int SFX(orc_collide_stream)(double *inif, double *inif_new,
 #pragma omp parallel for num_threads(nthreads) reduction(| : bad)
         for (int p = 0; p < n_fluid; p++) {
             double *f = inif + (size_t)p * NV;
-            SFX(collide)(f, 0., 0., 0., col, tau, tau_nh);
+            SFX(collide)(f, ax, ay, az, col, tau, tau_nh);
             if (dens_limit > 0) {
                 double d = SFX(density)(f);
                 if (fabs(d) > dens_limit || !isfinite(d)) bad |= 1;
@@ -532,7 +535,7 @@ int SFX(orc_collide_stream)(double *inif, double *inif_new,
     (void)nthreads;
     for (int p = 0; p < n_fluid; p++) {
         double *f = inif + (size_t)p * NV;
-        SFX(collide)(f, 0., 0., 0., col, tau, tau_nh);
+        SFX(collide)(f, ax, ay, az, col, tau, tau_nh);
         if (dens_limit > 0) {
             double d = SFX(density)(f);
             if (fabs(d) > dens_limit || !isfinite(d)) bad = 1;
@@ -540,4 +543,15 @@ int SFX(orc_collide_stream)(double *inif, double *inif_new,
         SFX(propag_site)(inif, inif_new, ini_stream, p);
     }
     return bad;
+}
+
+/* The same sweep with the drivers' zero body force. */
+int SFX(orc_collide_stream)(double *inif, double *inif_new,
+                            const int *ini_stream, int n_fluid, int col,
+                            double tau, double tau_nh, double dens_limit,
+                            int nthreads)
+{
+    return SFX(orc_collide_stream_acc)(inif, inif_new, ini_stream, n_fluid, col,
+                                       tau, tau_nh, 0., 0., 0., dens_limit,
+                                       nthreads);
 }

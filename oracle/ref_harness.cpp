@@ -171,10 +171,13 @@ void RSFX(ref_boundary)(int which, int nx, int ny, int nz, const int *mask,
 // The drivers' step loop: [ZouHe inlet] -> [devnull outlet] -> per site
 // {collide; [density check]; propag_site} -> swap, n_steps times, stopping
 // after a step whose check fails (the obstacle driver's loop condition).
+// ax, ay, az: the acc_x, acc_y, acc_z passed to colisao_sitio (the drivers
+// pass 0).
 int RSFX(ref_run_steps)(int nx, int ny, int nz, const int *mask, double *inif,
                         int n_steps, int col, double tau, double tau_nh,
                         int inlet_x, double u_in, int outlet_x, double rho_out,
-                        double dens_limit, int *violated)
+                        double dens_limit, double ax, double ay, double az,
+                        int *violated)
 {
     RefLattice r;
     build(r, nx, ny, nz, mask);
@@ -195,7 +198,7 @@ int RSFX(ref_run_steps)(int nx, int ny, int nz, const int *mask, double *inif,
         if (outlet_x >= 0) devnull_rho_x(outlet_x, rho_out, r.g, r.l);
         for (int pto = 0; pto < nf; pto++) {
             double *f = r.l.inif + pto * nvel;
-            colisao_sitio(f, 0., 0., 0., r.l, p, col);
+            colisao_sitio(f, ax, ay, az, r.l, p, col);
             if (dens_limit > 0) {
                 const double d = density(f);
                 if (fabs(d) > dens_limit || !std::isfinite(d)) bad = 1;

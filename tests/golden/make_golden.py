@@ -170,6 +170,69 @@ def trajectories():
     np.savez_compressed(os.path.join(HERE, "ref_trajectories.npz"), **data)
 
 
+def forced_trajectories():
+    """Multi-step runs of the reference's step loop under a body force (the
+    acc_x, acc_y, acc_z that colisao_sitio passes to every collision
+    operator): body-force-driven channels between halfway bounce-back walls
+    and periodic lattices, every collision model, strict and fma builds."""
+    o = O.Oracle()
+    data = {}
+    r9s, r9f = O.Reference(9), O.Reference(9, fma=True)
+    r19s, r19f = O.Reference(19), O.Reference(19, fma=True)
+    # D2Q9 channel 16 x 12, solid rows y = 0 and y = 11, periodic in x, from
+    # rest, a = (1e-5, 0): 400 steps, BGK / MRT / RLBM / E-RLBM
+    nx, ny = 16, 12
+    m = np.ones(nx * ny, np.int32)
+    m[[x + y * nx for y in (0, ny - 1) for x in range(nx)]] = 0
+    ini, st, nf = r9s.stream_map(nx, ny, 1, m)
+    f0 = np.array([o.dist_eq(9, 0, 0, 0, 1.0)] * nf)
+    data["pois9_mask"] = m
+    data["pois9_in"] = f0
+    acc9 = (1e-5, 0.0, 0.0)
+    data["pois9_acc"] = np.array(acc9)
+    for col in (O.COL_BGK, O.COL_MRT, O.COL_REG, O.COL_EXT_REG):
+        for tag, r in (("strict", r9s), ("fma", r9f)):
+            data[f"pois9_c{col}_out_{tag}"] = r.run_steps(nx, ny, 1, m, f0, 400, col, 0.8, 0.65, acc=acc9)[0]
+    # D2Q9 periodic double shear layer L = 32 with a = (2e-6, -1e-6): 200 steps
+    f0 = r9s.init_double_shear(32)
+    data["dsl32f_in"] = f0
+    acc = (2e-6, -1e-6, 0.0)
+    data["dsl32f_acc"] = np.array(acc)
+    for col in (O.COL_BGK, O.COL_MRT, O.COL_REG, O.COL_EXT_REG):
+        for tag, r in (("strict", r9s), ("fma", r9f)):
+            data[f"dsl32f_c{col}_out_{tag}"] = r.run_steps(32, 32, 1, None, f0, 200, col, 0.52,
+                                                           1.0 if col == O.COL_REG else 0.65, acc=acc)[0]
+    # D3Q19 duct 6 x 8 x 8, solid at y in {0, 7} and z in {0, 7}, periodic in
+    # x, from rest, a = (1e-5, 0, 2e-6): 300 steps, BGK / RLBM / E-RLBM
+    nx, ny, nz = 6, 8, 8
+    m = np.ones(nx * ny * nz, np.int32)
+    for z in range(nz):
+        for y in range(ny):
+            if y in (0, ny - 1) or z in (0, nz - 1):
+                m[[x + nx * (y + ny * z) for x in range(nx)]] = 0
+    ini, st, nf = r19s.stream_map(nx, ny, nz, m)
+    f0 = np.array([o.dist_eq(19, 0, 0, 0, 1.0)] * nf)
+    data["duct19_mask"] = m
+    data["duct19_in"] = f0
+    acc19 = (1e-5, 0.0, 2e-6)
+    data["duct19_acc"] = np.array(acc19)
+    for col in (O.COL_BGK, O.COL_REG, O.COL_EXT_REG):
+        for tag, r in (("strict", r19s), ("fma", r19f)):
+            data[f"duct19_c{col}_out_{tag}"] = r.run_steps(nx, ny, nz, m, f0, 300, col, 0.7, 0.65, acc=acc19)[0]
+    # D3Q19 periodic 8^3, seeded perturbation of a moving equilibrium, a =
+    # (1e-5, -2e-5, 3e-6): 100 steps, E-RLBM
+    rng = np.random.default_rng(SEED)
+    f0 = np.tile(o.dist_eq(19, 0.02, -0.01, 0.015, 1.0), (8 ** 3, 1))
+    f0 *= 1.0 + 1e-3 * rng.uniform(-1.0, 1.0, f0.shape)
+    data["per19f_in"] = f0
+    acc = (1e-5, -2e-5, 3e-6)
+    data["per19f_acc"] = np.array(acc)
+    for tag, r in (("strict", r19s), ("fma", r19f)):
+        data[f"per19f_c{O.COL_EXT_REG}_out_{tag}"] = r.run_steps(8, 8, 8, None, f0, 100, O.COL_EXT_REG, 0.6, 0.65,
+                                                                 acc=acc)[0]
+    np.savez_compressed(os.path.join(HERE, "ref_forced.npz"), **data)
+
+
 def outputs():
     """The reference's output files and monitors: rec_velocity /
     rec_vorticity / calc_enstrophy on seeded non-equilibrium lattices with
@@ -303,7 +366,7 @@ def map_disagreements(gpu_dir):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--maps", action="store_true", help="also rerun the 20x20 map subsamples (minutes)")
-    ap.add_argument("--only", choices=["functions", "outputs", "kappa4", "map_disagreements"],
+    ap.add_argument("--only", choices=["functions", "forced", "outputs", "kappa4", "map_disagreements"],
                     help="write only this group of fixtures")
     ap.add_argument("--gpu-maps", default=None, help="directory of the GPU's map files (map_disagreements)")
     a = ap.parse_args()
@@ -311,6 +374,9 @@ if __name__ == "__main__":
         O.build(ref=True)
     if a.only == "functions":
         per_function()
+        sys.exit(0)
+    if a.only == "forced":
+        forced_trajectories()
         sys.exit(0)
     if a.only == "outputs":
         outputs()
@@ -324,6 +390,7 @@ if __name__ == "__main__":
     published()
     per_function()
     trajectories()
+    forced_trajectories()
     outputs()
     kappa4_strict()
     if a.maps:

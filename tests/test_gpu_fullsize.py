@@ -1,0 +1,93 @@
+"""The double-shear-layer workload at its full size: D2Q9 E-RLBM, periodic
+L = 2048 (4,194,304 nodes), tau = 0.5 + 3 U0 L / Re (Re = 80000, U0 = 1/32),
+tau_nh = 0.65, equilibrium initialisation of src/main_double-shear-layer.cpp:
+48-66, 1000 steps, as the default kernels run it (three steps per launch).
+
+  * FP64: bitwise the strict oracle (the reference's operation order, no FMA
+    contraction); max relative population error <= 1e-10 against the
+    FMA-contracted oracle (the reference's default build);
+  * FP32-shifted storage: bitwise the one-step FP32 path, and relative L2
+    velocity error against the FP64 oracle below 5e-6 (the storage format's
+    error budget at this size and step count).
+
+The oracle runs once per module (about 25 s on 16 host threads per build).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+L, N = 2048, 1000
+TAU = 0.5 + 3.0 * (1.0 / 32.0) * L / 80000.0
+TAU_NH = 0.65
+FP32_L2_BOUND = 5e-6
+NTHREADS = min(16, len(os.sched_getaffinity(0)))
+
+
+@pytest.fixture(scope="module")
+def dsl_setup(orc):
+    import erlbm_amd as E
+    f0 = E.init_double_shear(L)
+    ini, st, nf = orc.stream_map(9, L, L, 1)
+    return f0, st
+
+
+@pytest.fixture(scope="module")
+def strict_1000(orc, dsl_setup):
+    f0, st = dsl_setup
+    print(f"oracle (strict): DSL L={L}, {N} steps, {NTHREADS} threads", flush=True)
+    return orc.run_steps(9, f0, st, N, O.COL_EXT_REG, TAU, TAU_NH, nthreads=NTHREADS)
+
+
+def _gpu_run(gpu, f0, precision):
+    with gpu.Lattice(9, L, L, 1, precision=precision) as lat:
+        lat.set_params(TAU, TAU_NH, 1.0, O.COL_EXT_REG)
+        lat.upload(f0)
+        shape, spl, desc = lat.step_shape()
+        lat.step(N)
+        assert lat.status() == (N, 0)
+        return lat.download(), (shape, spl, desc)
+
+
+def _vel(f):
+    c = np.array([[0, 0], [1, 0], [-1, 0], [0, 1], [0, -1], [1, 1], [-1, -1], [1, -1], [-1, 1]], float)
+    return (f @ c) / f.sum(1)[:, None]
+
+
+def test_dsl2048_1000_steps_default_kernel_bitwise(gpu, dsl_setup, strict_1000):
+    f0, _ = dsl_setup
+    got, (shape, spl, desc) = _gpu_run(gpu, f0, gpu.FP64)
+    print(f"GPU default kernel: {desc} (shape {shape}, {spl} steps per launch)", flush=True)
+    assert spl == 3
+    assert np.array_equal(got, strict_1000)
+
+
+def test_dsl2048_1000_steps_within_1e10_of_fma_oracle(gpu, dsl_setup):
+    f0, st = dsl_setup
+    print(f"oracle (FMA): DSL L={L}, {N} steps, {NTHREADS} threads", flush=True)
+    want = O.Oracle(fma=True).run_steps(9, f0, st, N, O.COL_EXT_REG, TAU, TAU_NH, nthreads=NTHREADS)
+    got, _ = _gpu_run(gpu, f0, gpu.FP64)
+    err = float(np.max(np.abs(got - want) / np.abs(want)))
+    print(f"max relative population error vs the FMA build: {err:.3e}", flush=True)
+    assert err <= 1e-10
+
+
+def test_dsl2048_fp32_shifted_1000_steps(gpu, monkeypatch, dsl_setup, strict_1000):
+    f0, _ = dsl_setup
+    got, (shape, spl, desc) = _gpu_run(gpu, f0, gpu.FP32_SHIFTED)
+    print(f"GPU FP32 default kernel: {desc} (shape {shape}, {spl} steps per launch)", flush=True)
+    assert spl == 3
+    monkeypatch.setenv("ERLBM_FUSE", "0")
+    one, (shape1, spl1, _) = _gpu_run(gpu, f0, gpu.FP32_SHIFTED)
+    assert (shape1, spl1) == (0, 1)
+    assert np.array_equal(got, one)
+    u, uw = _vel(got), _vel(strict_1000)
+    err = float(np.sqrt(np.sum((u - uw) ** 2) / np.sum(uw ** 2)))
+    mass = abs(got.sum() - strict_1000.sum()) / strict_1000.sum()
+    print(f"FP32-shifted vs FP64 oracle: L2 velocity error {err:.3e}, mass drift {mass:.2e}", flush=True)
+    assert err < FP32_L2_BOUND
+    assert mass < 1e-7
