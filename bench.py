@@ -101,7 +101,15 @@ def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True):
         launch = ("16-byte vectors, one step per launch (default)" if prec == E.FP16_SHIFTED
                   else lat.step_shape()[2] + " (default)")
     ms = lat.step_timed(steps)
+    # the dominant kernel alone: whole multi-step launches (the run is past
+    # its first step, so a call of a multiple of steps_per_launch steps is
+    # made of those launches only), HIP events on the lattice's stream
+    steps_per_launch = lat.step_shape()[1]
+    n_pre, n_launch = 10, 100
+    lat.step(steps_per_launch * n_pre)
+    kernel_ms = lat.step_timed(steps_per_launch * n_launch) / n_launch
     steps_done, reason = lat.status()
+    steps_total = warmup + steps + steps_per_launch * (n_pre + n_launch)
     ms_per_step = ms / steps
     t = time.perf_counter()
     post = lat.download(E.POST_COLLISION)
@@ -110,18 +118,17 @@ def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True):
     finite = bool(np.isfinite(post).all())
     del post
     lat.close()
-    ok = steps_done == warmup + steps and reason == 0 and finite and mass_err < (1e-9 if prec == E.FP64 else 1e-3)
+    ok = steps_done == steps_total and reason == 0 and finite and mass_err < (1e-9 if prec == E.FP64 else 1e-3)
     mlups = L * L / (ms_per_step * 1e-3) / 1e6
     # a launch reads and writes every population once, whether it advances
     # one step, two or three (the multi-step kernels keep the middle steps in
-    # LDS)
-    steps_per_launch = 3 if "three steps" in launch else 2 if "two steps" in launch else 1
-    ms_per_launch = ms_per_step * steps_per_launch
-    gbs = BYTES_PER_NODE[precision] * L * L / (ms_per_launch * 1e-3) / 1e9
+    # LDS): algorithmic bytes of one launch / its event-timed duration
+    gbs = BYTES_PER_NODE[precision] * L * L / (kernel_ms * 1e-3) / 1e9
     aos_bytes = L * L * 9 * 8
     return ok, {"MLUPS": round(mlups, 1), "ms_per_step": round(ms_per_step, 5), "achieved_GBs": round(gbs, 1),
                 "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "mass_rel_error": mass_err, "launch": launch,
-                "steps_per_launch": steps_per_launch, "ms_per_launch": round(ms_per_launch, 5),
+                "steps_per_launch": steps_per_launch, "ms_per_launch": round(kernel_ms, 5),
+                "kernel_MLUPS": round(L * L * steps_per_launch / (kernel_ms * 1e-3) / 1e6, 1),
                 "bytes_per_node_step": BYTES_PER_NODE[precision] / steps_per_launch,
                 "host_transfer": {"aos_bytes": aos_bytes, "upload_s": round(t_up, 4), "download_s": round(t_down, 4),
                                   "upload_GBs": round(aos_bytes / t_up / 1e9, 2),
@@ -182,9 +189,10 @@ E_COL_EXT_REG = 4
 
 
 def d3q19_section(E, n: int = 256, steps: int = 200):
-    """The D3Q19 plain path at n^3 (periodic, E-RLBM, FP64): 304 algorithmic
-    bytes per node and launch (19 FP64 populations read + 19 written), for
-    one step or, with the z-walk, two."""
+    """The D3Q19 plain path at n^3 (periodic, E-RLBM, FP64) with the default
+    kernel: 304 algorithmic bytes per node and launch (19 FP64 populations
+    read + 19 written), for one step or, with the z-walk, two; the launch
+    duration event-timed over whole launches."""
     f0 = np.tile(E.dist_eq(19, 0.02, -0.01, 0.015, 1.0), (n ** 3, 1))
     f0 *= 1.0 + 1e-3 * np.random.default_rng(0).uniform(-1.0, 1.0, f0.shape)
     with E.Lattice(19, n, n, n) as lat:
@@ -192,13 +200,15 @@ def d3q19_section(E, n: int = 256, steps: int = 200):
         lat.upload(f0)
         del f0
         lat.step(10)
-        launch = lat.autotune(50)[0]
+        shape, spl, launch = lat.step_shape()
+        steps = steps // spl * spl
         ms = lat.step_timed(steps) / steps
-    spl = 2 if "two steps" in launch else 1
+        done, reason = lat.status()
     gbs = 304 * n ** 3 / (ms * spl * 1e-3) / 1e9   # algorithmic bytes of one launch / its duration
     return {"workload": f"D3Q19 E-RLBM periodic {n}^3, tau 0.6, tau_nh 0.65, fp64", "MLUPS": round(n ** 3 / (ms * 1e-3) / 1e6, 1),
-            "ms_per_step": round(ms, 4), "launch": launch, "steps_per_launch": spl,
+            "ms_per_step": round(ms, 4), "launch": launch, "steps_per_launch": spl, "ms_per_launch": round(ms * spl, 4),
             "achieved_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "completed": done == 10 + steps and reason == 0,
             # what a one-step-per-launch kernel would have to stream for this MLUPS
             "one_step_equivalent_GBs": round(304 * n ** 3 / (ms * 1e-3) / 1e9, 1)}
 
@@ -286,6 +296,8 @@ def main() -> int:
                     help="storage of the headline run (fp32/fp16: shifted deviations, FP64 arithmetic)")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the calibration copy kernel (302 MB read + 302 MB written per launch)")
+    ap.add_argument("--only", choices=["d3q19", "maps", "serving"],
+                    help="run one section alone and print it (profiling runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -297,6 +309,10 @@ def main() -> int:
     if E.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
+    if a.only:
+        sec = {"d3q19": lambda: d3q19_section(E), "maps": lambda: maps_section(E), "serving": serving_section}
+        print(json.dumps({a.only: sec[a.only]()}))
+        return 0
     copy_bytes = BYTES_PER_NODE["fp64"] * L * L // 2     # a copy moving the bytes of one FP64 step
     copy_ms = E.hbm_copy_bench(copy_bytes, 5 if a.copy_only else 20)
     copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
@@ -333,16 +349,25 @@ def main() -> int:
         extra["serving"] = serving_section()
 
     traffic = valu_busy = None
-    if a.precision == "fp64" and os.path.exists(TRAFFIC_FILE):
+    bound = "hbm"
+    if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:
             measured = json.load(fh)
-        # HBM bytes per launch of the kernel this run used, if profiled
+        # HBM bytes per launch, VALU busy and binding resource of the kernel
+        # this run used, as profiled (rocprofv3 PMC runs of the same command)
         for entry in measured.get("kernels", []):
-            if entry.get("launch") and entry["launch"] in head["launch"]:
+            if entry.get("launch") and entry.get("precision", "fp64") == a.precision and \
+                    head["launch"].startswith(entry["launch"]):
                 traffic = entry["bytes_per_launch"]
                 valu_busy = entry.get("valu_busy")
+                bound = entry.get("bound", "hbm")
+                break
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
     achieved = head["achieved_GBs"]
+    spl = head["steps_per_launch"]
+    # the collisions' FP64 vector instructions per second in the kernel
+    # (algorithmic, halo recomputation excluded) against the FP64 issue peak
+    fp64_t = head["kernel_MLUPS"] * 1e6 * FP64_INSTR_PER_NODE_STEP / 1e12
     line = {
         "metric": "D2Q9 FP64 MLUPS (%HBM peak); map points/s + p99 latency, FP32 vs FP64",
         "value": head["MLUPS"],
@@ -361,23 +386,25 @@ def main() -> int:
                                f"tau=0.5024, tau_nh=0.65, {a.precision} row-blocked pull-stream+collide",
                    "L": L, "tau": TAU, "tau_nh": TAU_NH, "collision": "E-RLBM", "precision": a.precision,
                    "launch": head["launch"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        # the dominant kernel: one launch advances `steps_per_launch` steps and
+        # reads + writes every population once (algorithmic bytes), its
+        # duration event-timed over 100 whole launches on the lattice's stream
+        "roofline": {"bound": bound, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": head["launch"], "steps_per_launch": head["steps_per_launch"],
+                     "kernel": head["launch"], "steps_per_launch": spl,
                      "algorithmic_bytes_per_launch": BYTES_PER_NODE[a.precision] * L * L,
                      "ms_per_launch": head["ms_per_launch"],
-                     # what a one-step-per-launch kernel would have to stream for this MLUPS
-                     "one_step_equivalent_GBs": round(head["MLUPS"] * 1e6 * BYTES_PER_NODE[a.precision] / 1e9, 1),
-                     # the multi-step kernels trade HBM bytes for FP64 VALU work: the
-                     # profiled VALU busy fraction of the same kernel (rocprofv3)
-                     "valu_busy_profiled": valu_busy},
-        # the compute side of the same kernel: the collisions' FP64 vector
-        # instructions per second (algorithmic, halo recomputation excluded)
-        # against the FP64 vector issue peak
+                     "hbm_bytes_per_node_step": round(BYTES_PER_NODE[a.precision] / spl, 2),
+                     "one_step_kernel_bytes_per_node_step": BYTES_PER_NODE[a.precision],
+                     # what a one-step-per-launch kernel would have to stream for the kernel's rate
+                     "one_step_equivalent_GBs": round(head["kernel_MLUPS"] * 1e6 * BYTES_PER_NODE[a.precision] / 1e9, 1),
+                     "valu_busy_profiled": valu_busy,
+                     "fp64_issue_frac": round(fp64_t / FP64_VALU_PEAK_T, 4)},
         "fp64_valu": {"instr_per_node_step": FP64_INSTR_PER_NODE_STEP,
-                      "achieved_T_per_s": round(head["MLUPS"] * 1e6 * FP64_INSTR_PER_NODE_STEP / 1e12, 2),
+                      "achieved_T_per_s": round(fp64_t, 2),
                       "peak_T_per_s": FP64_VALU_PEAK_T,
-                      "frac": round(head["MLUPS"] * 1e6 * FP64_INSTR_PER_NODE_STEP / 1e12 / FP64_VALU_PEAK_T, 4)},
+                      "frac": round(fp64_t / FP64_VALU_PEAK_T, 4)},
+        "kernel_MLUPS": head["kernel_MLUPS"],
         "cpu_baseline": cpu,
         "copy_kernel_GBs": round(copy_gbs, 1),
         "mass_rel_error": head["mass_rel_error"],

@@ -888,24 +888,62 @@ __global__ __launch_bounds__(BLOCK) void k_monitor(MonArgs a)
     }
 }
 
-// Final reduction.  ordered: sequential sum of ke_node in fluid order (the
-// reference's loop); else: sum of the block partials in block order.
-// If check_step != ULLONG_MAX, applies the Taylor-Green map test
+// Final reduction, one workgroup.  ordered: the sum of ke_node in fluid
+// order, ((0 + e_0) + e_1) + ... as the reference's loop forms it: waves 1-3
+// stage the next chunk of ke_node into LDS (coalesced) while lane 0 of wave 0
+// adds the current one in order, so the sum runs at the FP64 add latency, not
+// the memory latency.  Else: a fixed-shape tree over the block partials
+// (thread t sums partials t, t + FIN_THREADS, ... in order; then the wave
+// tree; then the waves in order) -- reproducible, and not the reference's
+// rounding.  If check_step != NO_STOP, applies the Taylor-Green map test
 // (E > factor * E0 or non-finite -> stop before step check_step) with E0
 // read from *e0.  out[0] = E, out[1] = max |rho|.
-__global__ void k_finish(const double *ke_node, const double *partial_ke, const double *partial_rho,
-                         int n_fluid, int n_blocks, int ordered, double *out, const double *e0,
-                         double factor, unsigned long long check_step, unsigned long long *status)
+constexpr int FIN_THREADS = 256;
+constexpr int FIN_CHUNK = 2048;   // doubles per LDS stage (two stages: 32 KB)
+
+__global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *ke_node, const double *partial_ke,
+                                                        const double *partial_rho, int n_fluid, int n_blocks,
+                                                        int ordered, double *out, const double *e0, double factor,
+                                                        unsigned long long check_step, unsigned long long *status)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double e = 0;
+    __shared__ double stage[2][FIN_CHUNK];
+    __shared__ double red_e[FIN_THREADS / 64], red_r[FIN_THREADS / 64];
+    const int t = threadIdx.x;
+    double e = 0.0;
     if (ordered) {
-        for (int p = 0; p < n_fluid; p++) e = e + ke_node[p];
+        const int nchunk = (n_fluid + FIN_CHUNK - 1) / FIN_CHUNK;
+        for (int j = t; j < FIN_CHUNK && j < n_fluid; j += FIN_THREADS) stage[0][j] = ke_node[j];
+        __syncthreads();
+        for (int c = 0; c < nchunk; c++) {
+            if (t >= 64) {
+                const long long base = (long long)(c + 1) * FIN_CHUNK;
+                for (int j = t - 64; j < FIN_CHUNK && base + j < n_fluid; j += FIN_THREADS - 64)
+                    stage[(c + 1) & 1][j] = ke_node[base + j];
+            } else if (t == 0) {
+                const int m = min(FIN_CHUNK, n_fluid - c * FIN_CHUNK);
+                const double *b = stage[c & 1];
+                for (int j = 0; j < m; j++) e = e + b[j];
+            }
+            __syncthreads();
+        }
     } else {
-        for (int b = 0; b < n_blocks; b++) e = e + partial_ke[b];
+        for (int b = t; b < n_blocks; b += FIN_THREADS) e = e + partial_ke[b];
+        e = wave_sum(e);
     }
     double r = 0.0;
-    for (int b = 0; b < n_blocks; b++) r = fmax(r, partial_rho[b]);
+    for (int b = t; b < n_blocks; b += FIN_THREADS) r = fmax(r, partial_rho[b]);
+    r = wave_max(r);
+    if ((t & 63) == 0) {
+        red_e[t >> 6] = e;
+        red_r[t >> 6] = r;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    if (!ordered) {
+        e = red_e[0];
+        for (int w = 1; w < FIN_THREADS / 64; w++) e = e + red_e[w];
+    }
+    for (int w = 1; w < FIN_THREADS / 64; w++) r = fmax(r, red_r[w]);
     out[0] = e;
     out[1] = r;
     if (check_step != NO_STOP) {
@@ -977,7 +1015,7 @@ int steps_run(erlbm_lattice *h, long long *out, int *reason)
 
 int launch_finish(erlbm_lattice *h, bool ordered, double *out_dev, unsigned long long check_step, double factor)
 {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, h->stream, h->ke_node, h->partial_ke, h->partial_rho,
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FIN_THREADS), 0, h->stream, h->ke_node, h->partial_ke, h->partial_rho,
                        h->n_fluid, mon_blocks(h), ordered ? 1 : 0, out_dev, h->scalars + 2, factor, check_step,
                        h->status);
     ERLBM_HIP_TRY(hipGetLastError());
@@ -1981,6 +2019,22 @@ int dispatch_step(erlbm_lattice *h, const StepArgs &a, bool stream)
 
 }  // namespace
 
+namespace {
+
+// Two-step shape for a remainder of two steps after three-step launches (the
+// two-step strip walk, else the 64x8 tiles, else the wavefront strips, where
+// they fill the chip): one launch instead of two one-step launches (L = 2048:
+// ~116 us against ~192 us), bitwise the same steps.  0: none.
+int remainder2_shape(const erlbm_lattice *h)
+{
+    if (fuse_steps(h->fuse_active) != 3) return 0;
+    for (int f : {FUSE2_STRIP_256x32, 1, 16})
+        if (fuse_shape_fits(h, f) && fuse_workgroups(h, FUSE_SHAPES[f]) >= h->n_cu) return f;
+    return 0;
+}
+
+}  // namespace
+
 extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, double energy_factor)
 {
     if (!h || n_steps < 0) return ERLBM_ERR_ARG;
@@ -1993,6 +2047,7 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
     h->fuse_active = fuse_shape(h, energy_interval);
     const bool fuse = h->fuse_active > 0;
     const int spl = fuse_steps(h->fuse_active);   // steps per fused launch
+    const int rem2 = fuse ? remainder2_shape(h) : 0;
     for (int s = 0; s < n_steps;) {
         const long long k = h->launched;
         const int cur = (h->base + (int)(k & 1)) & 1;
@@ -2013,6 +2068,17 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
             h->launched += spl;
             if (spl == 2) h->base ^= 1;
             s += spl;
+            continue;
+        }
+        if (rem2 && k > 0 && n_steps - s == 2) {
+            const int keep = h->fuse_active;
+            h->fuse_active = rem2;
+            int rc = launch_fused(h, a);
+            h->fuse_active = keep;
+            if (rc) return rc;
+            h->launched += 2;
+            h->base ^= 1;
+            s += 2;
             continue;
         }
         int rc = dispatch_step(h, a, k > 0);

@@ -599,3 +599,32 @@ def test_d3q19_zwalk_monitors_equal_one_step(gpu, monkeypatch):
             assert np.array_equal(fused.download(gpu.POST_COLLISION), ref.download(gpu.POST_COLLISION))
             assert fused.kinetic_energy(gpu.POST_COLLISION) == ref.kinetic_energy(gpu.POST_COLLISION)
             assert fused.kinetic_energy(gpu.STATE) == ref.kinetic_energy(gpu.STATE)
+
+
+def test_two_step_remainder_bitwise(gpu, orc, monkeypatch):
+    """Calls whose step count leaves two steps after the three-step launches
+    run them as one two-step launch: at L = 2048, steps 4 (1 + 3), 5 (3 + 2),
+    2 (a lone two-step launch) and 1, bitwise the strict oracle (FP64) and
+    the one-step path (FP32 shifted storage)."""
+    L, calls = 2048, (4, 5, 2, 1)
+    tau = 0.5 + 3. * (1. / 32.) * L / 80000.
+    f0 = gpu.init_double_shear(L)
+    ini, st, nf = orc.stream_map(9, L, L, 1)
+    want = orc.run_steps(9, f0, st, sum(calls), O.COL_EXT_REG, tau, 0.65, nthreads=16)
+    with _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0) as lat:
+        assert lat.step_shape()[1] == 3
+        for n in calls:
+            lat.step(n)
+        assert lat.status() == (sum(calls), 0)
+        assert np.array_equal(lat.download(), want)
+    out = []
+    for fuse in (None, "0"):
+        if fuse is not None:
+            monkeypatch.setenv("ERLBM_FUSE", fuse)
+        with gpu.Lattice(9, L, L, 1, precision=gpu.FP32_SHIFTED) as lat:
+            lat.set_params(tau, 0.65, 1.0, O.COL_EXT_REG)
+            lat.upload(f0)
+            for n in calls:
+                lat.step(n)
+            out.append(lat.download())
+    assert np.array_equal(out[0], out[1])
