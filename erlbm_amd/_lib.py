@@ -27,6 +27,7 @@ _vp = C.c_void_p
 _i, _d, _ll = C.c_int, C.c_double, C.c_longlong
 
 INIT_REST, INIT_TAYLOR_GREEN = 0, 1
+NOALLOC_MAX_JOBS = 16384   # ERLBM_NOALLOC_MAX_JOBS
 FP_AUTO = 3
 
 
@@ -104,6 +105,7 @@ SIGNATURES = {
     "erlbm_batch_create": (_i, [_i, _i, _vp, _i, _i, _vp, C.POINTER(_vp)]),
     "erlbm_batch_destroy": (_i, [_vp]),
     "erlbm_batch_run": (_i, [_vp, _vp, _i, _vp]),
+    "erlbm_batch_run_noalloc": (_i, [_vp, _vp, _i, _vp]),
     "erlbm_batch_set_launch": (_i, [_vp, _i, _i]),
     "erlbm_batch_set_chunk": (_i, [_vp, _i]),
     "erlbm_lattice_autotune": (_i, [_vp, _i, C.c_char_p, _i, C.POINTER(C.c_float)]),

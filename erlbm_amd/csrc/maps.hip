@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -835,10 +836,11 @@ int run_chunked(erlbm_batch *b, const erlbm_job *jobs, int n_jobs, const std::ve
 
 }  // namespace
 
-extern "C" int erlbm_batch_run(erlbm_batch *b, const erlbm_job *jobs, int n_jobs, erlbm_job_result *results)
+namespace {
+
+// The checks erlbm_batch_run applies to a job list (ERLBM_OK or the error).
+int validate_jobs(const erlbm_batch *b, const erlbm_job *jobs, int n_jobs)
 {
-    if (!b || (n_jobs > 0 && (!jobs || !results)) || n_jobs < 0) return ERLBM_ERR_ARG;
-    if (n_jobs == 0) return ERLBM_OK;
     for (int k = 0; k < n_jobs; k++) {
         const MapJob &j = jobs[k];
         if (!(j.tau > 0) || !(j.tau_nh > 0) || j.n_steps < 0 || j.n_steps > 2000000000LL ||
@@ -858,6 +860,16 @@ extern "C" int erlbm_batch_run(erlbm_batch *b, const erlbm_job *jobs, int n_jobs
             return ERLBM_ERR_ARG;
         }
     }
+    return ERLBM_OK;
+}
+
+}  // namespace
+
+extern "C" int erlbm_batch_run(erlbm_batch *b, const erlbm_job *jobs, int n_jobs, erlbm_job_result *results)
+{
+    if (!b || (n_jobs > 0 && (!jobs || !results)) || n_jobs < 0) return ERLBM_ERR_ARG;
+    if (n_jobs == 0) return ERLBM_OK;
+    if (int rc = validate_jobs(b, jobs, n_jobs)) return rc;
     if (n_jobs > b->cap) {
         (void)hipFree(b->d_jobs);
         (void)hipFree(b->d_results);
@@ -918,6 +930,126 @@ extern "C" int erlbm_batch_run(erlbm_batch *b, const erlbm_job *jobs, int n_jobs
     }
     ERLBM_HIP_TRY(hipMemcpyAsync(results, b->d_results, sizeof(MapResult) * n_jobs, hipMemcpyDeviceToHost,
                                  b->stream));
+    ERLBM_HIP_TRY(hipStreamSynchronize(b->stream));
+    return ERLBM_OK;
+}
+
+// ---- batch run without allocation ------------------------------------------------
+// erlbm_batch_run's device-side job records, results, queue order and queue
+// counters in one static device array (`scratch`) instead of buffers the batch
+// allocates: NOALLOC_MAX_JOBS jobs at most.  Host code uses the array through
+// the address hipGetSymbolAddress gives for the current device (each GPU holds
+// its own copy), never through the symbol's name.
+namespace {
+
+constexpr int NOALLOC_MAX_JOBS = ERLBM_NOALLOC_MAX_JOBS;
+constexpr int NOALLOC_GROUPS = 16;   // (precision, collision) groups of one call
+constexpr size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+constexpr size_t SCR_JOBS = 0;
+constexpr size_t SCR_RESULTS = align256(SCR_JOBS + sizeof(MapJob) * NOALLOC_MAX_JOBS);
+constexpr size_t SCR_IDX = align256(SCR_RESULTS + sizeof(MapResult) * NOALLOC_MAX_JOBS);
+constexpr size_t SCR_COUNTERS = align256(SCR_IDX + sizeof(int) * NOALLOC_MAX_JOBS);
+constexpr size_t SCRATCH_BYTES = align256(SCR_COUNTERS + sizeof(unsigned int) * NOALLOC_GROUPS);
+
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, taken on the first call
+// per device.
+int scratch_address(unsigned char **out)
+{
+    int dev = 0;
+    ERLBM_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        set_last_error_msg("erlbm_batch_run_noalloc: device index out of range");
+        return ERLBM_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_last_error("erlbm_batch_run_noalloc: hipGetSymbolAddress(scratch)", e);
+            return ERLBM_ERR_HIP;
+        }
+        g_scratch_dev[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return ERLBM_OK;
+}
+
+// Clears the results of n jobs and the queue counters (a kernel, so that a
+// caller may capture the call in a graph).
+__global__ void k_clear_noalloc(MapResult *results, int n, unsigned int *counters)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) results[i] = MapResult{};
+    if (i < NOALLOC_GROUPS) counters[i] = 0u;
+}
+
+}  // namespace
+
+extern "C" int erlbm_batch_run_noalloc(erlbm_batch *b, const erlbm_job *jobs, int n_jobs,
+                                       erlbm_job_result *results)
+{
+    if (!b || (n_jobs > 0 && (!jobs || !results)) || n_jobs < 0) return ERLBM_ERR_ARG;
+    if (n_jobs > NOALLOC_MAX_JOBS) {
+        set_last_error_msg("erlbm_batch_run_noalloc: more than ERLBM_NOALLOC_MAX_JOBS (16384) jobs");
+        return ERLBM_ERR_ARG;
+    }
+    if (n_jobs == 0) return ERLBM_OK;
+    if (int rc = validate_jobs(b, jobs, n_jobs)) return rc;
+    unsigned char *base = nullptr;
+    if (int rc = scratch_address(&base)) return rc;
+    MapJob *d_jobs = reinterpret_cast<MapJob *>(base + SCR_JOBS);
+    MapResult *d_results = reinterpret_cast<MapResult *>(base + SCR_RESULTS);
+    int *d_idx = reinterpret_cast<int *>(base + SCR_IDX);
+    unsigned int *d_counters = reinterpret_cast<unsigned int *>(base + SCR_COUNTERS);
+    // the queue order of erlbm_batch_run -- grouped by (precision,
+    // collision), longest budgets first, ties by job index -- formed in a
+    // stack array (no heap allocation)
+    int idx[NOALLOC_MAX_JOBS];
+    auto key = [&](int k) { return resolve_precision(jobs[k]) * 8 + jobs[k].collision; };
+    for (int k = 0; k < n_jobs; k++) idx[k] = k;
+    std::sort(idx, idx + n_jobs, [&](int x, int y) {
+        if (key(x) != key(y)) return key(x) < key(y);
+        if (jobs[x].n_steps != jobs[y].n_steps) return jobs[x].n_steps > jobs[y].n_steps;
+        return x < y;
+    });
+    ERLBM_HIP_TRY(hipMemcpyAsync(d_jobs, jobs, sizeof(MapJob) * n_jobs, hipMemcpyHostToDevice, b->stream));
+    ERLBM_HIP_TRY(hipMemcpyAsync(d_idx, idx, sizeof(int) * n_jobs, hipMemcpyHostToDevice, b->stream));
+    const int clear = std::max(n_jobs, NOALLOC_GROUPS);
+    hipLaunchKernelGGL(k_clear_noalloc, dim3((clear + 255) / 256), dim3(256), 0, b->stream, d_results, n_jobs,
+                       d_counters);
+    ERLBM_HIP_TRY(hipGetLastError());
+    RunArgs a;
+    fill_geom(b, a.g);
+    a.jobs = d_jobs;
+    a.results = d_results;
+    a.n_jobs_dev = nullptr;
+    a.next_job = nullptr;
+    a.max_steps = 0;
+    a.saved = nullptr;
+    int start = 0, group = 0;
+    while (start < n_jobs) {
+        int end = start;
+        while (end < n_jobs && key(idx[end]) == key(idx[start])) end++;
+        if (group >= NOALLOC_GROUPS) return ERLBM_ERR_UNSUPPORTED;
+        a.job_idx = d_idx + start;
+        a.n_jobs = end - start;
+        const int p = resolve_precision(jobs[idx[start]]);
+        const int col = jobs[idx[start]].collision;
+        int rc = (p == ERLBM_FP64)            ? launch_col<F64>(b, col, a, d_counters + group)
+                 : (p == ERLBM_FP32_SHIFTED) ? launch_col<F32>(b, col, a, d_counters + group)
+                                             : launch_col<F32A>(b, col, a, d_counters + group);
+        if (rc) return rc;
+        start = end;
+        group++;
+    }
+    ERLBM_HIP_TRY(hipMemcpyAsync(results, d_results, sizeof(MapResult) * n_jobs, hipMemcpyDeviceToHost, b->stream));
     ERLBM_HIP_TRY(hipStreamSynchronize(b->stream));
     return ERLBM_OK;
 }

@@ -74,6 +74,15 @@ class Batch:
                                     res.ctypes.data_as(C.c_void_p)), "erlbm_batch_run")
         return res
 
+    def run_noalloc(self, jobs: np.ndarray) -> np.ndarray:
+        """run() through erlbm_batch_run_noalloc: no device allocation, at
+        most NOALLOC_MAX_JOBS jobs."""
+        jobs = np.ascontiguousarray(jobs, dtype=JOB_DTYPE)
+        res = np.zeros(jobs.size, dtype=RESULT_DTYPE)
+        check(lib().erlbm_batch_run_noalloc(self._h, jobs.ctypes.data_as(C.c_void_p), jobs.size,
+                                            res.ctypes.data_as(C.c_void_p)), "erlbm_batch_run_noalloc")
+        return res
+
 
 def tg_map_jobs(n_ma=400, n_nu=400, collision=L.COL_EXT_REG, tau_nh=0.65, precision=L.FP64, i0=0, i1=None, j0=0,
                 j1=None) -> np.ndarray:

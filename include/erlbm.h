@@ -275,6 +275,18 @@ int erlbm_batch_create(int nx, int ny, const int *mask, int inlet_x, int outlet_
 int erlbm_batch_destroy(erlbm_batch *b);
 /* Runs jobs[0..n_jobs) to retirement; results[k] for jobs[k].  Synchronous. */
 int erlbm_batch_run(erlbm_batch *b, const erlbm_job *jobs, int n_jobs, erlbm_job_result *results);
+/* No counterpart in the reference: erlbm_batch_run for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same jobs, same kernels, identical
+ * results; the call itself allocates nothing: the device-side job records,
+ * results, queue order and queue counters live in a static device array of
+ * the library, about 1.9 MB, that each GPU the library runs on holds until
+ * the process ends.  At most ERLBM_NOALLOC_MAX_JOBS jobs per call
+ * (ERLBM_ERR_ARG beyond).  Runs in the one-launch mode whatever
+ * erlbm_batch_set_chunk says (the results are the same).  Calls that overlap
+ * on one GPU share that array and race: serialise them.  Synchronous.     */
+enum { ERLBM_NOALLOC_MAX_JOBS = 16384 };
+int erlbm_batch_run_noalloc(erlbm_batch *b, const erlbm_job *jobs, int n_jobs, erlbm_job_result *results);
 /* Launch shape of the persistent kernels: workgroups per CU for the FP64 and
  * FP32 kernels (0 = occupancy maximum).  Chosen by the autotuner.       */
 int erlbm_batch_set_launch(erlbm_batch *b, int wg_per_cu_f64, int wg_per_cu_f32);
