@@ -214,26 +214,41 @@ def d3q19_section(E, n: int = 256, steps: int = 200):
 
 
 def maps_section(E):
-    """Full 400 x 400 stability maps (FP64, batched) vs the plain one-run-per-
-    launch FP64 path on a sample, and agreement with the published maps."""
+    """Full 400 x 400 stability maps (the three published ones in FP64, two
+    again in FP32-shifted storage, batched) vs the plain one-run-per-launch
+    FP64 path on a sample; agreement with the published maps and of FP32
+    with FP64."""
     d = np.load(os.path.join(ROOT, "tests", "golden", "published_maps.npz"), allow_pickle=False)
     pub = {k: np.unpackbits(d[k])[: 400 * 400].reshape(400, 400) for k in d.files}
     out = {}
-    for name, mk, jobs, nf in (("tg_erlbm065", E.tg_batch, E.tg_map_jobs(400, 400, E.COL_EXT_REG, 0.65, E.FP64), 256),
-                               ("obstacle", E.obstacle_batch, E.obstacle_map_jobs(400, 400, 0.1, E.FP64), 232)):
+    fp64 = {}
+    for name, prec, mk, nf in (("tg_erlbm065", "fp64", E.tg_batch, 256), ("tg_reg", "fp64", E.tg_batch, 256),
+                               ("obstacle", "fp64", E.obstacle_batch, 232),
+                               ("tg_erlbm065", "fp32", E.tg_batch, 256), ("obstacle", "fp32", E.obstacle_batch, 232)):
+        # fp32: FP32-shifted storage, FP64 arithmetic, FP64 below tau - 1/2 < 1e-3 (ERLBM_FP_AUTO)
+        p = E.FP64 if prec == "fp64" else E.FP_AUTO
+        jobs = (E.obstacle_map_jobs(400, 400, 0.1, p) if name == "obstacle" else
+                E.tg_map_jobs(400, 400, E.COL_REG if name == "tg_reg" else E.COL_EXT_REG,
+                              1.0 if name == "tg_reg" else 0.65, p))
         with mk() as b:
             pick = b.autotune(np.random.default_rng(3).choice(jobs, 4096, replace=False), max_steps=2000)
             t = time.perf_counter()
             res = b.run(jobs)
             dt = time.perf_counter() - t
         agree = float(np.mean(res["unstable"] == pub[name].reshape(-1)))
-        out[name] = {"points": int(jobs.size), "seconds": round(dt, 3), "points_per_s": round(jobs.size / dt, 1),
-                     "node_steps_per_s": float(np.sum(res["steps"].astype(float)) * nf / dt),
-                     # the same as D3Q19 MLUPS (fluid node updates per microsecond), against
-                     # 26,316 MLUPS for a kernel streaming 304 B/node-step at 8 TB/s
-                     "MLUPS": round(float(np.sum(res["steps"].astype(float)) * nf / dt) / 1e6, 1),
-                     "unstable": int(res["unstable"].sum()), "agreement_vs_published_data": round(agree, 5),
-                     "autotune": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in pick.items()}}
+        r = {"points": int(jobs.size), "seconds": round(dt, 3), "points_per_s": round(jobs.size / dt, 1),
+             "node_steps_per_s": float(np.sum(res["steps"].astype(float)) * nf / dt),
+             # the same as D3Q19 MLUPS (fluid node updates per microsecond), against
+             # 26,316 MLUPS for a kernel streaming 304 B/node-step at 8 TB/s
+             "MLUPS": round(float(np.sum(res["steps"].astype(float)) * nf / dt) / 1e6, 1),
+             "unstable": int(res["unstable"].sum()), "agreement_vs_published_data": round(agree, 5),
+             "autotune": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in pick.items()}}
+        if prec == "fp64":
+            fp64[name] = res["unstable"]
+            out[name] = r
+        else:
+            r["agreement_vs_fp64"] = round(float(np.mean(res["unstable"] == fp64[name])), 5)
+            out[name + "_fp32_storage"] = r
     # the plain FP64 path, one run at a time, on a random sample of points
     from erlbm_amd import serve as S
     rng = np.random.default_rng(5)

@@ -175,8 +175,8 @@ struct F32A {
     __device__ static void init(float (&d)[Q], double rho0, double ux, double uy) { F32::init(d, rho0, ux, uy); }
     __device__ static double ke(const float (&d)[Q], double rho0)
     {
-        float drho, rho, vx, vy, vz;
-        moments_shifted<Q>(d, (float)rho0, drho, rho, vx, vy, vz);
+        float drho, rho, vx, vy, vz, jx, jy, jz;
+        moments_shifted<Q>(d, (float)rho0, drho, rho, vx, vy, vz, jx, jy, jz);
         const double x = vx, y = vy, z = vz;
         return 0.5 * (x * x + y * y + z * z);
     }

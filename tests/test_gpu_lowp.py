@@ -20,11 +20,11 @@ def _l2(u, ref):
     return float(np.sqrt(np.sum((u - ref) ** 2) / np.sum(ref ** 2)))
 
 
-@pytest.mark.parametrize("prec,tol", [("FP32_SHIFTED", 1e-4), ("FP32_SHIFTED_F32", 1e-4), ("FP16_SHIFTED", 5e-2)])
+@pytest.mark.parametrize("prec,tol", [("FP32_SHIFTED", 1e-6), ("FP32_SHIFTED_F32", 1e-5), ("FP16_SHIFTED", 5e-2)])
 def test_double_shear_shifted_vs_fp64(gpu, orc, prec, tol):
     """D2Q9 double shear layer L=256, 1000 steps: relative L2 velocity error
-    against the FP64 oracle below tol (FP32 storage: 1e-4, FP16: 5e-2; FP32
-    storage with FP32 arithmetic measured 3.2e-5)."""
+    against the FP64 oracle below tol (FP32 storage: 1e-6, measured 2.1e-7;
+    FP32 arithmetic: 1e-5, measured 2.95e-6; FP16: 5e-2)."""
     L, n = 256, 1000
     tau = 0.5 + 3. * (1. / 32.) * L / 80000.
     f0 = gpu.init_double_shear(L)
@@ -47,11 +47,11 @@ def test_double_shear_shifted_vs_fp64(gpu, orc, prec, tol):
     assert abs(rho.sum() - rhow.sum()) / rhow.sum() < (1e-7 if prec.startswith("FP32") else 1e-3)
 
 
-@pytest.mark.parametrize("prec,tol", [("FP32_SHIFTED", 0.01), ("FP16_SHIFTED", None), ("FP32_SHIFTED_F32", None)])
+@pytest.mark.parametrize("prec,tol", [("FP32_SHIFTED", 0.01), ("FP16_SHIFTED", None), ("FP32_SHIFTED_F32", 0.01)])
 def test_taylor_green_shifted_error(gpu, orc, prec, tol):
-    """Config: TG D2Q9 L=64, E-RLBM 0.65, t=T.  FP32-shifted storage (FP64
-    arithmetic) keeps E_vector within 1% of the FP64 value 0.000298896; the
-    FP16 and FP32-arithmetic variants are measured and reported."""
+    """Config: TG D2Q9 L=64, E-RLBM 0.65, t=T.  FP32-shifted storage, with
+    FP64 arithmetic or with FP32 arithmetic, keeps E_vector within 1% of the
+    FP64 value 0.000298896; the FP16 variant is measured and reported."""
     L = 64
     u0 = 2.0 / L
     visc = u0 * L / 50.0

@@ -118,9 +118,11 @@ def test_obstacle_points_match_oracle_steps(gpu, orc):
 def test_fp32_shifted_agreement(gpu, golden, prec_name, floor):
     """FP32-shifted storage (FP_AUTO: FP64 arithmetic, FP64 storage below
     tau - 1/2 < 1e-3; FP32_SHIFTED_F32: FP32 arithmetic everywhere) against
-    the strict reference classification on the subsample (measured, bounded
-    below; measured 1.000 / 0.998 / 0.963 for FP_AUTO and 0.905 for the
-    Taylor-Green E-RLBM map in FP32 arithmetic)."""
+    the strict reference classification on the subsample: a regression floor
+    for the non-default formats, not a parity bar (measured 1.000 / 0.998 /
+    0.963 for FP_AUTO; 0.92 / 0.96 / 0.955 in FP32 arithmetic).  The
+    reference's own strict and FMA builds disagree on 0.5-1.25% of these
+    points: stability-edge points whose side is set by round-off."""
     g = golden("ref_map_subsample.npz")
     prec = getattr(gpu, prec_name)
     for key, kind, col, tnh in (("tg_erlbm065", "tg", O.COL_EXT_REG, 0.65), ("tg_reg", "tg", O.COL_REG, 1.0),
@@ -209,3 +211,19 @@ def test_batch_edge_cases(gpu):
         bad["collision"] = O.COL_MRT          # D2Q9-only operator on a D3Q19 batch
         with pytest.raises(RuntimeError):
             b.run(bad)
+
+
+def test_fp_auto_taylor_green_full_map_agreement(gpu):
+    """The full 400 x 400 Taylor-Green E-RLBM map with FP32-shifted storage
+    where tau - 1/2 >= 1e-3 and FP64 below (ERLBM_FP_AUTO) classifies at least
+    99.9% of the points as the FP64 map does (measured 0.99952)."""
+    out = {}
+    for name, prec in (("fp64", gpu.FP64), ("auto", gpu.FP_AUTO)):
+        jobs = gpu.tg_map_jobs(400, 400, O.COL_EXT_REG, 0.65, prec)
+        with gpu.tg_batch() as b:
+            out[name] = b.run(jobs)
+    agree = float(np.mean(out["auto"]["unstable"] == out["fp64"]["unstable"]))
+    n32 = int(np.sum(out["auto"]["precision"] != gpu.FP64))
+    print(f"TG E-RLBM full map, FP_AUTO vs FP64: agreement {agree:.5f}, {n32} points in FP32 storage", flush=True)
+    assert n32 > 70000
+    assert agree >= 0.999
