@@ -552,7 +552,12 @@ __global__ __launch_bounds__(64 * NW) void k_step3_gstrip(StepArgs a, int remap,
 template <int TX, int TY>
 constexpr int zwalk_threads() { return ((TX + 2) * (TY + 2) + 63) / 64 * 64; }
 
-template <int COL, int TX, int TY, int DIR>
+// NTP: which step-k loads carry the nontemporal hint (0 none, the default:
+// neighbouring tiles read a tile's halo lines again, and with the hint on
+// the c_x = 0 populations (1) the reads were 1.59x the algorithmic bytes
+// against 1.31x without, 1.200 against 1.100 ms per launch at 256^3;
+// 2: c_x = c_y = 0 only, 1.43x, 1.134 ms).
+template <int COL, int TX, int TY, int DIR, int NTP = 0>
 __device__ __forceinline__ void zwalk(const StepArgs &a, int x0, int y0, int za, int rows, double *ring)
 {
     using V = VSet<19>;
@@ -577,8 +582,9 @@ __device__ __forceinline__ void zwalk(const StepArgs &a, int x0, int y0, int za,
         const __amdgpu_buffer_rsrc_t rp[3] = {plane_buf(a.src, z + 1), plane_buf(a.src, z), plane_buf(a.src, z - 1)};
         static_for<19>([&](auto I) {
             constexpr int i = I;
-            f[i] = buf_load<V::cx[i] == 0 ? BUF_NT : 0>(rp[V::cz[i] + 1], (yo[V::cy[i] + 1] + xo[V::cx[i] + 1]) | oob1 | off,
-                                                        (int)(8 * i * a.g.si));
+            constexpr bool nt = NTP == 1 ? V::cx[i] == 0 : (NTP == 2 ? V::cx[i] == 0 && V::cy[i] == 0 : false);
+            f[i] = buf_load<nt ? BUF_NT : 0>(rp[V::cz[i] + 1], (yo[V::cy[i] + 1] + xo[V::cx[i] + 1]) | oob1 | off,
+                                             (int)(8 * i * a.g.si));
         });
     };
     auto slot = [&](int i, int z) { return zring_base<DIR>(i) + (z - za + 2) % zring_depth<DIR>(i); };
@@ -633,7 +639,7 @@ __device__ __forceinline__ void zwalk(const StepArgs &a, int x0, int y0, int za,
     }
 }
 
-template <int COL, int TX, int TY>
+template <int COL, int TX, int TY, int NTP = 0>
 __global__ __launch_bounds__((zwalk_threads<TX, TY>())) void k_step2_zwalk(StepArgs a, int remap, int nseg)
 {
     extern __shared__ double zring[];   // [ZRING_PLANES][(TX+2) (TY+2)]
@@ -645,8 +651,8 @@ __global__ __launch_bounds__((zwalk_threads<TX, TY>())) void k_step2_zwalk(StepA
     const int za = (int)((long long)seg * a.g.nz / nseg);
     const int rows = (int)((long long)(seg + 1) * a.g.nz / nseg) - za;
     const int x0 = (tile % tx) * TX, y0 = (tile / tx) * TY;
-    if (seg & 1) zwalk<COL, TX, TY, -1>(a, x0, y0, za, rows, zring);
-    else zwalk<COL, TX, TY, 1>(a, x0, y0, za, rows, zring);
+    if (seg & 1) zwalk<COL, TX, TY, -1, NTP>(a, x0, y0, za, rows, zring);
+    else zwalk<COL, TX, TY, 1, NTP>(a, x0, y0, za, rows, zring);
 }
 
 // Lane l of a wavefront receives the value of lane l-1 (WAVE_SHR1) or l+1
@@ -1162,6 +1168,7 @@ void set_layout(erlbm_lattice *h, int q, int nx, int ny, int nz, int precision)
     h->tune_fuse_remap = (int)env_ll("ERLBM_FUSE_REMAP", h->tune_fuse_remap);
     h->tune_fuse_slots = (int)env_ll("ERLBM_FUSE_SLOTS", h->tune_fuse_slots);
     h->tune_fuse_prio = (int)env_ll("ERLBM_FUSE_PRIO", h->tune_fuse_prio);
+    h->tune_zwalk_nt = (int)env_ll("ERLBM_ZWALK_NT", h->tune_zwalk_nt);
     if (h->tune_pad < 0 || h->tune_pad > (1 << 24)) h->tune_pad = 0;
     h->layout = (int)env_ll("ERLBM_LAYOUT", h->layout);
     const long long pad = (h->tune_pad + 63) / 64 * 64;
@@ -1890,11 +1897,15 @@ void launch_gstrip3_t(erlbm_lattice *h, const StepArgs &a)
 
 // Segments as many as make one workgroup per resident slot, at least 16
 // planes each.
-template <int COL, int TX, int TY>
+template <int COL, int TX, int TY, int NTP = 0>
 void launch_zwalk_t(erlbm_lattice *h, const StepArgs &a)
 {
+    if constexpr (TX == 64 && TY == 8 && NTP == 0) {   // the default tile: load-hint policy knob
+        if (h->tune_zwalk_nt == 1) return launch_zwalk_t<COL, TX, TY, 1>(h, a);
+        if (h->tune_zwalk_nt == 2) return launch_zwalk_t<COL, TX, TY, 2>(h, a);
+    }
     constexpr size_t lds = sizeof(double) * ZRING_PLANES * (TX + 2) * (TY + 2);
-    const void *k = reinterpret_cast<const void *>(&k_step2_zwalk<COL, TX, TY>);
+    const void *k = reinterpret_cast<const void *>(&k_step2_zwalk<COL, TX, TY, NTP>);
     static const int per_cu = [&] {
         if constexpr (lds > 65536) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         int nblk = 0;
@@ -1904,8 +1915,8 @@ void launch_zwalk_t(erlbm_lattice *h, const StepArgs &a)
     }();
     const int ntile = (h->nx / TX) * (h->ny / TY);
     const int nseg = (int)std::max(1ll, std::min<long long>(h->nz / 16, (long long)h->n_cu * per_cu / ntile));
-    hipLaunchKernelGGL((k_step2_zwalk<COL, TX, TY>), dim3((unsigned)(ntile * nseg)), dim3(zwalk_threads<TX, TY>()), lds,
-                       h->stream, a, h->tune_fuse_remap, nseg);
+    hipLaunchKernelGGL((k_step2_zwalk<COL, TX, TY, NTP>), dim3((unsigned)(ntile * nseg)), dim3(zwalk_threads<TX, TY>()),
+                       lds, h->stream, a, h->tune_fuse_remap, nseg);
 }
 
 template <int COL>

@@ -197,6 +197,8 @@ struct erlbm_lattice {
     int tune_fuse_slots = 0;  //   ERLBM_FUSE_SLOTS workgroups per CU the walks' segments are sized for (0: all that fit)
     int tune_fuse_prio = -1;  //   ERLBM_FUSE_PRIO issue priority by progress in the aligned walks (-1 auto, 0 off,
                               //                   1 equal quarters, 2 bands ending at 50 / 75 / 90%)
+    int tune_zwalk_nt = 0;    //   ERLBM_ZWALK_NT   nontemporal step-k loads of the 64x8 z-walk: 0 none (default),
+                              //                    1 c_x = 0, 2 c_x = c_y = 0
     bool checked = false;     // a stop test has run since the upload (no fused steps then)
     bool fuse_requested = false;   // ERLBM_FUSE set explicitly
     int fuse_active = 0;      // multi-step shape of the current erlbm_step call (0: none)
