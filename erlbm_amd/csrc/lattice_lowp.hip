@@ -415,7 +415,9 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_lp_step3_strip(LPArgs a
 // FP32 rings hold the other six (16 KB for four wavefronts); the values
 // kept, in LDS or registers, are the FP32 ones one-step launches would
 // store and read back.
-template <int COL, int NW, int DIR>
+// NTP 1: the c_x = 0 step-k loads carry the nontemporal hint; 0 (default):
+// none, so the halo rows neighbouring segments share stay in L2.
+template <int COL, int NW, int DIR, int NTP = 0>
 __device__ __forceinline__ void lp_strip3_walk_aligned(const LPArgs &a, int x0, int w, int ya, int rows, float *ring1,
                                                        float *ring2, int prio)
 {
@@ -435,7 +437,7 @@ __device__ __forceinline__ void lp_strip3_walk_aligned(const LPArgs &a, int x0, 
         static_for<9>([&](auto I) {
             constexpr int i = I;
             const int ys = V::cy[i] == 0 ? yw : wrap(yw - V::cy[i], ny);
-            d[i] = buf_load_f32<V::cx[i] == 0 ? BUF_NT : 0>(rs, vo[V::cx[i] + 1] | off, (int)(4 * at(a.g, i, ys, 0)));
+            d[i] = buf_load_f32<NTP == 1 && V::cx[i] == 0 ? BUF_NT : 0>(rs, vo[V::cx[i] + 1] | off, (int)(4 * at(a.g, i, ys, 0)));
         });
     };
     auto rrow = [&](int i, int y) { return xring_base<DIR>(i) + (y - ya + 2) % xring_depth<DIR>(i); };
@@ -516,7 +518,7 @@ __device__ __forceinline__ void lp_strip3_walk_aligned(const LPArgs &a, int x0, 
     }
 }
 
-template <int COL, int NW>
+template <int COL, int NW, int NTP = 0>
 __global__ __launch_bounds__(64 * NW) void k_lp_step3_astrip(LPArgs a, int remap, int nstrip, int nseg, int unit,
                                                             int prio)
 {
@@ -532,8 +534,8 @@ __global__ __launch_bounds__(64 * NW) void k_lp_step3_astrip(LPArgs a, int remap
     const int seg = (int)(b / (unsigned)nstrip);
     const int ya = (int)((long long)seg * a.g.ny / nseg);
     const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
-    if (seg & 1) lp_strip3_walk_aligned<COL, NW, -1>(a, x0, w, ya, rows, lp_rings, lp_rings + XRING_ROWS * E, prio);
-    else lp_strip3_walk_aligned<COL, NW, 1>(a, x0, w, ya, rows, lp_rings, lp_rings + XRING_ROWS * E, prio);
+    if (seg & 1) lp_strip3_walk_aligned<COL, NW, -1, NTP>(a, x0, w, ya, rows, lp_rings, lp_rings + XRING_ROWS * E, prio);
+    else lp_strip3_walk_aligned<COL, NW, 1, NTP>(a, x0, w, ya, rows, lp_rings, lp_rings + XRING_ROWS * E, prio);
 }
 
 // Strips of at most 252 positions, edges on 16-position units where the
@@ -557,8 +559,12 @@ void launch_lp_astrip(erlbm_lattice *h, const LPArgs &l)
     const int nseg = (int)std::max(1ll, std::min<long long>(h->ny / 16, (long long)h->n_cu * per_cu / nstrip));
     // issue priority by progress where two or more workgroups share a CU
     const int prio = h->tune_fuse_prio >= 0 ? h->tune_fuse_prio : (nstrip * nseg >= 2 * h->n_cu ? 2 : 0);
-    hipLaunchKernelGGL((k_lp_step3_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream, l,
-                       h->tune_fuse_remap, nstrip, nseg, unit, prio);
+    if (h->tune_walk_nt == 1)
+        hipLaunchKernelGGL((k_lp_step3_astrip<COL, NW, 1>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds,
+                           h->stream, l, h->tune_fuse_remap, nstrip, nseg, unit, prio);
+    else
+        hipLaunchKernelGGL((k_lp_step3_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds,
+                           h->stream, l, h->tune_fuse_remap, nstrip, nseg, unit, prio);
 }
 
 template <int COL, int TX, int SEG>
