@@ -2056,6 +2056,25 @@ int remainder2_shape(const erlbm_lattice *h)
 
 }  // namespace
 
+namespace {
+
+// The multi-step kernels do not read the stop word, so they run only while no
+// stop test can have stopped the run.  After calls with stop tests, a call
+// without them reads the status once (the tests queued before it have ended
+// by then): with no stop recorded, those kernels are allowed again.
+int refresh_checked(erlbm_lattice *h, int energy_interval)
+{
+    if (!h->checked || energy_interval > 0 || h->dens_limit > 0.0) return ERLBM_OK;
+    long long run = 0;
+    int reason = 0;
+    const int rc = steps_run(h, &run, &reason);
+    if (rc) return rc;
+    if (reason == 0) h->checked = false;
+    return ERLBM_OK;
+}
+
+}  // namespace
+
 extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, double energy_factor)
 {
     if (!h || n_steps < 0) return ERLBM_ERR_ARG;
@@ -2064,6 +2083,7 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
         return ERLBM_ERR_STATE;
     }
     StepArgs a = step_args(h);
+    if (int rc = refresh_checked(h, energy_interval)) return rc;
     if (energy_interval > 0 || h->dens_limit > 0.0) h->checked = true;
     h->fuse_active = fuse_shape(h, energy_interval);
     const bool fuse = h->fuse_active > 0;
@@ -2244,6 +2264,8 @@ std::string fuse_shape_desc(int f, int remap)
 extern "C" int erlbm_step_shape(erlbm_lattice *h, int *shape, int *steps_per_launch, char *desc, int desc_len)
 {
     if (!h) return ERLBM_ERR_ARG;
+    if (h->uploaded)
+        if (int rc = refresh_checked(h, 0)) return rc;
     const int f = fuse_shape(h, 0);
     if (shape) *shape = f;
     if (steps_per_launch) *steps_per_launch = f > 0 ? fuse_steps(f) : 1;

@@ -201,7 +201,7 @@ struct erlbm_lattice {
                               //                    1 c_x = 0, 2 c_x = c_y = 0
     int tune_walk_nt = 0;     //   ERLBM_WALK_NT    nontemporal step-k loads of the c_x = 0 populations in the
                               //                    four-wavefront lane-aligned walks, FP64 and FP32 (1), or none (0, default)
-    bool checked = false;     // a stop test has run since the upload (no fused steps then)
+    bool checked = false;     // a stop test ran and no later status read cleared it (no fused steps then)
     bool fuse_requested = false;   // ERLBM_FUSE set explicitly
     int fuse_active = 0;      // multi-step shape of the current erlbm_step call (0: none)
     int n_cu = 256;           // compute units of the device

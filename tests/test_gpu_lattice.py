@@ -303,6 +303,34 @@ def test_two_step_fused_kernel_bitwise(gpu, orc, monkeypatch, shape):
         assert np.array_equal(fused.download(), ref.download())
 
 
+def test_stop_tests_do_not_pin_single_steps(gpu, orc, monkeypatch):
+    """A call with an energy test runs one step per launch; a later call
+    without one goes back to the multi-step kernel when no stop was recorded
+    (bitwise the oracle), and leaves a stopped run where it stopped."""
+    L = 256
+    tau = 0.5 + 3. * (1. / 32.) * L / 80000.
+    f0 = gpu.init_double_shear(L)
+    ini, st, nf = orc.stream_map(9, L, L, 1)
+    monkeypatch.setenv("ERLBM_FUSE", "42")
+    with _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0) as lat:
+        assert lat.step_shape()[:2] == (42, 3)
+        lat.step(9, energy_interval=4, energy_factor=2.0)
+        assert lat.status() == (9, 0)
+        assert lat.step_shape()[:2] == (42, 3)
+        lat.step(20)
+        assert lat.status() == (29, 0)
+        assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, 29, O.COL_EXT_REG, tau, 0.65))
+    with _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0) as lat:
+        lat.step(9, energy_interval=4, energy_factor=0.5)   # E / E0 ~ 1 > 0.5: stops before step 4
+        assert lat.status() == (4, gpu.STOP_ENERGY)
+        stopped = lat.download()
+        assert np.array_equal(stopped, orc.run_steps(9, f0, st, 4, O.COL_EXT_REG, tau, 0.65))
+        assert lat.step_shape()[:2] == (0, 1)
+        lat.step(20)
+        assert lat.status() == (4, gpu.STOP_ENERGY)
+        assert np.array_equal(lat.download(), stopped)
+
+
 @pytest.mark.parametrize("col", [O.COL_BGK, O.COL_MRT, O.COL_REG, O.COL_EXT_REG])
 @pytest.mark.parametrize("shape,dims", [(16, (300, 77)), (18, (64, 5)), (17, (130, 33)), (18, (125, 64))])
 def test_wavefront_strips_ragged_bitwise(gpu, orc, monkeypatch, col, shape, dims):
