@@ -12,6 +12,9 @@
 //    +-0 term added to a running sum that starts at +0.0 never changes it;
 //  * opposite directions share cv*3 and ((0.5 cv) cv 3) 3, which are odd and
 //    even in cv under IEEE round-to-nearest;
+//  * 1/rho is the division's own sequence without its no-op scaling steps
+//    (recip_in_range), and the unforced momentum sums skip their leading 0.0 +
+//    (only the sign of a zero velocity differs, which no population sees);
 //  * the forcing term S (source(), LBM_functions.cpp:5645-5716) is +-0 at the
 //    zero body force every driver passes, so the unforced operator (FORCE =
 //    false) does not add it; with FORCE the equilibrium takes u + a/2, F = a rho
@@ -42,30 +45,61 @@ struct BodyForce {
     double factor;
 };
 
+// 1 / r, correctly rounded, for |r| in [2^-600, 2^600] (recip_range): the
+// steps of the IEEE FP64 division sequence gfx950 code runs for 1.0 / r
+// (reciprocal estimate, two Newton steps, remainder, final fused step)
+// without its scaling and special-case steps (v_div_scale, v_div_fixup),
+// which leave the quotient unchanged there: no operand or result comes near
+// the overflow or subnormal range.
+__device__ __forceinline__ bool recip_range(double r)
+{
+    const double ar = __builtin_fabs(r);
+    return ar >= 0x1p-600 && ar <= 0x1p+600;
+}
+
+__device__ __forceinline__ double recip_in_range(double r)
+{
+    const double y0 = __builtin_amdgcn_rcp(r);
+    const double y1 = __builtin_fma(y0, __builtin_fma(-r, y0, 1.0), y0);
+    const double y2 = __builtin_fma(y1, __builtin_fma(-r, y1, 1.0), y1);
+    return __builtin_fma(__builtin_fma(-r, y2, 1.0), y2, y2);
+}
+
 // rho, u as calcula() computes them (sums in velocity order from 0.0,
-// u = m * (1/rho), u = 0 when rho == 0).
-template <int Q>
+// u = m * (1/rho), u = 0 when rho == 0).  LEAD0 = false starts the momentum
+// sums at their first term instead of 0.0 + that term: the two differ only
+// in the sign of a zero sum, and a zero velocity component of either sign
+// gives the same equilibrium (it enters as 1 +- 3 c.u and through u.u), so
+// the unforced collision does not form the addition.
+template <int Q, bool LEAD0 = true>
 __device__ __forceinline__ void moments_exact(const double (&g)[Q], double &rho, double &vx,
                                               double &vy, double &vz)
 {
     using V = VSet<Q>;
     double r = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+    bool sx = LEAD0, sy = LEAD0, sz = LEAD0;   // sum started
+    auto acc = [](double &m, bool &st, double v, int sign) {
+        if (st) m = sign > 0 ? m + v : m - v;
+        else m = sign > 0 ? v : -v;
+        st = true;
+    };
     static_for<Q>([&](auto I) {
         constexpr int i = I;
         r = r + g[i];
-        if constexpr (V::cx[i] == 1) mx = mx + g[i];
-        if constexpr (V::cx[i] == -1) mx = mx - g[i];
-        if constexpr (V::cy[i] == 1) my = my + g[i];
-        if constexpr (V::cy[i] == -1) my = my - g[i];
-        if constexpr (V::cz[i] == 1) mz = mz + g[i];
-        if constexpr (V::cz[i] == -1) mz = mz - g[i];
+        if constexpr (V::cx[i] != 0) acc(mx, sx, g[i], V::cx[i]);
+        if constexpr (V::cy[i] != 0) acc(my, sy, g[i], V::cy[i]);
+        if constexpr (V::cz[i] != 0) acc(mz, sz, g[i], V::cz[i]);
     });
     rho = r;
-    if (r != 0.0) {
-        const double oor = 1.0 / r;
+    auto scale = [&](double oor) {
         vx = mx * oor;
         vy = my * oor;
         vz = (V::D == 3) ? mz * oor : 0.0;
+    };
+    if (recip_range(r)) {
+        scale(recip_in_range(r));
+    } else if (r != 0.0) {
+        scale(1.0 / r);
     } else {
         vx = 0.0;
         vy = 0.0;
@@ -210,15 +244,16 @@ __device__ __forceinline__ void collide_mrt_d2q9(double (&g)[9], const double (&
 }
 
 // One collision in place.  Returns nothing; g holds f* afterwards.
-// vx, vy, vz receive the pre-collision velocity (for monitors).  FORCE adds
-// the body force bf (the operators' acc_x, acc_y, acc_z arguments).
+// vx, vy, vz receive the pre-collision velocity (a zero component of the
+// unforced operator may carry either sign).  FORCE adds the body force bf
+// (the operators' acc_x, acc_y, acc_z arguments).
 template <int Q, int COL, bool FORCE = false>
 __device__ __forceinline__ void collide_exact(double (&g)[Q], const CollideConsts &k, double &vx,
                                               double &vy, double &vz, const BodyForce &bf = BodyForce{})
 {
     using V = VSet<Q>;
     double rho;
-    moments_exact<Q>(g, rho, vx, vy, vz);
+    moments_exact<Q, FORCE>(g, rho, vx, vy, vz);
     // velocity of the equilibrium: u + a/2 under a body force (coll_Ext_Reg
     // :1198-1200; coll_MRT keeps u, :637-640)
     double ux = vx, uy = vy, uz = vz;

@@ -303,6 +303,32 @@ def test_two_step_fused_kernel_bitwise(gpu, orc, monkeypatch, shape):
         assert np.array_equal(fused.download(), ref.download())
 
 
+@pytest.mark.parametrize("q,col", [(9, O.COL_EXT_REG), (9, O.COL_MRT), (9, O.COL_BGK), (19, O.COL_EXT_REG),
+                                   (19, O.COL_REG)])
+def test_extreme_density_scales_bitwise(gpu, orc, q, col):
+    """Sites whose populations are scaled by 2^e, e in [-1000, 1000]: one
+    collision equals the strict oracle's, bitwise, on both sides of the
+    short reciprocal sequence's range (|rho| in [2^-600, 2^600]) and with the
+    plain division beyond it."""
+    rng = np.random.default_rng(5)
+    dims = (16, 8, 1) if q == 9 else (8, 4, 4)
+    n = dims[0] * dims[1] * dims[2]
+    base = orc.dist_eq(q, 0.03, -0.02, 0.01 if q == 19 else 0.0, 1.0)
+    f = base[None] * (1.0 + 0.1 * rng.uniform(-1.0, 1.0, (n, q)))
+    e = rng.integers(-700, 701, n)
+    e[:10] = [-601, -600, -599, 599, 600, 601, -1000, 1000, 0, -1]
+    f = np.ldexp(f, e[:, None])
+    print(f"extreme density scales q={q} col={col}", flush=True)
+    with gpu.Lattice(q, *dims) as lat:
+        lat.set_params(0.6, 0.65, 1.0, col)
+        lat.upload(f)
+        lat.step(1)
+        got = lat.download(gpu.POST_COLLISION)
+    want = orc.collide(f, col, 0.6, 0.65)
+    assert np.isfinite(want).all()
+    assert np.array_equal(got, want)
+
+
 def test_stop_tests_do_not_pin_single_steps(gpu, orc, monkeypatch):
     """A call with an energy test runs one step per launch; a later call
     without one goes back to the multi-step kernel when no stop was recorded
