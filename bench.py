@@ -41,11 +41,12 @@ TAU_NH = 0.65
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_NODE = {"fp64": 144, "fp32": 72, "fp16": 36}
 # FP64 vector instructions of one D2Q9 E-RLBM collision in the walk kernels'
-# code (gfx950 ISA of the walking loop, per three collisions: 318 v_add_f64,
-# 177 v_mul_f64, and 39 for three IEEE divisions and the rho == 0 tests),
-# and the FP64 vector issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
-# (the 78.6 TFLOP/s FP64 vector figure counts an FMA as two)
-FP64_INSTR_PER_NODE_STEP = 178
+# code (gfx950 ISA of the walking loop, per three collisions as executed:
+# 312 v_add_f64, 180 v_mul_f64, 6 v_ldexp_f64, and 27 for the three
+# reciprocals 1/rho with their range tests), and the FP64 vector issue peak:
+# 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (the 78.6 TFLOP/s FP64 vector
+# figure counts an FMA as two)
+FP64_INSTR_PER_NODE_STEP = 175
 FP64_VALU_PEAK_T = 39.3
 # rocprofv3 PMC traffic per launch (FETCH_SIZE + WRITE_SIZE, calibrated against
 # a plain copy kernel), measured on the FP64 step kernel

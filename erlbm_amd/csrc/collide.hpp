@@ -70,8 +70,9 @@ __device__ __forceinline__ double recip_in_range(double r)
 // sums at their first term instead of 0.0 + that term: the two differ only
 // in the sign of a zero sum, and a zero velocity component of either sign
 // gives the same equilibrium (it enters as 1 +- 3 c.u and through u.u), so
-// the unforced collision does not form the addition.
-template <int Q, bool LEAD0 = true>
+// the unforced collision does not form the addition.  SHORT_RECIP: 1/rho by
+// recip_in_range where it applies, else the division.
+template <int Q, bool LEAD0 = true, bool SHORT_RECIP = true>
 __device__ __forceinline__ void moments_exact(const double (&g)[Q], double &rho, double &vx,
                                               double &vy, double &vz)
 {
@@ -96,7 +97,7 @@ __device__ __forceinline__ void moments_exact(const double (&g)[Q], double &rho,
         vy = my * oor;
         vz = (V::D == 3) ? mz * oor : 0.0;
     };
-    if (recip_range(r)) {
+    if (SHORT_RECIP && recip_range(r)) {
         scale(recip_in_range(r));
     } else if (r != 0.0) {
         scale(1.0 / r);
@@ -247,13 +248,17 @@ __device__ __forceinline__ void collide_mrt_d2q9(double (&g)[9], const double (&
 // vx, vy, vz receive the pre-collision velocity (a zero component of the
 // unforced operator may carry either sign).  FORCE adds the body force bf
 // (the operators' acc_x, acc_y, acc_z arguments).
-template <int Q, int COL, bool FORCE = false>
+// SHORT = false keeps the moments exactly as calcula() forms them (the
+// division for 1/rho, sums from 0.0): the batched tiny-lattice kernel, where
+// the short forms measured slower (obstacle map 8.47k against 8.68k points/s;
+// the streaming walks: 94.3-94.7k against 93.9-94.1k MLUPS).
+template <int Q, int COL, bool FORCE = false, bool SHORT = true>
 __device__ __forceinline__ void collide_exact(double (&g)[Q], const CollideConsts &k, double &vx,
                                               double &vy, double &vz, const BodyForce &bf = BodyForce{})
 {
     using V = VSet<Q>;
     double rho;
-    moments_exact<Q, FORCE>(g, rho, vx, vy, vz);
+    moments_exact<Q, FORCE || !SHORT, SHORT>(g, rho, vx, vy, vz);
     // velocity of the equilibrium: u + a/2 under a body force (coll_Ext_Reg
     // :1198-1200; coll_MRT keeps u, :637-640)
     double ux = vx, uy = vy, uz = vz;

@@ -90,14 +90,14 @@ struct F64 {
     __device__ static double ke(const double (&f)[Q], double)
     {
         double rho, vx, vy, vz;
-        moments_exact<Q>(f, rho, vx, vy, vz);
+        moments_exact<Q, true, false>(f, rho, vx, vy, vz);
         return ke_term<Q>(vx, vy, vz);
     }
     template <int COL>
     __device__ static void collide(double (&f)[Q], const K &k, double)
     {
         double vx, vy, vz;
-        collide_exact<Q, COL>(f, k.c, vx, vy, vz);
+        collide_exact<Q, COL, false, false>(f, k.c, vx, vy, vz);
     }
     __device__ static double density(const double (&f)[Q], double) { return density_exact<Q>(f); }
     __device__ static void to_abs(const double (&f)[Q], double (&a)[Q], double)
@@ -145,7 +145,7 @@ struct F32 {
         double f[Q];
         to_abs(d, f, rho0);
         double vx, vy, vz;
-        collide_exact<Q, COL>(f, k.c, vx, vy, vz);
+        collide_exact<Q, COL, false, false>(f, k.c, vx, vy, vz);
         from_abs(f, d, rho0);
     }
     __device__ static double density(const float (&d)[Q], double rho0)
