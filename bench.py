@@ -81,13 +81,15 @@ def cpu_baseline(seconds: float):
             "sample": f"D2Q9 E-RLBM double shear L={L}, first {n} steps ({dt:.1f} s wall, {how})"}
 
 
-def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True):
+def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True, arith: str = "exact"):
     """Timed D2Q9 double-shear steps at L=2048 (HIP events on the lattice's
-    stream), after the autotuner has picked the launch shape."""
+    stream), after the autotuner has picked the launch shape; arith "moment":
+    the collision in moment space (ERLBM_ARITH_MOMENT)."""
     prec = {"fp64": E.FP64, "fp32": E.FP32_SHIFTED, "fp16": E.FP16_SHIFTED}[precision]
     f0 = E.init_double_shear(L)
     lat = E.Lattice(9, L, L, 1, precision=prec)
     lat.set_params(TAU, TAU_NH, 1.0, E.COL_EXT_REG)
+    lat.set_arith(E.ARITH_MOMENT if arith == "moment" else E.ARITH_EXACT)
     t = time.perf_counter()
     lat.upload(f0)
     lat.synchronize()
@@ -189,7 +191,7 @@ def cpu_maps_baseline():
 E_COL_EXT_REG = 4
 
 
-def d3q19_section(E, n: int = 256, steps: int = 200):
+def d3q19_section(E, n: int = 256, steps: int = 200, arith: str = "exact"):
     """The D3Q19 plain path at n^3 (periodic, E-RLBM, FP64) with the default
     kernel: 304 algorithmic bytes per node and launch (19 FP64 populations
     read + 19 written), for one step or, with the z-walk, two; the launch
@@ -198,6 +200,7 @@ def d3q19_section(E, n: int = 256, steps: int = 200):
     f0 *= 1.0 + 1e-3 * np.random.default_rng(0).uniform(-1.0, 1.0, f0.shape)
     with E.Lattice(19, n, n, n) as lat:
         lat.set_params(0.6, 0.65, 1.0, E.COL_EXT_REG)
+        lat.set_arith(E.ARITH_MOMENT if arith == "moment" else E.ARITH_EXACT)
         lat.upload(f0)
         del f0
         lat.step(10)
@@ -206,7 +209,7 @@ def d3q19_section(E, n: int = 256, steps: int = 200):
         ms = lat.step_timed(steps) / steps
         done, reason = lat.status()
     gbs = 304 * n ** 3 / (ms * spl * 1e-3) / 1e9   # algorithmic bytes of one launch / its duration
-    return {"workload": f"D3Q19 E-RLBM periodic {n}^3, tau 0.6, tau_nh 0.65, fp64", "MLUPS": round(n ** 3 / (ms * 1e-3) / 1e6, 1),
+    return {"workload": f"D3Q19 E-RLBM periodic {n}^3, tau 0.6, tau_nh 0.65, fp64", "arith": arith, "MLUPS": round(n ** 3 / (ms * 1e-3) / 1e6, 1),
             "ms_per_step": round(ms, 4), "launch": launch, "steps_per_launch": spl, "ms_per_launch": round(ms * spl, 4),
             "achieved_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
             "completed": done == 10 + steps and reason == 0,
@@ -312,8 +315,11 @@ def main() -> int:
                     help="storage of the headline run (fp32/fp16: shifted deviations, FP64 arithmetic)")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the calibration copy kernel (302 MB read + 302 MB written per launch)")
-    ap.add_argument("--only", choices=["d3q19", "maps", "serving"],
+    ap.add_argument("--only", choices=["d3q19", "d3q19_moment", "maps", "serving"],
                     help="run one section alone and print it (profiling runs)")
+    ap.add_argument("--arith", default="exact", choices=["exact", "moment"],
+                    help="collision arithmetic of the headline run: the reference's operation order (bitwise its "
+                         "strict build) or the same operator in moment space (within 1e-10)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -326,7 +332,8 @@ def main() -> int:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
     if a.only:
-        sec = {"d3q19": lambda: d3q19_section(E), "maps": lambda: maps_section(E), "serving": serving_section}
+        sec = {"d3q19": lambda: d3q19_section(E), "d3q19_moment": lambda: d3q19_section(E, arith="moment"),
+               "maps": lambda: maps_section(E), "serving": serving_section}
         print(json.dumps({a.only: sec[a.only]()}))
         return 0
     copy_bytes = BYTES_PER_NODE["fp64"] * L * L // 2     # a copy moving the bytes of one FP64 step
@@ -336,7 +343,7 @@ def main() -> int:
         print(json.dumps({"copy_bytes_each_way": copy_bytes, "ms": copy_ms, "GBs": copy_gbs}))
         return 0
 
-    ok, head = dsl_run(E, a.precision, a.steps, a.warmup, tune=a.tune)
+    ok, head = dsl_run(E, a.precision, a.steps, a.warmup, tune=a.tune, arith=a.arith)
     if not ok:
         print(f"bench.py: run check failed {head}", file=sys.stderr)
         return 1

@@ -5,7 +5,7 @@ HIP kernels for gfx950 behind the C ABI of ``include/erlbm.h``
 package is the Python host side over that ABI.  Importing it does not need a
 GPU; using a lattice does, and there is no CPU fallback.
 """
-from ._lib import (COL_BGK, COL_EXT_REG, COL_MRT, COL_REG, FP16_SHIFTED, FP32_SHIFTED, FP32_SHIFTED_F32, FP64,
+from ._lib import (ARITH_EXACT, ARITH_MOMENT, COL_BGK, COL_EXT_REG, COL_MRT, COL_REG, FP16_SHIFTED, FP32_SHIFTED, FP32_SHIFTED_F32, FP64,
                    POST_COLLISION,
                    STATE, STOP_DENSITY, STOP_ENERGY, ErlbmError, device_count, hbm_copy_bench, lib)
 from ._lib import FP_AUTO, INIT_REST, INIT_TAYLOR_GREEN, JOB_DTYPE, RESULT_DTYPE
@@ -13,7 +13,7 @@ from .lattice import (Lattice, dist_eq, init_double_shear, init_taylor_green, in
                       obstacle_mask, taylor_green_error, tg_map_point_params)
 
 __all__ = [
-    "COL_BGK", "COL_MRT", "COL_REG", "COL_EXT_REG", "FP64", "FP32_SHIFTED", "FP16_SHIFTED", "FP32_SHIFTED_F32", "STATE",
+    "ARITH_EXACT", "ARITH_MOMENT", "COL_BGK", "COL_MRT", "COL_REG", "COL_EXT_REG", "FP64", "FP32_SHIFTED", "FP16_SHIFTED", "FP32_SHIFTED_F32", "STATE",
     "POST_COLLISION", "STOP_DENSITY", "STOP_ENERGY", "ErlbmError", "device_count", "hbm_copy_bench", "lib", "Lattice", "dist_eq",
     "init_double_shear", "init_taylor_green", "init_tg_map", "obstacle_mask", "tg_map_point_params",
     "obstacle_map_point_params", "FP_AUTO", "INIT_REST", "INIT_TAYLOR_GREEN", "JOB_DTYPE", "RESULT_DTYPE", "Batch", "taylor_green_error",

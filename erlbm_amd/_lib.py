@@ -17,6 +17,7 @@ ERLBM_OK, ERLBM_ERR_ARG, ERLBM_ERR_HIP, ERLBM_ERR_STATE, ERLBM_ERR_NOMEM, ERLBM_
 COL_BGK, COL_MRT, COL_REG, COL_EXT_REG = 1, 2, 3, 4
 FP64, FP32_SHIFTED, FP16_SHIFTED, FP32_SHIFTED_F32 = 0, 1, 2, 4
 STATE, POST_COLLISION = 0, 1
+ARITH_EXACT, ARITH_MOMENT = 0, 1
 STOP_DENSITY, STOP_ENERGY = 1, 2
 
 _lib = None
@@ -84,6 +85,7 @@ SIGNATURES = {
     "erlbm_set_outlet_rescale": (_i, [_vp, _i, _d]),
     "erlbm_set_density_check": (_i, [_vp, _d]),
     "erlbm_set_body_force": (_i, [_vp, _d, _d, _d]),
+    "erlbm_set_arith": (_i, [_vp, _i]),
     "erlbm_autotune_candidates": (_i, [_i, _i, _i, _i, _i, _i, _ip, _i, C.POINTER(_i)]),
     "erlbm_upload_aos": (_i, [_vp, _dp]),
     "erlbm_download_aos": (_i, [_vp, _i, _dp]),

@@ -30,11 +30,29 @@
 namespace erlbm {
 
 enum Collision { COL_BGK = 1, COL_MRT = 2, COL_REG = 3, COL_EXT_REG = 4 };
+// Flag or-ed into a kernel's COL template argument: the operator in moment
+// space (collide_moment.hpp) instead of the reference's operation order.
+constexpr int COL_MOMENT = 16;
+
+// Per-lattice constants of the moment-space collision (collide_moment.hpp),
+// computed on the host from tau, tau_nh and the operator; b_axis / b_diag
+// carry the velocity set's axis and diagonal weights.
+struct MomentConsts {
+    double W = 0.0;        // multiplier of f_i
+    double cr = 0.0;       // K0 + D K1 / 2
+    double t15 = 0.0;      // 1.5 T
+    double k15 = 0.0;      // 1.5 K1
+    double t45 = 0.0;      // 4.5 T
+    double k45 = 0.0;      // 4.5 K1
+    double b_axis = 0.0;   // 3 K0 w_axis
+    double b_diag = 0.0;   // 3 K0 w_diag
+};
 
 struct CollideConsts {
     double omega1;   // 1 - 1/tau   (coll_Ext_Reg :1192; coll_Reg's (1 - 1/tau))
     double omega2;   // 1 - 1/tau_nh (coll_Ext_Reg :1194)
     double oot;      // 1/tau        (coll_BGK :24)
+    MomentConsts m;  // the moment-space form of the same operator
 };
 
 // Body-force acceleration of colisao_sitio's acc_x / acc_y / acc_z arguments

@@ -195,7 +195,7 @@ __global__ __launch_bounds__(BS) void k_step_periodic(StepArgs a, int remap)
         const int x = xs_[k];
         if (x < a.g.nx) {
             double vx, vy, vz;
-            collide_exact<Q, COL>(f[k], a.k, vx, vy, vz);
+            collide_sel<Q, COL>(f[k], a.k, vx, vy, vz);
             if (a.dens_limit > 0.0) {
                 const double d = density_exact<Q>(f[k]);
                 bad = bad || !(fabs(d) <= a.dens_limit);
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(BLOCK) void k_step2_d2q9(StepArgs a, int remap)
         const int e = tid + p * BLOCK;
         if (e < F::NE) {
             double vx, vy, vz;
-            collide_exact<9, COL>(f[p], a.k, vx, vy, vz);
+            collide_sel<9, COL>(f[p], a.k, vx, vy, vz);
             static_for<9>([&](auto I) { h1[(int)I * F::NE + e] = f[p][(int)I]; });
         }
     }
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(BLOCK) void k_step2_d2q9(StepArgs a, int remap)
                 g[i] = h1[i * F::NE + e - V::cx[i] - V::cy[i] * F::EX];
             });
             double vx, vy, vz;
-            collide_exact<9, COL>(g, a.k, vx, vy, vz);
+            collide_sel<9, COL>(g, a.k, vx, vy, vz);
             static_for<9>([&](auto I) {
                 __builtin_nontemporal_store(g[(int)I], &a.dst[at(a.g, (int)I, y0 + oy, x0 + ox)]);
             });
@@ -338,7 +338,7 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
     auto collide_to_ring = [&](int y, double (&f)[9]) {
         if (!act1) return;
         double vx, vy, vz;
-        collide_exact<9, COL>(f, a.k, vx, vy, vz);
+        collide_sel<9, COL>(f, a.k, vx, vy, vz);
         static_for<9>([&](auto I) { ring[ring_row((int)I, y) * EX + t] = f[(int)I]; });
     };
     auto output_row = [&](int y) {            // step k+1, row y, from the ring
@@ -349,7 +349,7 @@ __global__ __launch_bounds__(strip_threads<TX>()) void k_step2_strip(StepArgs a,
                 g[i] = ring[ring_row(i, y - V::cy[i]) * EX + (t + 1 - V::cx[i])];
             });
             double vx, vy, vz;
-            collide_exact<9, COL>(g, a.k, vx, vy, vz);
+            collide_sel<9, COL>(g, a.k, vx, vy, vz);
         }
         static_for<9>([&](auto I) { buf_store_nt(g[(int)I], rd, vst | sto, (int)(8 * at(a.g, (int)I, y, 0))); });
     };
@@ -430,7 +430,7 @@ __device__ __forceinline__ void strip3_walk(const StepArgs &a, int x0, int w, in
     auto now = [](int i) { return !LEAN || V::cy[i] * DIR < 0; };
     auto collide = [&](double (&f)[9]) {
         double vx, vy, vz;
-        collide_exact<9, COL>(f, a.k, vx, vy, vz);
+        collide_sel<9, COL>(f, a.k, vx, vy, vz);
     };
     // step-k rows ya-2 .. ya+rows+1 in walking order
     const int r0 = DIR > 0 ? ya - 2 : ya + rows + 1;
@@ -591,7 +591,7 @@ __device__ __forceinline__ void zwalk(const StepArgs &a, int x0, int y0, int za,
     auto now = [](int i) { return V::cz[i] * DIR < 0; };
     auto collide = [&](double (&f)[19]) {
         double vx, vy, vz;
-        collide_exact<19, COL>(f, a.k, vx, vy, vz);
+        collide_sel<19, COL>(f, a.k, vx, vy, vz);
     };
     const int ox = act2 ? t % TX : 0, oy = act2 ? t / TX : 0;
     const unsigned vst = 8u * (unsigned)((y0 + oy) * a.g.sr + x0 + ox);
@@ -710,7 +710,7 @@ __device__ __forceinline__ void wave_walk(const StepArgs &a, int x0, int ya, int
     };
     auto collide = [&](double (&f)[9]) {
         double vx, vy, vz;
-        collide_exact<9, COL>(f, a.k, vx, vy, vz);
+        collide_sel<9, COL>(f, a.k, vx, vy, vz);
     };
     // step-k results of output row y: hb row y-DIR, h0 row y, hn row y+DIR
     double hb[9], h0[9], hn[9], fq[PD][9];
@@ -823,7 +823,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_d2q9_pair(StepArgs a, int remap)
 #pragma unroll
         for (int k = 0; k < 2; k++) {
             double vx, vy, vz;
-            collide_exact<9, COL>(f[k], a.k, vx, vy, vz);
+            collide_sel<9, COL>(f[k], a.k, vx, vy, vz);
             if (a.dens_limit > 0.0) {
                 const double d = density_exact<9>(f[k]);
                 bad = bad || !(fabs(d) <= a.dens_limit);
@@ -1379,6 +1379,21 @@ extern "C" int erlbm_set_body_force(erlbm_lattice *h, double ax, double ay, doub
     return ERLBM_OK;
 }
 
+extern "C" int erlbm_set_arith(erlbm_lattice *h, int arith)
+{
+    if (!h) return ERLBM_ERR_ARG;
+    if (arith != ERLBM_ARITH_EXACT && arith != ERLBM_ARITH_MOMENT) {
+        set_last_error_msg("erlbm_set_arith: arith must be ERLBM_ARITH_EXACT or ERLBM_ARITH_MOMENT");
+        return ERLBM_ERR_ARG;
+    }
+    if (arith == ERLBM_ARITH_MOMENT && h->elem != 8) {
+        set_last_error_msg("erlbm_set_arith: the moment-space arithmetic needs FP64 storage");
+        return ERLBM_ERR_UNSUPPORTED;
+    }
+    h->arith = arith;
+    return ERLBM_OK;
+}
+
 extern "C" int erlbm_set_density_check(erlbm_lattice *h, double limit)
 {
     if (!h) return ERLBM_ERR_ARG;
@@ -1435,7 +1450,7 @@ extern "C" int erlbm_upload_aos(erlbm_lattice *h, const double *inif)
 namespace {
 
 // Shapes of the two-step kernel (ERLBM_FUSE selects one; 0 = off).
-enum FuseKind { FUSE_TILE, FUSE_STRIP, FUSE_WAVE, FUSE_STRIP3, FUSE_ZWALK, FUSE_GSTRIP3 };
+enum FuseKind { FUSE_TILE, FUSE_STRIP, FUSE_WAVE, FUSE_STRIP3, FUSE_ZWALK, FUSE_GSTRIP3, FUSE_GSTRIP4 };
 struct FuseShape {
     int tx, ty;       // tile; (strip) strip width and rows walked; (wave) rows walked
     FuseKind kind;
@@ -1467,15 +1482,18 @@ constexpr FuseShape FUSE_SHAPES[] = {
     // the same with the lanes aligned across the phases (c_x = 0 populations
     // in registers), 2, 3, 4, 8, 5 and 6 wavefronts per workgroup
     {124, 0, FUSE_GSTRIP3},            {188, 0, FUSE_GSTRIP3},           {252, 0, FUSE_GSTRIP3},
-    {508, 0, FUSE_GSTRIP3},            {316, 0, FUSE_GSTRIP3},           {380, 0, FUSE_GSTRIP3}};
+    {508, 0, FUSE_GSTRIP3},            {316, 0, FUSE_GSTRIP3},           {380, 0, FUSE_GSTRIP3},
+    // four steps per launch: lane-aligned strips of any width up to 64 NW - 6,
+    // four wavefronts
+    {250, 0, FUSE_GSTRIP4}};
 constexpr int N_FUSE_SHAPES = sizeof(FUSE_SHAPES) / sizeof(FUSE_SHAPES[0]);
 
 // Steps one launch of the shape advances.
 int fuse_steps(int f)
 {
-    return f > 0 && f < N_FUSE_SHAPES && (FUSE_SHAPES[f].kind == FUSE_STRIP3 || FUSE_SHAPES[f].kind == FUSE_GSTRIP3)
-               ? 3
-               : 2;
+    if (f <= 0 || f >= N_FUSE_SHAPES) return 2;
+    const FuseKind k = FUSE_SHAPES[f].kind;
+    return k == FUSE_GSTRIP4 ? 4 : (k == FUSE_STRIP3 || k == FUSE_GSTRIP3 ? 3 : 2);
 }
 
 // Segments per strip for the wavefront-strip kernel: `rounds` times as many
@@ -1498,7 +1516,8 @@ bool fuse_geometry_fits(const erlbm_lattice *h, const FuseShape &fs)
     }
     if (h->nz != 1) return false;
     if (fs.kind == FUSE_WAVE) return h->nx >= 64 && h->extent * 8 < (1ll << 31);   // 31-bit buffer offsets
-    if (fs.kind == FUSE_GSTRIP3) return h->extent * h->elem < (1ll << 31) && h->nx >= 16 && h->ny >= 16;
+    if (fs.kind == FUSE_GSTRIP3 || fs.kind == FUSE_GSTRIP4)
+        return h->extent * h->elem < (1ll << 31) && h->nx >= 16 && h->ny >= 16 && (fs.kind == FUSE_GSTRIP3 || h->elem == 8);
     if ((fs.kind == FUSE_STRIP || fs.kind == FUSE_STRIP3) && h->elem == 8 && h->extent * 8 >= (1ll << 31)) return false;
     if (fs.ty == 0) return h->nx % fs.tx == 0 && h->ny >= 16;
     return h->nx % fs.tx == 0 && h->ny % fs.ty == 0;
@@ -1713,9 +1732,10 @@ __device__ __forceinline__ void strip3_walk_aligned(const StepArgs &a, int x0, i
     auto age = [](int i) { return 1 + V::cy[i] * DIR; };
     auto collide = [&](double (&f)[9]) {
         double vx, vy, vz;
-        collide_exact<9, COL>(f, a.k, vx, vy, vz);
+        collide_sel<9, COL>(f, a.k, vx, vy, vz);
     };
     const int r0 = DIR > 0 ? ya - 2 : ya + rows + 1;
+    const int nsteps = rows + 4;
     double fq[9];
     double h2[9];                  // the previous walking step's step-(k+1) row, still to be written to ring 2
     double k1a[9], k1b[9];         // c_x = 0: step-k results of the rows one and two walking steps back
@@ -1723,12 +1743,12 @@ __device__ __forceinline__ void strip3_walk_aligned(const StepArgs &a, int x0, i
     load_row(r0, fq, false);
     if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
     nine_dropped_stores<double>(rd);
-    for (int n = 0; n < rows + 4; n++) {
-        if (prio) progress_prio(n, rows + 4, prio);
+    for (int n = 0; n < nsteps; n++) {
+        if (prio) progress_prio(n, nsteps, prio);
         const int r = r0 + DIR * n;
         double f[9];
         static_for<9>([&](auto I) { f[(int)I] = fq[(int)I]; });
-        load_row(r + DIR, fq, n + 1 >= rows + 4);                    // in flight meanwhile
+        load_row(r + DIR, fq, n + 1 >= nsteps);                      // in flight meanwhile
         if (act1) {                                                   // phase 1: step k, row r
             collide(f);
             static_for<9>([&](auto I) {
@@ -1808,6 +1828,176 @@ __global__ __launch_bounds__(64 * NW) void k_step3_astrip(StepArgs a, int remap,
     const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
     if (seg & 1) strip3_walk_aligned<COL, NW, -1, NTP>(a, x0, w, ya, rows, rings, rings + XRING_ROWS * E, prio);
     else strip3_walk_aligned<COL, NW, 1, NTP>(a, x0, w, ya, rows, rings, rings + XRING_ROWS * E, prio);
+}
+
+// Four steps per launch: the lane-aligned walk with a third ring.  Lane t
+// holds position x0 - 3 + t in all four phases (phase 1: lanes 0 .. w+5,
+// 2: 1 .. w+4, 3: 2 .. w+3, 4 and the stores: 3 .. w+2).  With two barriers
+// per walking step the phases alternate between the two intervals they
+// make: walking step n (row r = r0 + DIR n) runs phase 1 on row r before
+// the first barrier, phases 2 (row r - DIR) and 4 (row r - 4 DIR) between
+// the barriers, and phase 3 (row r - 2 DIR) after the second; phase 4 so
+// reads the ring-3 rows phase 3 wrote in the walking steps before.  Rings
+// 1, 2 and 3 keep the c_x != 0 results of phases 1, 2 and 3 (lean layout,
+// 8 rows each: 48 KB for NW = 4, three workgroups per CU), the c_x = 0 ones
+// stay in registers; a ring's late writes (the populations not read from
+// the newest row) follow its first reader by one interval as in
+// strip3_walk_aligned: ring 1 after the second barrier, ring 2 after the
+// first barrier of the next walking step, ring 3 after the second barrier
+// of the next walking step.  Output rows ya .. ya+rows-1 need step-k rows
+// ya-3 .. ya+rows+2: rows + 7 walking steps.
+template <int COL, int NW, int DIR>
+__device__ __forceinline__ void strip4_walk_aligned(const StepArgs &a, int x0, int w, int ya, int rows, double *ring1,
+                                                    double *ring2, double *ring3, int prio)
+{
+    using V = VSet<9>;
+    constexpr int E = 64 * NW;
+    const int t = (int)threadIdx.x;
+    const int nx = a.g.nx, ny = a.g.ny;
+    const bool act1 = t < w + 6, act2 = t >= 1 && t < w + 5, act3 = t >= 2 && t < w + 4, act4 = t >= 3 && t < w + 3;
+    const int x1 = act1 ? wrap(x0 - 3 + t, nx) : 0;
+    const __amdgpu_buffer_rsrc_t rs = pop_buffer<8>(a.src, a.g), rd = pop_buffer<8>(a.dst, a.g);
+    const unsigned oob1 = act1 ? 0u : BUF_OOB;
+    const unsigned vo[3] = {8u * wrap(x1 + 1, nx) | oob1, 8u * x1 | oob1, 8u * wrap(x1 - 1, nx) | oob1};
+    const unsigned vst = act4 ? 8u * (x0 - 3 + t) : 0u;
+    auto load_row = [&](int y, double (&f)[9], bool skip) {   // pulls of step-k row y (-4 .. ny+3)
+        const unsigned off = skip ? BUF_OOB : 0u;
+        const int yw = wrap(y, ny);
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            const int ys = V::cy[i] == 0 ? yw : wrap(yw - V::cy[i], ny);
+            f[i] = buf_load<0>(rs, vo[V::cx[i] + 1] | off, (int)(8 * at(a.g, i, ys, 0)));
+        });
+    };
+    auto rrow = [&](int i, int y) { return xring_base<DIR>(i) + (y - ya + 4) % xring_depth<DIR>(i); };
+    auto inring = [](int i) { return V::cx[i] != 0; };
+    auto now = [](int i) { return V::cy[i] * DIR < 0; };
+    auto age = [](int i) { return 1 + V::cy[i] * DIR; };
+    auto collide = [&](double (&f)[9]) {
+        double vx, vy, vz;
+        collide_sel<9, COL>(f, a.k, vx, vy, vz);
+    };
+    // pull of a phase from the ring of the phase before (c_x != 0) and from
+    // that phase's results in registers (c_x = 0: this lane's own, of the
+    // newest, middle and oldest source row)
+    auto pull = [&](double (&g)[9], const double *ring, int y, const double (&r0)[9], const double (&r1)[9],
+                    const double (&r2)[9]) {
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            if constexpr (V::cx[i] != 0) g[i] = ring[rrow(i, y - V::cy[i]) * E + (t - V::cx[i])];
+            else g[i] = age(i) == 0 ? r0[i] : (age(i) == 1 ? r1[i] : r2[i]);
+        });
+    };
+    const int r0 = DIR > 0 ? ya - 3 : ya + rows + 2;
+    const int nsteps = rows + 7, nrows1 = rows + 6;   // walking steps; rows of phase 1
+    double fq[9];
+    double h2[9], h3[9];           // the previous walking step's phase-2 / phase-3 rows (late ring writes)
+    double k1a[9], k1b[9];         // c_x = 0: phase-1 results of the rows one and two walking steps back
+    double k2a[9], k2b[9];         // phase 2, likewise
+    double k3a[9], k3b[9];         // phase 3, one and two walking steps before h3
+    load_row(r0, fq, false);
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+    nine_dropped_stores<double>(rd);
+    for (int n = 0; n < nsteps; n++) {
+        if (prio) progress_prio(n, nsteps, prio);
+        const int r = r0 + DIR * n;
+        double f[9];
+        static_for<9>([&](auto I) { f[(int)I] = fq[(int)I]; });
+        load_row(r + DIR, fq, n + 1 >= nrows1);                      // in flight meanwhile
+        const bool p1 = act1 && n < nrows1;
+        if (p1) {                                                     // phase 1: step k, row r
+            collide(f);
+            static_for<9>([&](auto I) {
+                if (inring(I) && now(I)) ring1[rrow((int)I, r) * E + t] = f[(int)I];
+            });
+        }
+        lds_barrier();
+        if (act2 && n >= 3) {                                         // the rest of phase-2 row r-2 DIR
+            static_for<9>([&](auto I) {
+                if (inring(I) && !now(I)) ring2[rrow((int)I, r - 2 * DIR) * E + t] = h2[(int)I];
+            });
+        }
+        if (act2 && n >= 2 && n < nrows1) {                           // phase 2: step k+1, row r-DIR
+            const int y = r - DIR;
+            double g[9];
+            pull(g, ring1, y, f, k1a, k1b);
+            collide(g);
+            static_for<9>([&](auto I) {
+                if (inring(I) && now(I)) ring2[rrow((int)I, y) * E + t] = g[(int)I];
+                h2[(int)I] = g[(int)I];
+            });
+        }
+        static_for<9>([&](auto I) {
+            if constexpr (V::cx[(int)I] == 0) {
+                k1b[(int)I] = k1a[(int)I];
+                k1a[(int)I] = f[(int)I];
+            }
+        });
+        const bool out = n >= 7;                                      // phase 4: step k+3, row r-4 DIR
+        const int y4 = out ? r - 4 * DIR : ya;
+        double o[9] = {};
+        if (act4 && out) {
+            pull(o, ring3, y4, h3, k3a, k3b);
+            collide(o);
+        }
+        static_for<9>([&](auto I) {
+            if constexpr (V::cx[(int)I] == 0) {
+                k3b[(int)I] = k3a[(int)I];
+                k3a[(int)I] = h3[(int)I];
+            }
+        });
+        const unsigned sto = act4 && out ? 0u : BUF_OOB;
+        static_for<9>([&](auto I) { buf_store_nt(o[(int)I], rd, vst | sto, (int)(8 * at(a.g, (int)I, y4, 0))); });
+        lds_barrier();
+        if (p1) {                                                     // the rest of phase-1 row r
+            static_for<9>([&](auto I) {
+                if (inring(I) && !now(I)) ring1[rrow((int)I, r) * E + t] = f[(int)I];
+            });
+        }
+        if (act3 && n >= 5) {                                         // the rest of phase-3 row r-3 DIR
+            static_for<9>([&](auto I) {
+                if (inring(I) && !now(I)) ring3[rrow((int)I, r - 3 * DIR) * E + t] = h3[(int)I];
+            });
+        }
+        if (act3 && n >= 4 && n < nrows1) {                           // phase 3: step k+2, row r-2 DIR
+            const int y = r - 2 * DIR;
+            double g[9];
+            pull(g, ring2, y, h2, k2a, k2b);
+            collide(g);
+            static_for<9>([&](auto I) {
+                if (inring(I) && now(I)) ring3[rrow((int)I, y) * E + t] = g[(int)I];
+                h3[(int)I] = g[(int)I];
+            });
+        }
+        static_for<9>([&](auto I) {
+            if constexpr (V::cx[(int)I] == 0) {
+                k2b[(int)I] = k2a[(int)I];
+                k2a[(int)I] = h2[(int)I];
+            }
+        });
+    }
+}
+
+// Strip edges on multiples of `unit` positions, as k_step3_astrip.
+template <int COL, int NW>
+__global__ __launch_bounds__(64 * NW) void k_step4_astrip(StepArgs a, int remap, int nstrip, int nseg, int unit,
+                                                         int prio)
+{
+    constexpr int E = 64 * NW;
+    extern __shared__ double rings[];   // rings 1, 2, 3: [XRING_ROWS][E] each
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (remap) b = xcd_runs(b, nb);
+    const int s = (int)(b % (unsigned)nstrip);
+    const int units = a.g.nx / unit;
+    const int x0 = (int)((long long)s * units / nstrip) * unit;
+    const int w = (int)((long long)(s + 1) * units / nstrip) * unit - x0;
+    const int seg = (int)(b / (unsigned)nstrip);
+    const int ya = (int)((long long)seg * a.g.ny / nseg);
+    const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
+    double *r1 = rings, *r2 = rings + XRING_ROWS * E, *r3 = rings + 2 * XRING_ROWS * E;
+    if (seg & 1) strip4_walk_aligned<COL, NW, -1>(a, x0, w, ya, rows, r1, r2, r3, prio);
+    else strip4_walk_aligned<COL, NW, 1>(a, x0, w, ya, rows, r1, r2, r3, prio);
 }
 
 // Strips of at most 64 NW - 4 positions, as few as the width allows;
@@ -1891,7 +2081,13 @@ void launch_gstrip3_t(erlbm_lattice *h, const StepArgs &a)
     constexpr size_t lds = gstrip3_lds<NW, ALIGNED>();
     const int unit = ALIGNED ? astrip_unit(h) : 1;
     int nstrip, nseg;
-    strip3_grid(h, 64 * NW - 4, true, gstrip3_per_cu<COL, NW, ALIGNED>(), nstrip, nseg, unit);
+    // The moment-space build of the four-wavefront aligned walk fits five
+    // workgroups per CU (92 VGPRs); its segments are sized for four all the
+    // same (L = 2048, MLUPS: sized for five 83.7k, four 102.6k, three 101.2k):
+    // shorter segments recompute more halo rows
+    int per_cu = gstrip3_per_cu<COL, NW, ALIGNED>();
+    if ((COL & COL_MOMENT) != 0 && ALIGNED && NW == 4) per_cu = std::min(per_cu, 4);
+    strip3_grid(h, 64 * NW - 4, true, per_cu, nstrip, nseg, unit);
     if constexpr (ALIGNED) {
         if (NW == 4 && h->tune_walk_nt == 1)
             hipLaunchKernelGGL((k_step3_astrip<COL, NW, 1>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds,
@@ -1903,6 +2099,38 @@ void launch_gstrip3_t(erlbm_lattice *h, const StepArgs &a)
     else
         hipLaunchKernelGGL((k_step3_gstrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream,
                            a, h->tune_fuse_remap, nstrip, nseg);
+}
+
+// Four-step lane-aligned walk: three 8-row rings of 64 NW positions.
+template <int NW>
+constexpr size_t astrip4_lds()
+{
+    return sizeof(double) * 3 * XRING_ROWS * (64 * NW);
+}
+
+template <int COL, int NW>
+int astrip4_per_cu()
+{
+    static const int per_cu = [] {
+        constexpr size_t lds = astrip4_lds<NW>();
+        const void *k = reinterpret_cast<const void *>(&k_step4_astrip<COL, NW>);
+        if constexpr (lds > 65536) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int nblk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, k, 64 * NW, lds) != hipSuccess || nblk < 1) nblk = 1;
+        return nblk;
+    }();
+    return per_cu;
+}
+
+template <int COL, int NW>
+void launch_astrip4_t(erlbm_lattice *h, const StepArgs &a)
+{
+    constexpr size_t lds = astrip4_lds<NW>();
+    const int unit = astrip_unit(h);
+    int nstrip, nseg;
+    strip3_grid(h, 64 * NW - 6, true, astrip4_per_cu<COL, NW>(), nstrip, nseg, unit);
+    hipLaunchKernelGGL((k_step4_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream, a,
+                       h->tune_fuse_remap, nstrip, nseg, unit, walk_prio(h, nstrip * nseg));
 }
 
 // Segments as many as make one workgroup per resident slot, at least 16
@@ -1932,6 +2160,21 @@ void launch_zwalk_t(erlbm_lattice *h, const StepArgs &a)
 template <int COL>
 void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
 {
+    if constexpr ((COL & COL_MOMENT) != 0) {
+        // the moment-space collision on the shapes the size rule picks and
+        // the two-step remainder; any other shape keeps the reference's order
+        switch (h->fuse_active) {
+        case FUSE2_ZWALK_64x8: launch_zwalk_t<COL, 64, 8>(h, a); return;
+        case FUSE3_STRIP_SIZED: launch_strip3_t<COL, 256, 0, true>(h, a); return;
+        case FUSE3_GSTRIP_4W: launch_gstrip3_t<COL, 4>(h, a); return;
+        case FUSE3_ASTRIP_2W: launch_gstrip3_t<COL, 2, true>(h, a); return;
+        case FUSE3_ASTRIP_3W: launch_gstrip3_t<COL, 3, true>(h, a); return;
+        case FUSE3_ASTRIP_4W: launch_gstrip3_t<COL, 4, true>(h, a); return;
+        case FUSE4_ASTRIP_4W: launch_astrip4_t<COL, 4>(h, a); return;
+        case FUSE2_STRIP_256x32: launch_strip_t<COL, 256, 32>(h, a); return;
+        default: launch_fused_c<(COL & ~COL_MOMENT)>(h, a); return;
+        }
+    }
     if constexpr (COL != COL_MRT) {   // D3Q19 has no MRT
         switch (h->fuse_active) {
         case FUSE2_ZWALK_32x8: launch_zwalk_t<COL, 32, 8>(h, a); return;
@@ -1962,6 +2205,7 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
     case FUSE3_ASTRIP_8W: launch_gstrip3_t<COL, 8, true>(h, a); return;
     case FUSE3_ASTRIP_5W: launch_gstrip3_t<COL, 5, true>(h, a); return;
     case FUSE3_ASTRIP_6W: launch_gstrip3_t<COL, 6, true>(h, a); return;
+    case FUSE4_ASTRIP_4W: launch_astrip4_t<COL, 4>(h, a); return;
     case 16: launch_wave_t<COL, 1, 4, 1>(h, a, 2); return;
     case 17: launch_wave_t<COL, 1, 4, 1>(h, a, 1); return;
     case 18: launch_wave_t<COL, 2, 4, 1>(h, a, 2); return;
@@ -1992,6 +2236,17 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
 int launch_fused(erlbm_lattice *h, const StepArgs &a)
 {
     if (h->elem != 8) return lowp_step2(h, a);
+    if (h->arith == ERLBM_ARITH_MOMENT) {
+        switch (h->col) {   // coll_MRT keeps the reference's order
+        case ERLBM_COL_BGK: launch_fused_c<COL_BGK | COL_MOMENT>(h, a); break;
+        case ERLBM_COL_MRT: launch_fused_c<COL_MRT>(h, a); break;
+        case ERLBM_COL_REG: launch_fused_c<COL_REG | COL_MOMENT>(h, a); break;
+        case ERLBM_COL_EXT_REG: launch_fused_c<COL_EXT_REG | COL_MOMENT>(h, a); break;
+        default: return ERLBM_ERR_UNSUPPORTED;
+        }
+        ERLBM_HIP_TRY(hipGetLastError());
+        return ERLBM_OK;
+    }
     switch (h->col) {
     case ERLBM_COL_BGK: launch_fused_c<COL_BGK>(h, a); break;
     case ERLBM_COL_MRT: launch_fused_c<COL_MRT>(h, a); break;
@@ -2015,6 +2270,7 @@ StepArgs step_args(const erlbm_lattice *h)
     a.k.omega1 = 1.0 - 1.0 / h->tau;
     a.k.omega2 = 1.0 - 1.0 / h->tau_nh;
     a.k.oot = 1.0 / h->tau;
+    a.k.m = moment_consts(h->q, h->col, h->tau, h->tau_nh);
     a.bf.ax = h->acc[0];
     a.bf.ay = h->acc[1];
     a.bf.az = h->acc[2];
@@ -2042,13 +2298,22 @@ int dispatch_step(erlbm_lattice *h, const StepArgs &a, bool stream)
 
 namespace {
 
-// Two-step shape for a remainder of two steps after three-step launches (the
+// Three-step shape for a remainder of three steps after four-step launches:
+// the lane-aligned three-step walk where it fills the chip.  0: none.
+int remainder3_shape(const erlbm_lattice *h)
+{
+    if (fuse_steps(h->fuse_active) != 4) return 0;
+    const int f = FUSE3_ASTRIP_4W;
+    return fuse_shape_fits(h, f) && strip3_workgroups(h, f) >= (19ll * h->n_cu + 19) / 20 ? f : 0;
+}
+
+// Two-step shape for a remainder of two steps after three- or four-step launches (the
 // two-step strip walk, else the 64x8 tiles, else the wavefront strips, where
 // they fill the chip): one launch instead of two one-step launches (L = 2048:
 // ~116 us against ~192 us), bitwise the same steps.  0: none.
 int remainder2_shape(const erlbm_lattice *h)
 {
-    if (fuse_steps(h->fuse_active) != 3) return 0;
+    if (fuse_steps(h->fuse_active) < 3) return 0;
     for (int f : {FUSE2_STRIP_256x32, 1, 16})
         if (fuse_shape_fits(h, f) && fuse_workgroups(h, FUSE_SHAPES[f]) >= h->n_cu) return f;
     return 0;
@@ -2089,6 +2354,7 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
     const bool fuse = h->fuse_active > 0;
     const int spl = fuse_steps(h->fuse_active);   // steps per fused launch
     const int rem2 = fuse ? remainder2_shape(h) : 0;
+    const int rem3 = fuse ? remainder3_shape(h) : 0;
     for (int s = 0; s < n_steps;) {
         const long long k = h->launched;
         const int cur = (h->base + (int)(k & 1)) & 1;
@@ -2107,8 +2373,18 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
             int rc = launch_fused(h, a);
             if (rc) return rc;
             h->launched += spl;
-            if (spl == 2) h->base ^= 1;
+            if (spl % 2 == 0) h->base ^= 1;
             s += spl;
+            continue;
+        }
+        if (rem3 && k > 0 && n_steps - s == 3) {
+            const int keep = h->fuse_active;
+            h->fuse_active = rem3;
+            int rc = launch_fused(h, a);
+            h->fuse_active = keep;
+            if (rc) return rc;
+            h->launched += 3;
+            s += 3;
             continue;
         }
         if (rem2 && k > 0 && n_steps - s == 2) {
@@ -2255,6 +2531,7 @@ std::string fuse_shape_desc(int f, int remap)
             std::to_string((fs.tx + 4) / 64) + " wavefronts)";
         break;
     case FUSE_ZWALK: d = "two steps, " + tx + "x" + ty + " tiles walked along z"; break;
+    case FUSE_GSTRIP4: d = "four steps, lane-aligned strips of any width (" + std::to_string((fs.tx + 6) / 64) + " wavefronts)"; break;
     }
     return remap ? d + ", XCD runs" : d;
 }

@@ -8,7 +8,7 @@
 
 #include <cstdint>
 
-#include "collide.hpp"
+#include "collide_moment.hpp"
 #include "common.hpp"
 
 namespace erlbm {
@@ -132,6 +132,8 @@ constexpr int FUSE3_GSTRIP_1W = 36, FUSE3_GSTRIP_2W = 37, FUSE3_GSTRIP_3W = 38, 
 // the same with the lanes aligned across the walk's phases, 2, 3, 4, 8 wavefronts
 constexpr int FUSE3_ASTRIP_2W = 40, FUSE3_ASTRIP_3W = 41, FUSE3_ASTRIP_4W = 42, FUSE3_ASTRIP_8W = 43,
               FUSE3_ASTRIP_5W = 44, FUSE3_ASTRIP_6W = 45;
+// D2Q9 four steps per launch, lane-aligned strips of any width, 4 wavefronts
+constexpr int FUSE4_ASTRIP_4W = 46;
 
 struct erlbm_lattice {
     int q = 0, nx = 0, ny = 0, nz = 0;
@@ -170,6 +172,8 @@ struct erlbm_lattice {
     // one-step kernel
     bool forced = false;
     double acc[3] = {0.0, 0.0, 0.0};
+    // collision arithmetic of the multi-step kernels (erlbm_set_arith)
+    int arith = ERLBM_ARITH_EXACT;
 
     // launch tuning (environment overrides read at creation):
     //   ERLBM_PLANE_PAD  doubles added to each population plane's stride
@@ -210,6 +214,41 @@ struct erlbm_lattice {
 
 namespace erlbm {
 namespace lat {
+
+// Constants of the moment-space collision (collide_moment.hpp) for a
+// velocity set, operator and relaxation times: (W, K0, K1) = (1 - 1/tau_nh,
+// 1/tau_nh, 1/tau_nh - 1/tau) for E-RLBM, (0, 1, 1 - 1/tau) for the
+// regularized operator, (1 - 1/tau, 1/tau, 0) for BGK; T = 1/tau.
+inline MomentConsts moment_consts(int q, int col, double tau, double tau_nh)
+{
+    double W, K0, K1;
+    if (col == ERLBM_COL_BGK) {
+        W = 1.0 - 1.0 / tau;
+        K0 = 1.0 / tau;
+        K1 = 0.0;
+    } else if (col == ERLBM_COL_REG) {
+        W = 0.0;
+        K0 = 1.0;
+        K1 = 1.0 - 1.0 / tau;
+    } else {
+        W = 1.0 - 1.0 / tau_nh;
+        K0 = 1.0 / tau_nh;
+        K1 = 1.0 / tau_nh - 1.0 / tau;
+    }
+    const double T = 1.0 / tau;
+    const double D = q == 9 ? 2.0 : 3.0;
+    const double wa = q == 9 ? 1.0 / 9.0 : 1.0 / 18.0, wd = 1.0 / 36.0;
+    MomentConsts m;
+    m.W = W;
+    m.cr = K0 + 0.5 * D * K1;
+    m.t15 = 1.5 * T;
+    m.k15 = 1.5 * K1;
+    m.t45 = 4.5 * T;
+    m.k45 = 4.5 * K1;
+    m.b_axis = (3.0 * wa) * K0;
+    m.b_diag = (3.0 * wd) * K0;
+    return m;
+}
 
 inline int mon_blocks(const erlbm_lattice *h) { return (h->n_fluid + BLOCK - 1) / BLOCK; }
 

@@ -100,6 +100,21 @@ int erlbm_set_outlet_rescale(erlbm_lattice *h, int pos_x, double rho);
  * operator.  A forced lattice needs FP64 storage (ERLBM_ERR_UNSUPPORTED
  * otherwise) and advances one step per launch.                             */
 int erlbm_set_body_force(erlbm_lattice *h, double ax, double ay, double az);
+/* Collision arithmetic of the multi-step kernels (the reference chooses its
+ * rounding at build time: its Makefile's FMA contraction or not).
+ * ERLBM_ARITH_EXACT (default): the reference's operation order, every finite
+ * population bitwise its -ffp-contract=off build.  ERLBM_ARITH_MOMENT: the
+ * same BGK / RLBM / E-RLBM operator (coll_BGK, coll_Reg, coll_Ext_Reg,
+ * Collision_operators.cpp:11-49, 859-967, 1181-1307) re-associated in moment
+ * space, f*_i = W f_i + w_i (alpha + beta.c_i + c_i.Gamma.c_i), with less
+ * than half the FP64 instructions; within 1e-10 maximum relative population
+ * error of the reference after 1000 steps.  Applies to the multi-step
+ * kernels of FP64 periodic lattices; one-step launches, coll_MRT and
+ * lattices with boundary columns, solids, stop tests or a body force keep
+ * the reference's order.  FP64 storage only (ERLBM_ERR_UNSUPPORTED
+ * otherwise).                                                              */
+enum { ERLBM_ARITH_EXACT = 0, ERLBM_ARITH_MOMENT = 1 };
+int erlbm_set_arith(erlbm_lattice *h, int arith);
 /* Per-step post-collision density test of main_stability-map_obstacle.cpp:
  * 104-106: a step whose some site has |rho| > limit or non-finite rho stops
  * the run after that step completes.  limit <= 0 disables it.             */

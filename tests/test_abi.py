@@ -59,6 +59,7 @@ def test_null_handles_are_argument_errors():
     assert lib.erlbm_step_shape(None, C.byref(f), C.byref(spl), None, 0) == ERLBM_ERR_ARG
     assert (f.value, spl.value) == (-1, -1)
     assert lib.erlbm_step(None, 1, 0, 0.0) == ERLBM_ERR_ARG
+    assert lib.erlbm_set_arith(None, erlbm_amd.ARITH_MOMENT) == ERLBM_ERR_ARG
 
 
 def test_host_initial_conditions_bitwise(orc):

@@ -6,6 +6,8 @@ tau_nh = 0.65, equilibrium initialisation of src/main_double-shear-layer.cpp:
   * FP64: bitwise the strict oracle (the reference's operation order, no FMA
     contraction); max relative population error <= 1e-10 against the
     FMA-contracted oracle (the reference's default build);
+  * FP64 with the moment-space collision arithmetic (ERLBM_ARITH_MOMENT):
+    max relative population error <= 1e-10 against both oracle builds;
   * FP32-shifted storage: bitwise the one-step FP32 path, and relative L2
     velocity error against the FP64 oracle below 5e-6 (the storage format's
     error budget at this size and step count).
@@ -43,12 +45,21 @@ def strict_1000(orc, dsl_setup):
     return orc.run_steps(9, f0, st, N, O.COL_EXT_REG, TAU, TAU_NH, nthreads=NTHREADS)
 
 
-def _gpu_run(gpu, f0, precision):
+@pytest.fixture(scope="module")
+def fma_1000(dsl_setup):
+    f0, st = dsl_setup
+    print(f"oracle (FMA): DSL L={L}, {N} steps, {NTHREADS} threads", flush=True)
+    return O.Oracle(fma=True).run_steps(9, f0, st, N, O.COL_EXT_REG, TAU, TAU_NH, nthreads=NTHREADS)
+
+
+def _gpu_run(gpu, f0, precision, arith=0, calls=(N,)):
     with gpu.Lattice(9, L, L, 1, precision=precision) as lat:
         lat.set_params(TAU, TAU_NH, 1.0, O.COL_EXT_REG)
+        lat.set_arith(arith)
         lat.upload(f0)
         shape, spl, desc = lat.step_shape()
-        lat.step(N)
+        for n in calls:
+            lat.step(n)
         assert lat.status() == (N, 0)
         return lat.download(), (shape, spl, desc)
 
@@ -66,14 +77,35 @@ def test_dsl2048_1000_steps_default_kernel_bitwise(gpu, dsl_setup, strict_1000):
     assert np.array_equal(got, strict_1000)
 
 
-def test_dsl2048_1000_steps_within_1e10_of_fma_oracle(gpu, dsl_setup):
+def test_dsl2048_1000_steps_within_1e10_of_fma_oracle(gpu, dsl_setup, fma_1000):
     f0, st = dsl_setup
-    print(f"oracle (FMA): DSL L={L}, {N} steps, {NTHREADS} threads", flush=True)
-    want = O.Oracle(fma=True).run_steps(9, f0, st, N, O.COL_EXT_REG, TAU, TAU_NH, nthreads=NTHREADS)
     got, _ = _gpu_run(gpu, f0, gpu.FP64)
-    err = float(np.max(np.abs(got - want) / np.abs(want)))
+    err = float(np.max(np.abs(got - fma_1000) / np.abs(fma_1000)))
     print(f"max relative population error vs the FMA build: {err:.3e}", flush=True)
     assert err <= 1e-10
+
+
+def test_dsl2048_moment_arith_1000_steps_within_1e10(gpu, dsl_setup, strict_1000, fma_1000):
+    f0, _ = dsl_setup
+    got, (shape, spl, desc) = _gpu_run(gpu, f0, gpu.FP64, gpu.ARITH_MOMENT)
+    print(f"GPU default kernel, moment-space arithmetic: {desc} (shape {shape}, {spl} steps per launch)", flush=True)
+    assert spl == 3
+    e_strict = float(np.max(np.abs(got - strict_1000) / np.abs(strict_1000)))
+    e_fma = float(np.max(np.abs(got - fma_1000) / np.abs(fma_1000)))
+    print(f"max relative population error: {e_strict:.3e} vs strict, {e_fma:.3e} vs FMA build", flush=True)
+    assert 0.0 < e_strict <= 1e-10
+    assert e_fma <= 1e-10
+
+
+def test_dsl2048_moment_arith_call_pattern(gpu, dsl_setup, strict_1000):
+    """The same 1000 steps in calls of 500, 1, 250 and 249 steps: one- and
+    two-step remainders (the two-step strip walk) between three-step
+    launches."""
+    f0, _ = dsl_setup
+    got, _ = _gpu_run(gpu, f0, gpu.FP64, gpu.ARITH_MOMENT, calls=(500, 1, 250, 249))
+    err = float(np.max(np.abs(got - strict_1000) / np.abs(strict_1000)))
+    print(f"calls 500+1+250+249: max relative population error {err:.3e} vs strict", flush=True)
+    assert 0.0 < err <= 1e-10
 
 
 def test_dsl2048_fp32_shifted_1000_steps(gpu, monkeypatch, dsl_setup, strict_1000):
