@@ -10,11 +10,14 @@ resident in HBM.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 
-Prints ONE JSON line: the FP64 MLUPS (`value`), the roofline of the step
-kernel (144 algorithmic bytes per node-step: 9 FP64 populations read + 9
-written) against MI355X's 8 TB/s, and the reference's own CPU step loop
-(compiled from its sources into oracle/_ref, OpenMP on the host cores) timed
-on a bounded sample as the CPU baseline.  The rest of the metric rides along:
+Prints ONE JSON line: the FP64 MLUPS (`value`; the collision in moment space
+by default, within 1e-10 max relative population error of the reference
+after 1000 steps, with the bitwise path's MLUPS beside it), the roofline of
+the step kernel (144 algorithmic bytes per node and launch: 9 FP64
+populations read + 9 written, for the steps one launch advances) against
+MI355X's 8 TB/s, and the reference's own CPU step loop (compiled from its
+sources into oracle/_ref, OpenMP on the host cores) timed on a bounded sample
+as the CPU baseline.  The rest of the metric rides along:
 FP32 / FP16 shifted storage of the same run (`precision`), the full 400 x 400
 stability maps in points/s against the plain one-run-per-launch FP64 path
 (`maps`), and the continuous-batching serving mix with p50/p99 latency
@@ -47,6 +50,10 @@ BYTES_PER_NODE = {"fp64": 144, "fp32": 72, "fp16": 36}
 # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (the 78.6 TFLOP/s FP64 vector
 # figure counts an FMA as two)
 FP64_INSTR_PER_NODE_STEP = 175
+# the same count for the moment-space collision (ERLBM_ARITH_MOMENT: 210
+# v_add_f64, 108 v_mul_f64, 162 v_fma_f64 / v_fmac_f64, 12 v_rcp_f64 and the
+# range tests of 1/rho per six collisions of the walking loop)
+FP64_INSTR_PER_NODE_STEP_MOMENT = 85
 FP64_VALU_PEAK_T = 39.3
 # rocprofv3 PMC traffic per launch (FETCH_SIZE + WRITE_SIZE, calibrated against
 # a plain copy kernel), measured on the FP64 step kernel
@@ -317,9 +324,10 @@ def main() -> int:
                     help="run only the calibration copy kernel (302 MB read + 302 MB written per launch)")
     ap.add_argument("--only", choices=["d3q19", "d3q19_moment", "maps", "serving"],
                     help="run one section alone and print it (profiling runs)")
-    ap.add_argument("--arith", default="exact", choices=["exact", "moment"],
-                    help="collision arithmetic of the headline run: the reference's operation order (bitwise its "
-                         "strict build) or the same operator in moment space (within 1e-10)")
+    ap.add_argument("--arith", default="moment", choices=["exact", "moment"],
+                    help="collision arithmetic of the headline run: the same operator in moment space (within 1e-10 "
+                         "of the reference after 1000 steps, the default) or the reference's operation order "
+                         "(bitwise its strict build; timed beside the headline either way)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -343,11 +351,22 @@ def main() -> int:
         print(json.dumps({"copy_bytes_each_way": copy_bytes, "ms": copy_ms, "GBs": copy_gbs}))
         return 0
 
-    ok, head = dsl_run(E, a.precision, a.steps, a.warmup, tune=a.tune, arith=a.arith)
+    arith = a.arith if a.precision == "fp64" else "exact"
+    ok, head = dsl_run(E, a.precision, a.steps, a.warmup, tune=a.tune, arith=arith)
     if not ok:
         print(f"bench.py: run check failed {head}", file=sys.stderr)
         return 1
     extra = {}
+    if a.precision == "fp64":
+        # the other arithmetic on the same workload: the bitwise path beside a
+        # moment-space headline, or the moment-space one beside a bitwise headline
+        other = "exact" if arith == "moment" else "moment"
+        oko, ro = dsl_run(E, "fp64", a.steps, a.warmup, tune=a.tune, arith=other)
+        keep = ("MLUPS", "ms_per_step", "kernel_MLUPS", "ms_per_launch", "launch", "steps_per_launch", "mass_rel_error")
+        extra["fp64_" + other + "_arith"] = {**{k: ro[k] for k in keep},
+                                             "parity": ("bitwise the reference's strict build" if other == "exact" else
+                                                        "within 1e-10 max relative population error after 1000 steps"),
+                                             **({} if oko else {"error": "run check failed"})}
     if not a.quick:
         prec = {}
         for p in ("fp64", "fp32", "fp16"):
@@ -362,6 +381,7 @@ def main() -> int:
                                     "(tests/test_gpu_lowp.py); L=2048 to t=L/U0: full_run_to_t_L_over_U0")
         extra["precision"] = prec
         extra["d3q19"] = d3q19_section(E)
+        extra["d3q19_moment_arith"] = d3q19_section(E, arith="moment")
         extra["maps"] = maps_section(E)
         if not a.no_cpu:
             cm = cpu_maps_baseline()
@@ -380,7 +400,7 @@ def main() -> int:
         # this run used, as profiled (rocprofv3 PMC runs of the same command)
         for entry in measured.get("kernels", []):
             if entry.get("launch") and entry.get("precision", "fp64") == a.precision and \
-                    head["launch"].startswith(entry["launch"]):
+                    entry.get("arith", "exact") == arith and head["launch"].startswith(entry["launch"]):
                 traffic = entry["bytes_per_launch"]
                 valu_busy = entry.get("valu_busy")
                 bound = entry.get("bound", "hbm")
@@ -390,7 +410,8 @@ def main() -> int:
     spl = head["steps_per_launch"]
     # the collisions' FP64 vector instructions per second in the kernel
     # (algorithmic, halo recomputation excluded) against the FP64 issue peak
-    fp64_t = head["kernel_MLUPS"] * 1e6 * FP64_INSTR_PER_NODE_STEP / 1e12
+    instr = FP64_INSTR_PER_NODE_STEP_MOMENT if arith == "moment" else FP64_INSTR_PER_NODE_STEP
+    fp64_t = head["kernel_MLUPS"] * 1e6 * instr / 1e12
     line = {
         "metric": "D2Q9 FP64 MLUPS (%HBM peak); map points/s + p99 latency, FP32 vs FP64",
         "value": head["MLUPS"],
@@ -408,6 +429,10 @@ def main() -> int:
         "config": {"workload": "D2Q9 E-RLBM double shear layer, periodic L=2048 (4,194,304 nodes), "
                                f"tau=0.5024, tau_nh=0.65, {a.precision} row-blocked pull-stream+collide",
                    "L": L, "tau": TAU, "tau_nh": TAU_NH, "collision": "E-RLBM", "precision": a.precision,
+                   "arith": {"moment": "moment-space collision (ERLBM_ARITH_MOMENT): within 1e-10 max relative "
+                                       "population error of the reference after 1000 steps",
+                             "exact": "the reference's operation order (ERLBM_ARITH_EXACT): bitwise its strict "
+                                      "build"}[arith],
                    "launch": head["launch"]},
         # the dominant kernel: one launch advances `steps_per_launch` steps and
         # reads + writes every population once (algorithmic bytes), its
@@ -423,7 +448,7 @@ def main() -> int:
                      "one_step_equivalent_GBs": round(head["kernel_MLUPS"] * 1e6 * BYTES_PER_NODE[a.precision] / 1e9, 1),
                      "valu_busy_profiled": valu_busy,
                      "fp64_issue_frac": round(fp64_t / FP64_VALU_PEAK_T, 4)},
-        "fp64_valu": {"instr_per_node_step": FP64_INSTR_PER_NODE_STEP,
+        "fp64_valu": {"instr_per_node_step": instr,
                       "achieved_T_per_s": round(fp64_t, 2),
                       "peak_T_per_s": FP64_VALU_PEAK_T,
                       "frac": round(fp64_t / FP64_VALU_PEAK_T, 4)},

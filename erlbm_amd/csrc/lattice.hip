@@ -1570,6 +1570,7 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
 // up); shifted FP32 storage: the lane-aligned three-step walk, else the
 // two-step strip walk.
 long long strip3_workgroups(const erlbm_lattice *h, int f);
+long long astrip4_workgroups(const erlbm_lattice *h);
 
 int fuse_shape(const erlbm_lattice *h, int energy_interval)
 {
@@ -1587,6 +1588,15 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
     // against 49.8 for the two-step walk and 50.0 for the 256-wide
     // three-step one), else the two-step walk
     const int first = h->elem == 8 ? h->tune_fuse : FUSE3_ASTRIP_4W;
+    // FP64: the four-step lane-aligned walk where it makes at least 1.5
+    // workgroups per CU (MLUPS, four-step / three-step rule below, exact and
+    // moment-space arithmetic: L = 1280 81.0 / 74.3, 100.5 / 87.0; 1536 94.5 /
+    // 84.4, 110.2 / 89.7; 2048 108.3 / 95.8, 129.6 / 102.8; 3000 114.5 /
+    // 93.1, 123.0 / 96.4; 4096 120.6 / 105.1, 141.0 / 111.4; L = 1024, 320
+    // workgroups: 52.9 / 60.1, 71.0 / 74.4)
+    if (h->elem == 8 && first == FUSE3_STRIP_SIZED && fuse_shape_fits(h, FUSE4_ASTRIP_4W) &&
+        2 * astrip4_workgroups(h) >= 3ll * h->n_cu)
+        return FUSE4_ASTRIP_4W;
     // FP64: the lane-aligned walk on strips of any width (four wavefronts,
     // four workgroups per CU) before the 256-wide walk where it puts no more
     // workgroups on a CU than that one, or makes three per CU or more
@@ -2120,6 +2130,21 @@ int astrip4_per_cu()
         return nblk;
     }();
     return per_cu;
+}
+
+// Workgroups one launch of the four-step walk makes here.
+long long astrip4_workgroups(const erlbm_lattice *h)
+{
+    int pc = 1;
+    switch (h->col) {
+    case ERLBM_COL_BGK: pc = astrip4_per_cu<COL_BGK, 4>(); break;
+    case ERLBM_COL_MRT: pc = astrip4_per_cu<COL_MRT, 4>(); break;
+    case ERLBM_COL_REG: pc = astrip4_per_cu<COL_REG, 4>(); break;
+    default: pc = astrip4_per_cu<COL_EXT_REG, 4>(); break;
+    }
+    int nstrip, nseg;
+    strip3_grid(h, 250, true, pc, nstrip, nseg, astrip_unit(h));
+    return (long long)nstrip * nseg;
 }
 
 template <int COL, int NW>

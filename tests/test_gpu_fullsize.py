@@ -1,7 +1,8 @@
 """The double-shear-layer workload at its full size: D2Q9 E-RLBM, periodic
 L = 2048 (4,194,304 nodes), tau = 0.5 + 3 U0 L / Re (Re = 80000, U0 = 1/32),
 tau_nh = 0.65, equilibrium initialisation of src/main_double-shear-layer.cpp:
-48-66, 1000 steps, as the default kernels run it (three steps per launch).
+48-66, 1000 steps, as the default kernels run it (four steps per launch in
+FP64, three in FP32-shifted storage), and on the three-step walk.
 
   * FP64: bitwise the strict oracle (the reference's operation order, no FMA
     contraction); max relative population error <= 1e-10 against the
@@ -69,11 +70,16 @@ def _vel(f):
     return (f @ c) / f.sum(1)[:, None]
 
 
-def test_dsl2048_1000_steps_default_kernel_bitwise(gpu, dsl_setup, strict_1000):
+@pytest.mark.parametrize("fuse,want_spl", [(None, 4), ("42", 3)])
+def test_dsl2048_1000_steps_default_kernel_bitwise(gpu, monkeypatch, dsl_setup, strict_1000, fuse, want_spl):
+    """The default kernel (four steps per launch) and the three-step
+    lane-aligned walk."""
     f0, _ = dsl_setup
+    if fuse:
+        monkeypatch.setenv("ERLBM_FUSE", fuse)
     got, (shape, spl, desc) = _gpu_run(gpu, f0, gpu.FP64)
-    print(f"GPU default kernel: {desc} (shape {shape}, {spl} steps per launch)", flush=True)
-    assert spl == 3
+    print(f"GPU kernel: {desc} (shape {shape}, {spl} steps per launch)", flush=True)
+    assert spl == want_spl
     assert np.array_equal(got, strict_1000)
 
 
@@ -85,11 +91,15 @@ def test_dsl2048_1000_steps_within_1e10_of_fma_oracle(gpu, dsl_setup, fma_1000):
     assert err <= 1e-10
 
 
-def test_dsl2048_moment_arith_1000_steps_within_1e10(gpu, dsl_setup, strict_1000, fma_1000):
+@pytest.mark.parametrize("fuse,want_spl", [(None, 4), ("42", 3)])
+def test_dsl2048_moment_arith_1000_steps_within_1e10(gpu, monkeypatch, dsl_setup, strict_1000, fma_1000, fuse,
+                                                     want_spl):
     f0, _ = dsl_setup
+    if fuse:
+        monkeypatch.setenv("ERLBM_FUSE", fuse)
     got, (shape, spl, desc) = _gpu_run(gpu, f0, gpu.FP64, gpu.ARITH_MOMENT)
-    print(f"GPU default kernel, moment-space arithmetic: {desc} (shape {shape}, {spl} steps per launch)", flush=True)
-    assert spl == 3
+    print(f"GPU kernel, moment-space arithmetic: {desc} (shape {shape}, {spl} steps per launch)", flush=True)
+    assert spl == want_spl
     e_strict = float(np.max(np.abs(got - strict_1000) / np.abs(strict_1000)))
     e_fma = float(np.max(np.abs(got - fma_1000) / np.abs(fma_1000)))
     print(f"max relative population error: {e_strict:.3e} vs strict, {e_fma:.3e} vs FMA build", flush=True)
@@ -98,9 +108,9 @@ def test_dsl2048_moment_arith_1000_steps_within_1e10(gpu, dsl_setup, strict_1000
 
 
 def test_dsl2048_moment_arith_call_pattern(gpu, dsl_setup, strict_1000):
-    """The same 1000 steps in calls of 500, 1, 250 and 249 steps: one- and
-    two-step remainders (the two-step strip walk) between three-step
-    launches."""
+    """The same 1000 steps in calls of 500, 1, 250 and 249 steps: one-,
+    two- and three-step remainders (the two-step strip walk, the three-step
+    lane-aligned walk) between four-step launches."""
     f0, _ = dsl_setup
     got, _ = _gpu_run(gpu, f0, gpu.FP64, gpu.ARITH_MOMENT, calls=(500, 1, 250, 249))
     err = float(np.max(np.abs(got - strict_1000) / np.abs(strict_1000)))

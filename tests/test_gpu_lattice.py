@@ -504,7 +504,7 @@ def test_lattice_errors_leave_the_handle_usable(gpu, orc):
 
 def test_headline_default_kernel_full_size(gpu, orc):
     """The headline workload as the bench runs it (D2Q9 double shear,
-    L = 2048, the default multi-step kernel: three steps per launch): 25
+    L = 2048, the default multi-step kernel: four steps per launch): 25
     steps bitwise the strict oracle; total mass conserved to rounding."""
     L, n = 2048, 25
     tau = 0.5 + 3. * (1. / 32.) * L / 80000.
@@ -514,7 +514,7 @@ def test_headline_default_kernel_full_size(gpu, orc):
     with _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0) as lat:
         shape, spl, desc = lat.step_shape()
         print(f"headline default: shape {shape}, {desc}", flush=True)
-        assert spl == 3
+        assert spl == 4
         lat.step(n)
         got = lat.download()
     assert np.array_equal(got, want)
@@ -542,18 +542,21 @@ def test_aligned_walk_issue_priority_bitwise(gpu, orc, monkeypatch, prio):
 
 def test_default_shape_rule(gpu):
     """The default multi-step shape per lattice size (periodic FP64 D2Q9
-    E-RLBM; gfx950 parts have 256 CUs): the lane-aligned any-width walk
-    where it loads the CUs no more unevenly than the 256-wide walk or makes
-    three workgroups per CU, the 256-wide walk at L = 1024, the 64x8 two-step
-    tiles at L = 768, and steps_per_launch / descriptions that match."""
-    want = {768: (1, 2), 1000: (42, 3), 1024: (26, 3), 1280: (42, 3), 2048: (42, 3), 3000: (42, 3), 4096: (42, 3)}
+    E-RLBM; gfx950 parts have 256 CUs): the four-step lane-aligned walk
+    where it makes 1.5 workgroups per CU or more, else the three-step
+    lane-aligned any-width walk where it loads the CUs no more unevenly
+    than the 256-wide walk or makes three workgroups per CU, the 256-wide
+    walk at L = 1024, the 64x8 two-step tiles at L = 768, and
+    steps_per_launch / descriptions that match."""
+    want = {768: (1, 2), 1000: (42, 3), 1024: (26, 3), 1280: (46, 4), 2048: (46, 4), 3000: (46, 4), 4096: (46, 4)}
+    words = {2: "two steps", 3: "three steps", 4: "four steps"}
     for L, (shape, spl) in want.items():
         with gpu.Lattice(9, L, L, 1) as lat:
             lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
             got = lat.step_shape()
             print(f"L={L}: {got}", flush=True)
             assert got[:2] == (shape, spl), L
-            assert got[2].startswith("three steps" if spl == 3 else "two steps")
+            assert got[2].startswith(words[spl])
     # shifted storage: FP32 takes the lane-aligned walk, FP16 one step per launch
     for prec, want_shape in ((gpu.FP32_SHIFTED, (42, 3)), (gpu.FP16_SHIFTED, (0, 1))):
         with gpu.Lattice(9, 2048, 2048, 1, precision=prec) as lat:
@@ -657,20 +660,24 @@ def test_d3q19_zwalk_monitors_equal_one_step(gpu, monkeypatch):
 
 def test_two_step_remainder_bitwise(gpu, orc, monkeypatch):
     """Calls whose step count leaves two steps after the three-step launches
-    run them as one two-step launch: at L = 2048, steps 4 (1 + 3), 5 (3 + 2),
-    2 (a lone two-step launch) and 1, bitwise the strict oracle (FP64) and
-    the one-step path (FP32 shifted storage)."""
+    run them as one two-step launch: at L = 2048 on the three-step
+    lane-aligned walk, steps 4 (1 + 3), 5 (3 + 2), 2 (a lone two-step launch)
+    and 1, bitwise the strict oracle (FP64) and the one-step path (FP32
+    shifted storage).  (Remainders after four-step launches:
+    test_gpu_walk4.py.)"""
     L, calls = 2048, (4, 5, 2, 1)
     tau = 0.5 + 3. * (1. / 32.) * L / 80000.
     f0 = gpu.init_double_shear(L)
     ini, st, nf = orc.stream_map(9, L, L, 1)
     want = orc.run_steps(9, f0, st, sum(calls), O.COL_EXT_REG, tau, 0.65, nthreads=16)
+    monkeypatch.setenv("ERLBM_FUSE", "42")
     with _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0) as lat:
         assert lat.step_shape()[1] == 3
         for n in calls:
             lat.step(n)
         assert lat.status() == (sum(calls), 0)
         assert np.array_equal(lat.download(), want)
+    monkeypatch.delenv("ERLBM_FUSE")
     out = []
     for fuse in (None, "0"):
         if fuse is not None:
