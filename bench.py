@@ -148,16 +148,19 @@ def dsl_run(E, precision: str, steps: int, warmup: int, tune: bool = True, arith
 def dsl_full_run_accuracy(E):
     """The double-shear run to t = L/U0 (round(L/U0) + 1 = 65,537 steps at
     L = 2048, as src/main_double-shear-layer.cpp counts them) in each
-    storage precision: wall time of the whole run, and the L2 velocity error
-    of the FP32 / FP16 runs against the FP64 one (tools/error_calculator's
-    E_vector / E_magnitude)."""
+    storage precision (FP64 in both arithmetics): wall time of the whole
+    run, and the L2 velocity error of the other runs against the FP64 one
+    in the reference's operation order (tools/error_calculator's E_vector /
+    E_magnitude)."""
     from erlbm_amd import fields
     n = int(round(L / U0)) + 1
     out, ref = {"steps": n}, None
-    for p, prec in (("fp64", E.FP64), ("fp32", E.FP32_SHIFTED), ("fp16", E.FP16_SHIFTED)):
+    for p, prec, ar in (("fp64", E.FP64, E.ARITH_EXACT), ("fp64_moment_arith", E.FP64, E.ARITH_MOMENT),
+                        ("fp32", E.FP32_SHIFTED, E.ARITH_EXACT), ("fp16", E.FP16_SHIFTED, E.ARITH_EXACT)):
         f0 = E.init_double_shear(L)
         with E.Lattice(9, L, L, 1, precision=prec) as lat:
             lat.set_params(TAU, TAU_NH, 1.0, E.COL_EXT_REG)
+            lat.set_arith(ar)
             lat.upload(f0)
             del f0
             ms = lat.step_timed(n)
