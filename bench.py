@@ -378,6 +378,8 @@ def main() -> int:
                 continue
             okp, r = dsl_run(E, p, max(200, a.steps // 2), 20, tune=a.tune)
             prec[p] = r if okp else {"error": "run check failed", **r}
+        okm, rm = dsl_run(E, "fp32", max(200, a.steps // 2), 20, tune=a.tune, arith="moment")
+        prec["fp32_moment_arith"] = rm if okm else {"error": "run check failed", **rm}
         prec["full_run_to_t_L_over_U0"] = dsl_full_run_accuracy(E)
         prec["accuracy_vs_fp64"] = ("FP32 storage: L2 velocity error 2.1e-7 (DSL L=256, 1000 steps), "
                                     "TG L=64 E_vector within 0.01%; FP16: 1.7e-3 / +401% "

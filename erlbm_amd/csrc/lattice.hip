@@ -1386,8 +1386,8 @@ extern "C" int erlbm_set_arith(erlbm_lattice *h, int arith)
         set_last_error_msg("erlbm_set_arith: arith must be ERLBM_ARITH_EXACT or ERLBM_ARITH_MOMENT");
         return ERLBM_ERR_ARG;
     }
-    if (arith == ERLBM_ARITH_MOMENT && h->elem != 8) {
-        set_last_error_msg("erlbm_set_arith: the moment-space arithmetic needs FP64 storage");
+    if (arith == ERLBM_ARITH_MOMENT && (h->elem == 2 || h->f32_arith)) {
+        set_last_error_msg("erlbm_set_arith: the moment-space arithmetic needs FP64 arithmetic on FP64 or FP32 storage");
         return ERLBM_ERR_UNSUPPORTED;
     }
     h->arith = arith;
@@ -1450,7 +1450,7 @@ extern "C" int erlbm_upload_aos(erlbm_lattice *h, const double *inif)
 namespace {
 
 // Shapes of the two-step kernel (ERLBM_FUSE selects one; 0 = off).
-enum FuseKind { FUSE_TILE, FUSE_STRIP, FUSE_WAVE, FUSE_STRIP3, FUSE_ZWALK, FUSE_GSTRIP3, FUSE_GSTRIP4 };
+enum FuseKind { FUSE_TILE, FUSE_STRIP, FUSE_WAVE, FUSE_STRIP3, FUSE_ZWALK, FUSE_GSTRIP3, FUSE_GSTRIP4, FUSE_GSTRIP5 };
 struct FuseShape {
     int tx, ty;       // tile; (strip) strip width and rows walked; (wave) rows walked
     FuseKind kind;
@@ -1483,9 +1483,9 @@ constexpr FuseShape FUSE_SHAPES[] = {
     // in registers), 2, 3, 4, 8, 5 and 6 wavefronts per workgroup
     {124, 0, FUSE_GSTRIP3},            {188, 0, FUSE_GSTRIP3},           {252, 0, FUSE_GSTRIP3},
     {508, 0, FUSE_GSTRIP3},            {316, 0, FUSE_GSTRIP3},           {380, 0, FUSE_GSTRIP3},
-    // four steps per launch: lane-aligned strips of any width up to 64 NW - 6,
-    // four wavefronts
-    {250, 0, FUSE_GSTRIP4}};
+    // four and five steps per launch: lane-aligned strips of any width up to
+    // 64 NW - 6 / 64 NW - 8, four wavefronts
+    {250, 0, FUSE_GSTRIP4},            {248, 0, FUSE_GSTRIP5}};
 constexpr int N_FUSE_SHAPES = sizeof(FUSE_SHAPES) / sizeof(FUSE_SHAPES[0]);
 
 // Steps one launch of the shape advances.
@@ -1493,7 +1493,7 @@ int fuse_steps(int f)
 {
     if (f <= 0 || f >= N_FUSE_SHAPES) return 2;
     const FuseKind k = FUSE_SHAPES[f].kind;
-    return k == FUSE_GSTRIP4 ? 4 : (k == FUSE_STRIP3 || k == FUSE_GSTRIP3 ? 3 : 2);
+    return k == FUSE_GSTRIP5 ? 5 : (k == FUSE_GSTRIP4 ? 4 : (k == FUSE_STRIP3 || k == FUSE_GSTRIP3 ? 3 : 2));
 }
 
 // Segments per strip for the wavefront-strip kernel: `rounds` times as many
@@ -1516,8 +1516,8 @@ bool fuse_geometry_fits(const erlbm_lattice *h, const FuseShape &fs)
     }
     if (h->nz != 1) return false;
     if (fs.kind == FUSE_WAVE) return h->nx >= 64 && h->extent * 8 < (1ll << 31);   // 31-bit buffer offsets
-    if (fs.kind == FUSE_GSTRIP3 || fs.kind == FUSE_GSTRIP4)
-        return h->extent * h->elem < (1ll << 31) && h->nx >= 16 && h->ny >= 16 && (fs.kind == FUSE_GSTRIP3 || h->elem == 8);
+    if (fs.kind == FUSE_GSTRIP3 || fs.kind == FUSE_GSTRIP4 || fs.kind == FUSE_GSTRIP5)
+        return h->extent * h->elem < (1ll << 31) && h->nx >= 16 && h->ny >= 16;
     if ((fs.kind == FUSE_STRIP || fs.kind == FUSE_STRIP3) && h->elem == 8 && h->extent * 8 >= (1ll << 31)) return false;
     if (fs.ty == 0) return h->nx % fs.tx == 0 && h->ny >= 16;
     return h->nx % fs.tx == 0 && h->ny % fs.ty == 0;
@@ -1552,7 +1552,8 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
     // VALU-bound once the bytes are that few)
     if (h->elem != 8) {
         const bool two = f == FUSE2_STRIP_256x32 || f == FUSE2_STRIP_128x32;
-        const bool three = f == FUSE3_STRIP_256x32 || f == FUSE3_STRIP_128x32 || f == FUSE3_ASTRIP_4W;
+        const bool three = f == FUSE3_STRIP_256x32 || f == FUSE3_STRIP_128x32 || f == FUSE3_ASTRIP_4W ||
+                           f == FUSE4_ASTRIP_4W;
         if (h->f32_arith || !(two || (three && h->elem == 4))) return false;
         if (h->elem == 2 && !h->fuse_requested) return false;
         if (h->extent * h->elem >= (1ll << 31)) return false;   // 31-bit buffer offsets
@@ -1563,14 +1564,16 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
 // The multi-step shape a call of erlbm_step uses (0: one step per launch).
 // Without an explicit ERLBM_FUSE, a shape must give every CU a workgroup
 // (small lattices have too few strips or tiles to fill the chip: there the
-// one-step kernel is faster).  FP64: the three-step walk, lane-aligned on
-// strips of any width or on 256-wide strips (the rule below), else on
-// strips of any width (four wavefronts, lanes shifted), else the two-step
-// walk, else the 64x8 tiles, else the wavefront strips (any width from 64
-// up); shifted FP32 storage: the lane-aligned three-step walk, else the
-// two-step strip walk.
+// one-step kernel is faster).  FP64: the four-step lane-aligned walk, else
+// the three-step walk, lane-aligned on strips of any width or on 256-wide
+// strips (the rule below), else on strips of any width (four wavefronts,
+// lanes shifted), else the two-step walk, else the 64x8 tiles, else the
+// wavefront strips (any width from 64 up); shifted FP32 storage: the
+// four-step lane-aligned walk, else the three-step one, else the two-step
+// strip walk.
 long long strip3_workgroups(const erlbm_lattice *h, int f);
 long long astrip4_workgroups(const erlbm_lattice *h);
+long long astrip5_workgroups(const erlbm_lattice *h);
 
 int fuse_shape(const erlbm_lattice *h, int energy_interval)
 {
@@ -1594,7 +1597,19 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
     // 84.4, 110.2 / 89.7; 2048 108.3 / 95.8, 129.6 / 102.8; 3000 114.5 /
     // 93.1, 123.0 / 96.4; 4096 120.6 / 105.1, 141.0 / 111.4; L = 1024, 320
     // workgroups: 52.9 / 60.1, 71.0 / 74.4)
-    if (h->elem == 8 && first == FUSE3_STRIP_SIZED && fuse_shape_fits(h, FUSE4_ASTRIP_4W) &&
+    // FP64 with the moment-space collision: the five-step walk where it makes
+    // at least 1.5 workgroups per CU (MLUPS, five-step / four-step: L = 1280
+    // 111.9 / 100.8; 1536 126.5 / 111.0; 2048 143.4 / 128.6; 3000 138.9 /
+    // 123.4; 4096 160.7 / 140.7; with the reference's operation order the
+    // five-step walk is no faster, 79.9 / 80.0 at L = 1280, 103.2 / 106.3
+    // at 2048: FP64 issue binds it)
+    if (h->elem == 8 && first == FUSE3_STRIP_SIZED && h->arith == ERLBM_ARITH_MOMENT && h->col != ERLBM_COL_MRT &&
+        fuse_shape_fits(h, FUSE5_ASTRIP_4W) && 2 * astrip5_workgroups(h) >= 3ll * h->n_cu)
+        return FUSE5_ASTRIP_4W;
+    // FP32 storage likewise (L = 2048, MLUPS, four-step / three-step, exact
+    // and moment-space arithmetic: 111.6 / 107.7, 162.6 / 155.2; L = 4096
+    // 125.8 / 122.8, 193.7 / 180.4; L = 1280 78.7 / 79.4, 112.0 / 109.2)
+    if (((h->elem == 8 && first == FUSE3_STRIP_SIZED) || h->elem == 4) && fuse_shape_fits(h, FUSE4_ASTRIP_4W) &&
         2 * astrip4_workgroups(h) >= 3ll * h->n_cu)
         return FUSE4_ASTRIP_4W;
     // FP64: the lane-aligned walk on strips of any width (four wavefronts,
@@ -1840,174 +1855,31 @@ __global__ __launch_bounds__(64 * NW) void k_step3_astrip(StepArgs a, int remap,
     else strip3_walk_aligned<COL, NW, 1, NTP>(a, x0, w, ya, rows, rings, rings + XRING_ROWS * E, prio);
 }
 
-// Four steps per launch: the lane-aligned walk with a third ring.  Lane t
-// holds position x0 - 3 + t in all four phases (phase 1: lanes 0 .. w+5,
-// 2: 1 .. w+4, 3: 2 .. w+3, 4 and the stores: 3 .. w+2).  With two barriers
-// per walking step the phases alternate between the two intervals they
-// make: walking step n (row r = r0 + DIR n) runs phase 1 on row r before
-// the first barrier, phases 2 (row r - DIR) and 4 (row r - 4 DIR) between
-// the barriers, and phase 3 (row r - 2 DIR) after the second; phase 4 so
-// reads the ring-3 rows phase 3 wrote in the walking steps before.  Rings
-// 1, 2 and 3 keep the c_x != 0 results of phases 1, 2 and 3 (lean layout,
-// 8 rows each: 48 KB for NW = 4, three workgroups per CU), the c_x = 0 ones
-// stay in registers; a ring's late writes (the populations not read from
-// the newest row) follow its first reader by one interval as in
-// strip3_walk_aligned: ring 1 after the second barrier, ring 2 after the
-// first barrier of the next walking step, ring 3 after the second barrier
-// of the next walking step.  Output rows ya .. ya+rows-1 need step-k rows
-// ya-3 .. ya+rows+2: rows + 7 walking steps.
-template <int COL, int NW, int DIR>
-__device__ __forceinline__ void strip4_walk_aligned(const StepArgs &a, int x0, int w, int ya, int rows, double *ring1,
-                                                    double *ring2, double *ring3, int prio)
-{
-    using V = VSet<9>;
-    constexpr int E = 64 * NW;
-    const int t = (int)threadIdx.x;
-    const int nx = a.g.nx, ny = a.g.ny;
-    const bool act1 = t < w + 6, act2 = t >= 1 && t < w + 5, act3 = t >= 2 && t < w + 4, act4 = t >= 3 && t < w + 3;
-    const int x1 = act1 ? wrap(x0 - 3 + t, nx) : 0;
-    const __amdgpu_buffer_rsrc_t rs = pop_buffer<8>(a.src, a.g), rd = pop_buffer<8>(a.dst, a.g);
-    const unsigned oob1 = act1 ? 0u : BUF_OOB;
-    const unsigned vo[3] = {8u * wrap(x1 + 1, nx) | oob1, 8u * x1 | oob1, 8u * wrap(x1 - 1, nx) | oob1};
-    const unsigned vst = act4 ? 8u * (x0 - 3 + t) : 0u;
-    auto load_row = [&](int y, double (&f)[9], bool skip) {   // pulls of step-k row y (-4 .. ny+3)
-        const unsigned off = skip ? BUF_OOB : 0u;
-        const int yw = wrap(y, ny);
-        static_for<9>([&](auto I) {
-            constexpr int i = I;
-            const int ys = V::cy[i] == 0 ? yw : wrap(yw - V::cy[i], ny);
-            f[i] = buf_load<0>(rs, vo[V::cx[i] + 1] | off, (int)(8 * at(a.g, i, ys, 0)));
-        });
-    };
-    auto rrow = [&](int i, int y) { return xring_base<DIR>(i) + (y - ya + 4) % xring_depth<DIR>(i); };
-    auto inring = [](int i) { return V::cx[i] != 0; };
-    auto now = [](int i) { return V::cy[i] * DIR < 0; };
-    auto age = [](int i) { return 1 + V::cy[i] * DIR; };
-    auto collide = [&](double (&f)[9]) {
-        double vx, vy, vz;
-        collide_sel<9, COL>(f, a.k, vx, vy, vz);
-    };
-    // pull of a phase from the ring of the phase before (c_x != 0) and from
-    // that phase's results in registers (c_x = 0: this lane's own, of the
-    // newest, middle and oldest source row)
-    auto pull = [&](double (&g)[9], const double *ring, int y, const double (&r0)[9], const double (&r1)[9],
-                    const double (&r2)[9]) {
-        static_for<9>([&](auto I) {
-            constexpr int i = I;
-            if constexpr (V::cx[i] != 0) g[i] = ring[rrow(i, y - V::cy[i]) * E + (t - V::cx[i])];
-            else g[i] = age(i) == 0 ? r0[i] : (age(i) == 1 ? r1[i] : r2[i]);
-        });
-    };
-    const int r0 = DIR > 0 ? ya - 3 : ya + rows + 2;
-    const int nsteps = rows + 7, nrows1 = rows + 6;   // walking steps; rows of phase 1
-    double fq[9];
-    double h2[9], h3[9];           // the previous walking step's phase-2 / phase-3 rows (late ring writes)
-    double k1a[9], k1b[9];         // c_x = 0: phase-1 results of the rows one and two walking steps back
-    double k2a[9], k2b[9];         // phase 2, likewise
-    double k3a[9], k3b[9];         // phase 3, one and two walking steps before h3
-    load_row(r0, fq, false);
-    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
-    nine_dropped_stores<double>(rd);
-    for (int n = 0; n < nsteps; n++) {
-        if (prio) progress_prio(n, nsteps, prio);
-        const int r = r0 + DIR * n;
-        double f[9];
-        static_for<9>([&](auto I) { f[(int)I] = fq[(int)I]; });
-        load_row(r + DIR, fq, n + 1 >= nrows1);                      // in flight meanwhile
-        const bool p1 = act1 && n < nrows1;
-        if (p1) {                                                     // phase 1: step k, row r
-            collide(f);
-            static_for<9>([&](auto I) {
-                if (inring(I) && now(I)) ring1[rrow((int)I, r) * E + t] = f[(int)I];
-            });
-        }
-        lds_barrier();
-        if (act2 && n >= 3) {                                         // the rest of phase-2 row r-2 DIR
-            static_for<9>([&](auto I) {
-                if (inring(I) && !now(I)) ring2[rrow((int)I, r - 2 * DIR) * E + t] = h2[(int)I];
-            });
-        }
-        if (act2 && n >= 2 && n < nrows1) {                           // phase 2: step k+1, row r-DIR
-            const int y = r - DIR;
-            double g[9];
-            pull(g, ring1, y, f, k1a, k1b);
-            collide(g);
-            static_for<9>([&](auto I) {
-                if (inring(I) && now(I)) ring2[rrow((int)I, y) * E + t] = g[(int)I];
-                h2[(int)I] = g[(int)I];
-            });
-        }
-        static_for<9>([&](auto I) {
-            if constexpr (V::cx[(int)I] == 0) {
-                k1b[(int)I] = k1a[(int)I];
-                k1a[(int)I] = f[(int)I];
-            }
-        });
-        const bool out = n >= 7;                                      // phase 4: step k+3, row r-4 DIR
-        const int y4 = out ? r - 4 * DIR : ya;
-        double o[9] = {};
-        if (act4 && out) {
-            pull(o, ring3, y4, h3, k3a, k3b);
-            collide(o);
-        }
-        static_for<9>([&](auto I) {
-            if constexpr (V::cx[(int)I] == 0) {
-                k3b[(int)I] = k3a[(int)I];
-                k3a[(int)I] = h3[(int)I];
-            }
-        });
-        const unsigned sto = act4 && out ? 0u : BUF_OOB;
-        static_for<9>([&](auto I) { buf_store_nt(o[(int)I], rd, vst | sto, (int)(8 * at(a.g, (int)I, y4, 0))); });
-        lds_barrier();
-        if (p1) {                                                     // the rest of phase-1 row r
-            static_for<9>([&](auto I) {
-                if (inring(I) && !now(I)) ring1[rrow((int)I, r) * E + t] = f[(int)I];
-            });
-        }
-        if (act3 && n >= 5) {                                         // the rest of phase-3 row r-3 DIR
-            static_for<9>([&](auto I) {
-                if (inring(I) && !now(I)) ring3[rrow((int)I, r - 3 * DIR) * E + t] = h3[(int)I];
-            });
-        }
-        if (act3 && n >= 4 && n < nrows1) {                           // phase 3: step k+2, row r-2 DIR
-            const int y = r - 2 * DIR;
-            double g[9];
-            pull(g, ring2, y, h2, k2a, k2b);
-            collide(g);
-            static_for<9>([&](auto I) {
-                if (inring(I) && now(I)) ring3[rrow((int)I, y) * E + t] = g[(int)I];
-                h3[(int)I] = g[(int)I];
-            });
-        }
-        static_for<9>([&](auto I) {
-            if constexpr (V::cx[(int)I] == 0) {
-                k2b[(int)I] = k2a[(int)I];
-                k2a[(int)I] = h2[(int)I];
-            }
-        });
-    }
-}
-
-// Strip edges on multiples of `unit` positions, as k_step3_astrip.
+// Four and five steps per launch (stripP_walk_aligned): strip edges on
+// multiples of `unit` positions, as k_step3_astrip; P-1 rings of
+// XRING_ROWS x 64 NW values in dynamic LDS.
 template <int COL, int NW>
 __global__ __launch_bounds__(64 * NW) void k_step4_astrip(StepArgs a, int remap, int nstrip, int nseg, int unit,
                                                          int prio)
 {
-    constexpr int E = 64 * NW;
-    extern __shared__ double rings[];   // rings 1, 2, 3: [XRING_ROWS][E] each
-    const unsigned nb = gridDim.x;
-    unsigned b = blockIdx.x;
-    if (remap) b = xcd_runs(b, nb);
-    const int s = (int)(b % (unsigned)nstrip);
-    const int units = a.g.nx / unit;
-    const int x0 = (int)((long long)s * units / nstrip) * unit;
-    const int w = (int)((long long)(s + 1) * units / nstrip) * unit - x0;
-    const int seg = (int)(b / (unsigned)nstrip);
-    const int ya = (int)((long long)seg * a.g.ny / nseg);
-    const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
-    double *r1 = rings, *r2 = rings + XRING_ROWS * E, *r3 = rings + 2 * XRING_ROWS * E;
-    if (seg & 1) strip4_walk_aligned<COL, NW, -1>(a, x0, w, ya, rows, r1, r2, r3, prio);
-    else strip4_walk_aligned<COL, NW, 1>(a, x0, w, ya, rows, r1, r2, r3, prio);
+    extern __shared__ double rings[];
+    auto collide = [&](double (&f)[9]) {
+        double vx, vy, vz;
+        collide_sel<9, COL>(f, a.k, vx, vy, vz);
+    };
+    astripP_workgroup<double, NW, 4>(a, remap, nstrip, nseg, unit, prio, rings, collide);
+}
+
+template <int COL, int NW>
+__global__ __launch_bounds__(64 * NW) void k_step5_astrip(StepArgs a, int remap, int nstrip, int nseg, int unit,
+                                                         int prio)
+{
+    extern __shared__ double rings[];
+    auto collide = [&](double (&f)[9]) {
+        double vx, vy, vz;
+        collide_sel<9, COL>(f, a.k, vx, vy, vz);
+    };
+    astripP_workgroup<double, NW, 5>(a, remap, nstrip, nseg, unit, prio, rings, collide);
 }
 
 // Strips of at most 64 NW - 4 positions, as few as the width allows;
@@ -2147,6 +2019,48 @@ long long astrip4_workgroups(const erlbm_lattice *h)
     return (long long)nstrip * nseg;
 }
 
+// Five-step lane-aligned walk: four 8-row rings of 64 NW positions (64 KB
+// for NW = 4: two workgroups per CU).
+template <int NW>
+constexpr size_t astrip5_lds()
+{
+    return sizeof(double) * 4 * XRING_ROWS * (64 * NW);
+}
+
+template <int COL, int NW>
+int astrip5_per_cu()
+{
+    static const int per_cu = [] {
+        constexpr size_t lds = astrip5_lds<NW>();
+        const void *k = reinterpret_cast<const void *>(&k_step5_astrip<COL, NW>);
+        if constexpr (lds >= 65536) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int nblk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, k, 64 * NW, lds) != hipSuccess || nblk < 1) nblk = 1;
+        return nblk;
+    }();
+    return per_cu;
+}
+
+// Workgroups one launch of the five-step walk makes here (the moment-space
+// E-RLBM build; the walk's four rings allow two per CU for any operator).
+long long astrip5_workgroups(const erlbm_lattice *h)
+{
+    int nstrip, nseg;
+    strip3_grid(h, 248, true, astrip5_per_cu<COL_EXT_REG | COL_MOMENT, 4>(), nstrip, nseg, astrip_unit(h));
+    return (long long)nstrip * nseg;
+}
+
+template <int COL, int NW>
+void launch_astrip5_t(erlbm_lattice *h, const StepArgs &a)
+{
+    constexpr size_t lds = astrip5_lds<NW>();
+    const int unit = astrip_unit(h);
+    int nstrip, nseg;
+    strip3_grid(h, 64 * NW - 8, true, astrip5_per_cu<COL, NW>(), nstrip, nseg, unit);
+    hipLaunchKernelGGL((k_step5_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream, a,
+                       h->tune_fuse_remap, nstrip, nseg, unit, walk_prio(h, nstrip * nseg));
+}
+
 template <int COL, int NW>
 void launch_astrip4_t(erlbm_lattice *h, const StepArgs &a)
 {
@@ -2196,6 +2110,7 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
         case FUSE3_ASTRIP_3W: launch_gstrip3_t<COL, 3, true>(h, a); return;
         case FUSE3_ASTRIP_4W: launch_gstrip3_t<COL, 4, true>(h, a); return;
         case FUSE4_ASTRIP_4W: launch_astrip4_t<COL, 4>(h, a); return;
+        case FUSE5_ASTRIP_4W: launch_astrip5_t<COL, 4>(h, a); return;
         case FUSE2_STRIP_256x32: launch_strip_t<COL, 256, 32>(h, a); return;
         default: launch_fused_c<(COL & ~COL_MOMENT)>(h, a); return;
         }
@@ -2231,6 +2146,7 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
     case FUSE3_ASTRIP_5W: launch_gstrip3_t<COL, 5, true>(h, a); return;
     case FUSE3_ASTRIP_6W: launch_gstrip3_t<COL, 6, true>(h, a); return;
     case FUSE4_ASTRIP_4W: launch_astrip4_t<COL, 4>(h, a); return;
+    case FUSE5_ASTRIP_4W: launch_astrip5_t<COL, 4>(h, a); return;
     case 16: launch_wave_t<COL, 1, 4, 1>(h, a, 2); return;
     case 17: launch_wave_t<COL, 1, 4, 1>(h, a, 1); return;
     case 18: launch_wave_t<COL, 2, 4, 1>(h, a, 2); return;
@@ -2323,11 +2239,21 @@ int dispatch_step(erlbm_lattice *h, const StepArgs &a, bool stream)
 
 namespace {
 
-// Three-step shape for a remainder of three steps after four-step launches:
+// Four-step shape for a remainder of four steps after five-step launches:
+// the four-step walk where the size rule would take it.  0: none.
+long long astrip4_workgroups(const erlbm_lattice *h);
+int remainder4_shape(const erlbm_lattice *h)
+{
+    if (fuse_steps(h->fuse_active) != 5) return 0;
+    const int f = FUSE4_ASTRIP_4W;
+    return fuse_shape_fits(h, f) && 2 * astrip4_workgroups(h) >= 3ll * h->n_cu ? f : 0;
+}
+
+// Three-step shape for a remainder of three steps after four- or five-step launches:
 // the lane-aligned three-step walk where it fills the chip.  0: none.
 int remainder3_shape(const erlbm_lattice *h)
 {
-    if (fuse_steps(h->fuse_active) != 4) return 0;
+    if (fuse_steps(h->fuse_active) < 4) return 0;
     const int f = FUSE3_ASTRIP_4W;
     return fuse_shape_fits(h, f) && strip3_workgroups(h, f) >= (19ll * h->n_cu + 19) / 20 ? f : 0;
 }
@@ -2380,6 +2306,7 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
     const int spl = fuse_steps(h->fuse_active);   // steps per fused launch
     const int rem2 = fuse ? remainder2_shape(h) : 0;
     const int rem3 = fuse ? remainder3_shape(h) : 0;
+    const int rem4 = fuse ? remainder4_shape(h) : 0;
     for (int s = 0; s < n_steps;) {
         const long long k = h->launched;
         const int cur = (h->base + (int)(k & 1)) & 1;
@@ -2400,6 +2327,17 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
             h->launched += spl;
             if (spl % 2 == 0) h->base ^= 1;
             s += spl;
+            continue;
+        }
+        if (rem4 && k > 0 && n_steps - s == 4) {
+            const int keep = h->fuse_active;
+            h->fuse_active = rem4;
+            int rc = launch_fused(h, a);
+            h->fuse_active = keep;
+            if (rc) return rc;
+            h->launched += 4;
+            h->base ^= 1;
+            s += 4;
             continue;
         }
         if (rem3 && k > 0 && n_steps - s == 3) {
@@ -2557,6 +2495,7 @@ std::string fuse_shape_desc(int f, int remap)
         break;
     case FUSE_ZWALK: d = "two steps, " + tx + "x" + ty + " tiles walked along z"; break;
     case FUSE_GSTRIP4: d = "four steps, lane-aligned strips of any width (" + std::to_string((fs.tx + 6) / 64) + " wavefronts)"; break;
+    case FUSE_GSTRIP5: d = "five steps, lane-aligned strips of any width (" + std::to_string((fs.tx + 8) / 64) + " wavefronts)"; break;
     }
     return remap ? d + ", XCD runs" : d;
 }

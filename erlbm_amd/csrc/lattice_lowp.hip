@@ -97,7 +97,7 @@ __device__ __forceinline__ double collide_lp(float (&d)[Q], const LPArgs &a)
         double f[Q];
         to_abs<Q>(d, f, a.shift);
         double vx, vy, vz;
-        collide_exact<Q, COL>(f, a.k64, vx, vy, vz);
+        collide_sel<Q, COL>(f, a.k64, vx, vy, vz);
         from_abs<Q>(f, d, a.shift);
         return density_exact<Q>(f);
     }
@@ -538,6 +538,43 @@ __global__ __launch_bounds__(64 * NW) void k_lp_step3_astrip(LPArgs a, int remap
     else lp_strip3_walk_aligned<COL, NW, 1, NTP>(a, x0, w, ya, rows, lp_rings, lp_rings + XRING_ROWS * E, prio);
 }
 
+// Four steps per launch on FP32 shifted storage: the FP64 kernel's walk
+// (stripP_walk_aligned) with FP32 rings (24 KB for NW = 4), the kept values
+// the FP32 ones one-step launches would store, so bitwise the one-step FP32
+// path.
+template <int COL, int NW>
+__global__ __launch_bounds__(64 * NW) void k_lp_step4_astrip(LPArgs a, int remap, int nstrip, int nseg, int unit,
+                                                            int prio)
+{
+    extern __shared__ float lp_rings[];
+    auto collide = [&](float (&d)[9]) { (void)collide_lp<9, COL, false>(d, a); };
+    astripP_workgroup<float, NW, 4>(a, remap, nstrip, nseg, unit, prio, lp_rings, collide);
+}
+
+// Strips of at most 250 positions (64 NW - 6), edges on 16-position units
+// where the width allows; segments sized to the resident workgroups.
+template <int COL>
+void launch_lp_astrip4(erlbm_lattice *h, const LPArgs &l)
+{
+    constexpr int NW = 4;
+    constexpr size_t lds = sizeof(float) * 3 * XRING_ROWS * (64 * NW);
+    static const int per_cu = [] {
+        int nblk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, reinterpret_cast<const void *>(&k_lp_step4_astrip<COL, NW>),
+                                                         64 * NW, lds) != hipSuccess ||
+            nblk < 1)
+            nblk = 1;
+        return nblk;
+    }();
+    const int pc = h->tune_fuse_slots > 0 ? std::min(per_cu, h->tune_fuse_slots) : per_cu;
+    const int unit = h->nx % 16 == 0 ? 16 : 1;
+    const int nstrip = (h->nx / unit + (64 * NW - 6) / unit - 1) / ((64 * NW - 6) / unit);
+    const int nseg = (int)std::max(1ll, std::min<long long>(h->ny / 16, (long long)h->n_cu * pc / nstrip));
+    const int prio = h->tune_fuse_prio >= 0 ? h->tune_fuse_prio : (nstrip * nseg >= 2 * h->n_cu ? 2 : 0);
+    hipLaunchKernelGGL((k_lp_step4_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream, l,
+                       h->tune_fuse_remap, nstrip, nseg, unit, prio);
+}
+
 // Strips of at most 252 positions, edges on 16-position units where the
 // width allows; segments as many as make one workgroup per resident slot,
 // at least 16 rows each.
@@ -588,10 +625,21 @@ void launch_lp_strip(erlbm_lattice *h, const LPArgs &l)
 template <int COL, typename T>
 void launch_lp_step2_ct(erlbm_lattice *h, const LPArgs &l)
 {
-    if constexpr (sizeof(T) == 4) {
+    if constexpr ((COL & COL_MOMENT) != 0) {
+        // the moment-space collision on FP32 storage: the lane-aligned
+        // three-step walk and the two-step remainder walk; any other shape
+        // keeps the reference's order
+        if constexpr (sizeof(T) == 4) {
+            if (h->fuse_active == FUSE4_ASTRIP_4W) return launch_lp_astrip4<COL>(h, l);
+            if (h->fuse_active == FUSE3_ASTRIP_4W) return launch_lp_astrip<COL>(h, l);
+            if (h->fuse_active == FUSE2_STRIP_256x32) return launch_lp_strip<COL, T, 256, 32>(h, l);
+        }
+        return launch_lp_step2_ct<(COL & ~COL_MOMENT), T>(h, l);
+    } else if constexpr (sizeof(T) == 4) {
         if (h->fuse_active == FUSE3_STRIP_256x32) return launch_lp_strip3<COL, 256, 32>(h, l);
         if (h->fuse_active == FUSE3_STRIP_128x32) return launch_lp_strip3<COL, 128, 32>(h, l);
         if (h->fuse_active == FUSE3_ASTRIP_4W) return launch_lp_astrip<COL>(h, l);
+        if (h->fuse_active == FUSE4_ASTRIP_4W) return launch_lp_astrip4<COL>(h, l);
     }
     if (h->fuse_active == FUSE2_STRIP_128x32) launch_lp_strip<COL, T, 128, 32>(h, l);
     else launch_lp_strip<COL, T, 256, 32>(h, l);
@@ -600,6 +648,17 @@ void launch_lp_step2_ct(erlbm_lattice *h, const LPArgs &l)
 template <typename T>
 int launch_lp_step2_t(erlbm_lattice *h, const LPArgs &l)
 {
+    if (h->arith == ERLBM_ARITH_MOMENT) {
+        switch (h->col) {   // coll_MRT keeps the reference's order
+        case ERLBM_COL_BGK: launch_lp_step2_ct<COL_BGK | COL_MOMENT, T>(h, l); break;
+        case ERLBM_COL_MRT: launch_lp_step2_ct<COL_MRT, T>(h, l); break;
+        case ERLBM_COL_REG: launch_lp_step2_ct<COL_REG | COL_MOMENT, T>(h, l); break;
+        case ERLBM_COL_EXT_REG: launch_lp_step2_ct<COL_EXT_REG | COL_MOMENT, T>(h, l); break;
+        default: return ERLBM_ERR_UNSUPPORTED;
+        }
+        ERLBM_HIP_TRY(hipGetLastError());
+        return ERLBM_OK;
+    }
     switch (h->col) {
     case ERLBM_COL_BGK: launch_lp_step2_ct<COL_BGK, T>(h, l); break;
     case ERLBM_COL_MRT: launch_lp_step2_ct<COL_MRT, T>(h, l); break;
@@ -625,6 +684,7 @@ LPArgs make_args(erlbm_lattice *h, const StepArgs &a)
     l.k64.omega1 = 1.0 - 1.0 / h->tau;
     l.k64.omega2 = 1.0 - 1.0 / h->tau_nh;
     l.k64.oot = 1.0 / h->tau;
+    l.k64.m = moment_consts(h->q, h->col, h->tau, h->tau_nh);
     l.bc = a.bc;
     l.shift = h->shift;
     l.dens_limit = a.dens_limit;

@@ -163,6 +163,208 @@ __device__ __forceinline__ void dropped_stores(__amdgpu_buffer_rsrc_t r)
     static_for<N>([&](auto I) { buf_store_nt(T(0), r, BUF_OOB | ((unsigned)sizeof(T) * (int)I)); });
 }
 
+// One element of a population array of element type T (raw buffer load).
+template <typename T>
+__device__ __forceinline__ T buf_load_elem(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff);
+template <>
+__device__ __forceinline__ double buf_load_elem<double>(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff)
+{
+    return buf_load<0>(r, voff, soff);
+}
+template <>
+__device__ __forceinline__ float buf_load_elem<float>(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff)
+{
+    return buf_load_f32<0>(r, voff, soff);
+}
+
+// Four or five steps per launch (P phases): the lane-aligned walk with P-1
+// rings.  Lane t holds position x0 - (P-1) + t in all phases (phase p:
+// lanes p-1 .. w+2P-p-2; the last phase stores lanes P-1 .. w+P-2).  With two
+// barriers per walking step the phases alternate between the two intervals
+// they make: walking step n (row r = r0 + DIR n) runs phase 1 on row r
+// before the first barrier, phases 2 (row r - DIR) and 4 (row r - 4 DIR)
+// between the barriers, phases 3 (row r - 2 DIR) and 5 (row r - 5 DIR)
+// after the second; phase 4 so reads the ring-3 rows phase 3 wrote in the
+// walking steps before, phase 5 the ring-4 rows phase 4 wrote in the same
+// one.  Ring p keeps the c_x != 0 results of phase p (lean layout, 8 rows:
+// 16 KB per ring of FP64 for NW = 4), the c_x = 0 ones stay in registers;
+// a ring's late writes (the populations not read from the newest row)
+// follow its first reader by one interval as in strip3_walk_aligned: rings
+// 1 and 3 after the second barrier (of the same / the next walking step),
+// rings 2 and 4 after the first barrier of the next walking step.  Output
+// rows ya .. ya+rows-1 need step-k rows ya-(P-1) .. ya+rows+P-2: rows + 7
+// (P = 4) or rows + 9 (P = 5) walking steps.
+// T: the population arrays' element type (FP64 storage, or FP32 shifted
+// storage whose collision converts to FP64 and back); Args: the launch's
+// arguments (src, dst, g, step, stop_ro); collide(T (&)[9]) collides one
+// node in place; ring: P-1 rings of XRING_ROWS x 64 NW elements.
+template <typename T, int NW, int DIR, int P, class Args, class Collide>
+__device__ __forceinline__ void stripP_walk_aligned(const Args &a, int x0, int w, int ya, int rows, T *ring, int prio,
+                                                    Collide collide)
+{
+    static_assert(P == 4 || P == 5, "four or five phases");
+    constexpr unsigned SZ = sizeof(T);
+    using V = VSet<9>;
+    constexpr int E = 64 * NW;
+    T *const ring1 = ring, *const ring2 = ring + XRING_ROWS * E, *const ring3 = ring + 2 * XRING_ROWS * E;
+    T *const ring4 = ring + 3 * XRING_ROWS * E;   // P = 5 only
+    const int t = (int)threadIdx.x;
+    const int nx = a.g.nx, ny = a.g.ny;
+    // phase p covers positions x0-(P-p) .. x0+w-1+(P-p): lanes p-1 .. w+2P-p-2
+    auto act = [&](int p) { return t >= p - 1 && t < w + 2 * P - p - 1; };
+    const bool act1 = act(1), act2 = act(2), act3 = act(3), act4 = act(4), act5 = act(5);
+    const bool actl = P == 4 ? act4 : act5;   // lanes of the last phase: the stores
+    const int x1 = act1 ? wrap(x0 - (P - 1) + t, nx) : 0;
+    const __amdgpu_buffer_rsrc_t rs = pop_buffer<SZ>(a.src, a.g), rd = pop_buffer<SZ>(a.dst, a.g);
+    const unsigned oob1 = act1 ? 0u : BUF_OOB;
+    const unsigned vo[3] = {SZ * wrap(x1 + 1, nx) | oob1, SZ * x1 | oob1, SZ * wrap(x1 - 1, nx) | oob1};
+    const unsigned vst = actl ? SZ * (x0 - (P - 1) + t) : 0u;
+    auto load_row = [&](int y, T (&f)[9], bool skip) {   // pulls of step-k row y (-P .. ny+P-1)
+        const unsigned off = skip ? BUF_OOB : 0u;
+        const int yw = wrap(y, ny);
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            const int ys = V::cy[i] == 0 ? yw : wrap(yw - V::cy[i], ny);
+            f[i] = buf_load_elem<T>(rs, vo[V::cx[i] + 1] | off, (int)(SZ * at(a.g, i, ys, 0)));
+        });
+    };
+    auto rrow = [&](int i, int y) { return xring_base<DIR>(i) + (y - ya + P) % xring_depth<DIR>(i); };
+    auto inring = [](int i) { return V::cx[i] != 0; };
+    auto now = [](int i) { return V::cy[i] * DIR < 0; };
+    auto age = [](int i) { return 1 + V::cy[i] * DIR; };
+    // pull of a phase from the ring of the phase before (c_x != 0) and from
+    // that phase's results in registers (c_x = 0: this lane's own, of the
+    // newest, middle and oldest source row)
+    auto pull = [&](T (&g)[9], const T *rg, int y, const T (&r0)[9], const T (&r1)[9], const T (&r2)[9]) {
+        static_for<9>([&](auto I) {
+            constexpr int i = I;
+            if constexpr (V::cx[i] != 0) g[i] = rg[rrow(i, y - V::cy[i]) * E + (t - V::cx[i])];
+            else g[i] = age(i) == 0 ? r0[i] : (age(i) == 1 ? r1[i] : r2[i]);
+        });
+    };
+    auto write_now = [&](T *rg, int y, const T (&g)[9]) {
+        static_for<9>([&](auto I) {
+            if (inring(I) && now(I)) rg[rrow((int)I, y) * E + t] = g[(int)I];
+        });
+    };
+    auto write_late = [&](T *rg, int y, const T (&g)[9]) {
+        static_for<9>([&](auto I) {
+            if (inring(I) && !now(I)) rg[rrow((int)I, y) * E + t] = g[(int)I];
+        });
+    };
+    auto shift = [](T (&ka)[9], T (&kb)[9], const T (&g)[9]) {   // c_x = 0 results one walking step older
+        static_for<9>([&](auto I) {
+            if constexpr (V::cx[(int)I] == 0) {
+                kb[(int)I] = ka[(int)I];
+                ka[(int)I] = g[(int)I];
+            }
+        });
+    };
+    auto store = [&](const T (&o)[9], int y, bool on) {
+        const unsigned sto = on ? 0u : BUF_OOB;
+        static_for<9>([&](auto I) { buf_store_nt(o[(int)I], rd, vst | sto, (int)(SZ * at(a.g, (int)I, y, 0))); });
+    };
+    const int r0 = DIR > 0 ? ya - (P - 1) : ya + rows + (P - 2);
+    const int nrows1 = rows + 2 * (P - 1);           // rows of phase 1
+    const int nsteps = P == 4 ? rows + 7 : rows + 9;  // walking steps
+    T fq[9];
+    T h2[9], h3[9], h4[9];         // the previous walking step's phase-2 / 3 / 4 rows (late ring writes)
+    T k1a[9], k1b[9];              // c_x = 0: phase-1 results of the rows one and two walking steps back
+    T k2a[9], k2b[9];              // phase 2, likewise
+    T k3a[9], k3b[9];              // phase 3, one and two walking steps before h3
+    T k4a[9], k4b[9];              // phase 4, likewise (P = 5)
+    load_row(r0, fq, false);
+    if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
+    nine_dropped_stores<T>(rd);
+    for (int n = 0; n < nsteps; n++) {
+        if (prio) progress_prio(n, nsteps, prio);
+        const int r = r0 + DIR * n;
+        T f[9];
+        static_for<9>([&](auto I) { f[(int)I] = fq[(int)I]; });
+        load_row(r + DIR, fq, n + 1 >= nrows1);                      // in flight meanwhile
+        const bool p1 = act1 && n < nrows1;
+        if (p1) {                                                     // phase 1: step k, row r
+            collide(f);
+            write_now(ring1, r, f);
+        }
+        lds_barrier();
+        if (act2 && n >= 3) write_late(ring2, r - 2 * DIR, h2);       // the rest of phase-2 row r-2 DIR
+        if constexpr (P == 5) {
+            if (act4 && n >= 8) write_late(ring4, r - 5 * DIR, h4);   // the rest of phase-4 row r-5 DIR
+        }
+        if (act2 && n >= 2 && n < nrows1) {                           // phase 2: step k+1, row r-DIR
+            T g[9];
+            pull(g, ring1, r - DIR, f, k1a, k1b);
+            collide(g);
+            write_now(ring2, r - DIR, g);
+            static_for<9>([&](auto I) { h2[(int)I] = g[(int)I]; });
+        }
+        shift(k1a, k1b, f);
+        const int y4 = n >= 7 ? r - 4 * DIR : ya;                     // phase 4: step k+3, row r-4 DIR
+        if constexpr (P == 4) {
+            T o[9] = {};
+            if (act4 && n >= 7) {
+                pull(o, ring3, y4, h3, k3a, k3b);
+                collide(o);
+            }
+            shift(k3a, k3b, h3);
+            store(o, y4, act4 && n >= 7);
+        } else {
+            if (act4 && n >= 7) {
+                T g[9];
+                pull(g, ring3, y4, h3, k3a, k3b);
+                collide(g);
+                write_now(ring4, y4, g);
+                static_for<9>([&](auto I) { h4[(int)I] = g[(int)I]; });
+            }
+            shift(k3a, k3b, h3);
+        }
+        lds_barrier();
+        if (p1) write_late(ring1, r, f);                              // the rest of phase-1 row r
+        if (act3 && n >= 5) write_late(ring3, r - 3 * DIR, h3);       // the rest of phase-3 row r-3 DIR
+        if (act3 && n >= 4 && n < nrows1) {                           // phase 3: step k+2, row r-2 DIR
+            T g[9];
+            pull(g, ring2, r - 2 * DIR, h2, k2a, k2b);
+            collide(g);
+            write_now(ring3, r - 2 * DIR, g);
+            static_for<9>([&](auto I) { h3[(int)I] = g[(int)I]; });
+        }
+        shift(k2a, k2b, h2);
+        if constexpr (P == 5) {                                       // phase 5: step k+4, row r-5 DIR
+            const int y5 = n >= 9 ? r - 5 * DIR : ya;
+            T o[9] = {};
+            if (act5 && n >= 9) {
+                pull(o, ring4, y5, h4, k4a, k4b);
+                collide(o);
+            }
+            shift(k4a, k4b, h4);
+            store(o, y5, act5 && n >= 9);
+        }
+    }
+}
+
+// One workgroup of the P-step lane-aligned walk: strip s = b mod nstrip
+// (edges on multiples of `unit` positions), segment b / nstrip (rows split
+// evenly, odd segments walking upwards), b the workgroup number after the
+// XCD-contiguous renumbering.
+template <typename T, int NW, int P, class Args, class Collide>
+__device__ __forceinline__ void astripP_workgroup(const Args &a, int remap, int nstrip, int nseg, int unit, int prio,
+                                                  T *rings, Collide collide)
+{
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (remap) b = xcd_runs(b, nb);
+    const int s = (int)(b % (unsigned)nstrip);
+    const int units = a.g.nx / unit;
+    const int x0 = (int)((long long)s * units / nstrip) * unit;
+    const int w = (int)((long long)(s + 1) * units / nstrip) * unit - x0;
+    const int seg = (int)(b / (unsigned)nstrip);
+    const int ya = (int)((long long)seg * a.g.ny / nseg);
+    const int rows = (int)((long long)(seg + 1) * a.g.ny / nseg) - ya;
+    if (seg & 1) stripP_walk_aligned<T, NW, -1, P>(a, x0, w, ya, rows, rings, prio, collide);
+    else stripP_walk_aligned<T, NW, 1, P>(a, x0, w, ya, rows, rings, prio, collide);
+}
+
 // Threads of a strip-walk workgroup for TX-wide strips (the halo positions
 // of the first steps spill into a fifth wavefront at TX = 256).
 template <int TX>

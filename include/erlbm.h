@@ -109,10 +109,11 @@ int erlbm_set_body_force(erlbm_lattice *h, double ax, double ay, double az);
  * space, f*_i = W f_i + w_i (alpha + beta.c_i + c_i.Gamma.c_i), with less
  * than half the FP64 instructions; within 1e-10 maximum relative population
  * error of the reference after 1000 steps.  Applies to the multi-step
- * kernels of FP64 periodic lattices; one-step launches, coll_MRT and
- * lattices with boundary columns, solids, stop tests or a body force keep
- * the reference's order.  FP64 storage only (ERLBM_ERR_UNSUPPORTED
- * otherwise).                                                              */
+ * kernels of periodic lattices in FP64 storage and in FP32 shifted storage
+ * (FP64 arithmetic); one-step launches, coll_MRT and lattices with
+ * boundary columns, solids, stop tests or a body force keep the
+ * reference's order.  FP16 storage and FP32 arithmetic:
+ * ERLBM_ERR_UNSUPPORTED.                                                   */
 enum { ERLBM_ARITH_EXACT = 0, ERLBM_ARITH_MOMENT = 1 };
 int erlbm_set_arith(erlbm_lattice *h, int arith);
 /* Per-step post-collision density test of main_stability-map_obstacle.cpp:

@@ -1,8 +1,9 @@
 """The double-shear-layer workload at its full size: D2Q9 E-RLBM, periodic
 L = 2048 (4,194,304 nodes), tau = 0.5 + 3 U0 L / Re (Re = 80000, U0 = 1/32),
 tau_nh = 0.65, equilibrium initialisation of src/main_double-shear-layer.cpp:
-48-66, 1000 steps, as the default kernels run it (four steps per launch in
-FP64, three in FP32-shifted storage), and on the three-step walk.
+48-66, 1000 steps, as the default kernels run it (four steps per launch;
+five with the moment-space arithmetic), and on the three- and four-step
+walks.
 
   * FP64: bitwise the strict oracle (the reference's operation order, no FMA
     contraction); max relative population error <= 1e-10 against the
@@ -91,7 +92,7 @@ def test_dsl2048_1000_steps_within_1e10_of_fma_oracle(gpu, dsl_setup, fma_1000):
     assert err <= 1e-10
 
 
-@pytest.mark.parametrize("fuse,want_spl", [(None, 4), ("42", 3)])
+@pytest.mark.parametrize("fuse,want_spl", [(None, 5), ("46", 4), ("42", 3)])
 def test_dsl2048_moment_arith_1000_steps_within_1e10(gpu, monkeypatch, dsl_setup, strict_1000, fma_1000, fuse,
                                                      want_spl):
     f0, _ = dsl_setup
@@ -108,21 +109,44 @@ def test_dsl2048_moment_arith_1000_steps_within_1e10(gpu, monkeypatch, dsl_setup
 
 
 def test_dsl2048_moment_arith_call_pattern(gpu, dsl_setup, strict_1000):
-    """The same 1000 steps in calls of 500, 1, 250 and 249 steps: one-,
-    two- and three-step remainders (the two-step strip walk, the three-step
-    lane-aligned walk) between four-step launches."""
+    """The same 1000 steps in calls of 500, 1, 253 and 246 steps: one-,
+    three- and four-step remainders (the one-step kernel in the reference's
+    order, the three- and four-step lane-aligned walks) between five-step
+    launches."""
     f0, _ = dsl_setup
-    got, _ = _gpu_run(gpu, f0, gpu.FP64, gpu.ARITH_MOMENT, calls=(500, 1, 250, 249))
+    got, _ = _gpu_run(gpu, f0, gpu.FP64, gpu.ARITH_MOMENT, calls=(500, 1, 253, 246))
     err = float(np.max(np.abs(got - strict_1000) / np.abs(strict_1000)))
-    print(f"calls 500+1+250+249: max relative population error {err:.3e} vs strict", flush=True)
+    print(f"calls 500+1+253+246: max relative population error {err:.3e} vs strict", flush=True)
     assert 0.0 < err <= 1e-10
+
+
+def test_dsl2048_fp32_shifted_moment_arith_1000_steps(gpu, dsl_setup, strict_1000):
+    """FP32-shifted storage with the moment-space collision (FP64
+    arithmetic): the storage format's error budget against the FP64 oracle
+    holds, and the run differs from the exact-arithmetic FP32 run only at
+    FP32 rounding level."""
+    f0, _ = dsl_setup
+    got, (shape, spl, desc) = _gpu_run(gpu, f0, gpu.FP32_SHIFTED, gpu.ARITH_MOMENT)
+    print(f"GPU FP32 kernel, moment-space arithmetic: {desc} (shape {shape}, {spl} steps per launch)", flush=True)
+    assert spl == 4
+    exact, _ = _gpu_run(gpu, f0, gpu.FP32_SHIFTED)
+    u, uw, ue = _vel(got), _vel(strict_1000), _vel(exact)
+    err = float(np.sqrt(np.sum((u - uw) ** 2) / np.sum(uw ** 2)))
+    diff = float(np.sqrt(np.sum((u - ue) ** 2) / np.sum(ue ** 2)))
+    mass = abs(got.sum() - strict_1000.sum()) / strict_1000.sum()
+    print(f"FP32 moment vs FP64 oracle: L2 velocity error {err:.3e}; vs FP32 exact arithmetic {diff:.3e}; "
+          f"mass drift {mass:.2e}", flush=True)
+    assert not np.array_equal(got, exact)
+    assert err < FP32_L2_BOUND
+    assert diff < FP32_L2_BOUND
+    assert mass < 1e-7
 
 
 def test_dsl2048_fp32_shifted_1000_steps(gpu, monkeypatch, dsl_setup, strict_1000):
     f0, _ = dsl_setup
     got, (shape, spl, desc) = _gpu_run(gpu, f0, gpu.FP32_SHIFTED)
     print(f"GPU FP32 default kernel: {desc} (shape {shape}, {spl} steps per launch)", flush=True)
-    assert spl == 3
+    assert spl == 4
     monkeypatch.setenv("ERLBM_FUSE", "0")
     one, (shape1, spl1, _) = _gpu_run(gpu, f0, gpu.FP32_SHIFTED)
     assert (shape1, spl1) == (0, 1)

@@ -543,13 +543,14 @@ def test_aligned_walk_issue_priority_bitwise(gpu, orc, monkeypatch, prio):
 def test_default_shape_rule(gpu):
     """The default multi-step shape per lattice size (periodic FP64 D2Q9
     E-RLBM; gfx950 parts have 256 CUs): the four-step lane-aligned walk
-    where it makes 1.5 workgroups per CU or more, else the three-step
+    (five-step with the moment-space arithmetic) where it makes 1.5
+    workgroups per CU or more, else the three-step
     lane-aligned any-width walk where it loads the CUs no more unevenly
     than the 256-wide walk or makes three workgroups per CU, the 256-wide
     walk at L = 1024, the 64x8 two-step tiles at L = 768, and
     steps_per_launch / descriptions that match."""
     want = {768: (1, 2), 1000: (42, 3), 1024: (26, 3), 1280: (46, 4), 2048: (46, 4), 3000: (46, 4), 4096: (46, 4)}
-    words = {2: "two steps", 3: "three steps", 4: "four steps"}
+    words = {2: "two steps", 3: "three steps", 4: "four steps", 5: "five steps"}
     for L, (shape, spl) in want.items():
         with gpu.Lattice(9, L, L, 1) as lat:
             lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
@@ -557,8 +558,18 @@ def test_default_shape_rule(gpu):
             print(f"L={L}: {got}", flush=True)
             assert got[:2] == (shape, spl), L
             assert got[2].startswith(words[spl])
-    # shifted storage: FP32 takes the lane-aligned walk, FP16 one step per launch
-    for prec, want_shape in ((gpu.FP32_SHIFTED, (42, 3)), (gpu.FP16_SHIFTED, (0, 1))):
+    # the moment-space arithmetic: five steps per launch where they fill the GPU
+    for L, want_shape in ((1024, (26, 3)), (1280, (47, 5)), (2048, (47, 5)), (4096, (47, 5))):
+        with gpu.Lattice(9, L, L, 1) as lat:
+            lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
+            lat.set_arith(gpu.ARITH_MOMENT)
+            got = lat.step_shape()
+            assert got[:2] == want_shape, (L, got)
+            lat.set_params(0.51, 0.65, 1.0, O.COL_MRT)   # MRT keeps the reference's order: four steps
+            assert lat.step_shape()[:2] == ((26, 3) if L == 1024 else (46, 4)), L
+    # shifted storage: FP32 takes the four-step lane-aligned walk, FP16 one
+    # step per launch
+    for prec, want_shape in ((gpu.FP32_SHIFTED, (46, 4)), (gpu.FP16_SHIFTED, (0, 1))):
         with gpu.Lattice(9, 2048, 2048, 1, precision=prec) as lat:
             lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
             assert lat.step_shape()[:2] == want_shape, prec

@@ -56,11 +56,11 @@ def tg_oracle(orc):
 
 
 @pytest.mark.parametrize("col", sorted(CASES))
-@pytest.mark.parametrize("shape", [42, 40, 26, 39, 11, 46])
+@pytest.mark.parametrize("shape", [42, 40, 26, 39, 11, 46, 47])
 def test_moment_d2q9_walks_within_tolerance(gpu, monkeypatch, tg_oracle, col, shape):
     """The three-step walks (lane-aligned with 4 and 2 wavefronts, 256-wide,
-    any-width), the two-step strip walk and the four-step lane-aligned walk,
-    every operator."""
+    any-width), the two-step strip walk and the four- and five-step
+    lane-aligned walks, every operator."""
     f0, want = tg_oracle
     tau, tnh = CASES[col]
     print(f"moment TG L={TG_L} col={col} shape={shape}", flush=True)
@@ -147,6 +147,9 @@ def test_set_arith_rejects_bad_values(gpu):
     with gpu.Lattice(9, 64, 64, 1) as lat:
         with pytest.raises(gpu.ErlbmError):
             lat.set_arith(7)
+    for prec in (gpu.FP16_SHIFTED, gpu.FP32_SHIFTED_F32):
+        with gpu.Lattice(9, 64, 64, 1, precision=prec) as lat:
+            with pytest.raises(gpu.ErlbmError):
+                lat.set_arith(gpu.ARITH_MOMENT)
     with gpu.Lattice(9, 64, 64, 1, precision=gpu.FP32_SHIFTED) as lat:
-        with pytest.raises(gpu.ErlbmError):
-            lat.set_arith(gpu.ARITH_MOMENT)
+        lat.set_arith(gpu.ARITH_MOMENT)
