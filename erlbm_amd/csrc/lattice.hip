@@ -1553,7 +1553,7 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
     if (h->elem != 8) {
         const bool two = f == FUSE2_STRIP_256x32 || f == FUSE2_STRIP_128x32;
         const bool three = f == FUSE3_STRIP_256x32 || f == FUSE3_STRIP_128x32 || f == FUSE3_ASTRIP_4W ||
-                           f == FUSE4_ASTRIP_4W;
+                           f == FUSE4_ASTRIP_4W || f == FUSE5_ASTRIP_4W;
         if (h->f32_arith || !(two || (three && h->elem == 4))) return false;
         if (h->elem == 2 && !h->fuse_requested) return false;
         if (h->extent * h->elem >= (1ll << 31)) return false;   // 31-bit buffer offsets
@@ -1597,14 +1597,12 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
     // 84.4, 110.2 / 89.7; 2048 108.3 / 95.8, 129.6 / 102.8; 3000 114.5 /
     // 93.1, 123.0 / 96.4; 4096 120.6 / 105.1, 141.0 / 111.4; L = 1024, 320
     // workgroups: 52.9 / 60.1, 71.0 / 74.4)
-    // FP64 with the moment-space collision: the five-step walk where it makes
-    // at least 1.5 workgroups per CU (MLUPS, five-step / four-step: L = 1280
-    // 111.9 / 100.8; 1536 126.5 / 111.0; 2048 143.4 / 128.6; 3000 138.9 /
-    // 123.4; 4096 160.7 / 140.7; with the reference's operation order the
-    // five-step walk is no faster, 79.9 / 80.0 at L = 1280, 103.2 / 106.3
-    // at 2048: FP64 issue binds it)
-    if (h->elem == 8 && first == FUSE3_STRIP_SIZED && h->arith == ERLBM_ARITH_MOMENT && h->col != ERLBM_COL_MRT &&
-        fuse_shape_fits(h, FUSE5_ASTRIP_4W) && 2 * astrip5_workgroups(h) >= 3ll * h->n_cu)
+    // FP64: the five-step walk where it makes at least 1.5 workgroups per CU
+    // (MLUPS, five-step / four-step, exact then moment-space arithmetic:
+    // L = 1280 80.2 / 79.9, 110.8 / 100.2; 2048 106.8 / 104.8, 147.5 / 129.2;
+    // 4096 122.2 / 117.0, 165.7 / 142.3)
+    if (h->elem == 8 && first == FUSE3_STRIP_SIZED && fuse_shape_fits(h, FUSE5_ASTRIP_4W) &&
+        2 * astrip5_workgroups(h) >= 3ll * h->n_cu)
         return FUSE5_ASTRIP_4W;
     // FP32 storage likewise (L = 2048, MLUPS, four-step / three-step, exact
     // and moment-space arithmetic: 111.6 / 107.7, 162.6 / 155.2; L = 4096
@@ -1857,7 +1855,7 @@ __global__ __launch_bounds__(64 * NW) void k_step3_astrip(StepArgs a, int remap,
 
 // Four and five steps per launch (stripP_walk_aligned): strip edges on
 // multiples of `unit` positions, as k_step3_astrip; P-1 rings of
-// XRING_ROWS x 64 NW values in dynamic LDS.
+// XSLOTS x 64 NW values in dynamic LDS.
 template <int COL, int NW>
 __global__ __launch_bounds__(64 * NW) void k_step4_astrip(StepArgs a, int remap, int nstrip, int nseg, int unit,
                                                          int prio)
@@ -1983,11 +1981,12 @@ void launch_gstrip3_t(erlbm_lattice *h, const StepArgs &a)
                            a, h->tune_fuse_remap, nstrip, nseg);
 }
 
-// Four-step lane-aligned walk: three 8-row rings of 64 NW positions.
+// Four-step lane-aligned walk: three rings of one row per c_x != 0
+// population, 64 NW positions each (36 KB for NW = 4).
 template <int NW>
 constexpr size_t astrip4_lds()
 {
-    return sizeof(double) * 3 * XRING_ROWS * (64 * NW);
+    return sizeof(double) * 3 * XSLOTS * (64 * NW);
 }
 
 template <int COL, int NW>
@@ -2019,12 +2018,11 @@ long long astrip4_workgroups(const erlbm_lattice *h)
     return (long long)nstrip * nseg;
 }
 
-// Five-step lane-aligned walk: four 8-row rings of 64 NW positions (64 KB
-// for NW = 4: two workgroups per CU).
+// Five-step lane-aligned walk: four such rings (48 KB for NW = 4).
 template <int NW>
 constexpr size_t astrip5_lds()
 {
-    return sizeof(double) * 4 * XRING_ROWS * (64 * NW);
+    return sizeof(double) * 4 * XSLOTS * (64 * NW);
 }
 
 template <int COL, int NW>
@@ -2041,8 +2039,8 @@ int astrip5_per_cu()
     return per_cu;
 }
 
-// Workgroups one launch of the five-step walk makes here (the moment-space
-// E-RLBM build; the walk's four rings allow two per CU for any operator).
+// Workgroups one launch of the five-step walk makes here (counted for the
+// moment-space E-RLBM build, the one the size rule gives this walk).
 long long astrip5_workgroups(const erlbm_lattice *h)
 {
     int nstrip, nseg;

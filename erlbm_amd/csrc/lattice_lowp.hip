@@ -539,7 +539,7 @@ __global__ __launch_bounds__(64 * NW) void k_lp_step3_astrip(LPArgs a, int remap
 }
 
 // Four steps per launch on FP32 shifted storage: the FP64 kernel's walk
-// (stripP_walk_aligned) with FP32 rings (24 KB for NW = 4), the kept values
+// (stripP_walk_aligned) with FP32 rings (18 KB for NW = 4), the kept values
 // the FP32 ones one-step launches would store, so bitwise the one-step FP32
 // path.
 template <int COL, int NW>
@@ -551,28 +551,42 @@ __global__ __launch_bounds__(64 * NW) void k_lp_step4_astrip(LPArgs a, int remap
     astripP_workgroup<float, NW, 4>(a, remap, nstrip, nseg, unit, prio, lp_rings, collide);
 }
 
-// Strips of at most 250 positions (64 NW - 6), edges on 16-position units
-// where the width allows; segments sized to the resident workgroups.
-template <int COL>
-void launch_lp_astrip4(erlbm_lattice *h, const LPArgs &l)
+template <int COL, int NW>
+__global__ __launch_bounds__(64 * NW) void k_lp_step5_astrip(LPArgs a, int remap, int nstrip, int nseg, int unit,
+                                                            int prio)
+{
+    extern __shared__ float lp_rings[];
+    auto collide = [&](float (&d)[9]) { (void)collide_lp<9, COL, false>(d, a); };
+    astripP_workgroup<float, NW, 5>(a, remap, nstrip, nseg, unit, prio, lp_rings, collide);
+}
+
+// P steps per launch (4 or 5): strips of at most 64 NW - 2 (P-1) positions,
+// edges on 16-position units where the width allows; segments sized to the
+// resident workgroups.
+template <int COL, int P>
+void launch_lp_astripP(erlbm_lattice *h, const LPArgs &l)
 {
     constexpr int NW = 4;
-    constexpr size_t lds = sizeof(float) * 3 * XRING_ROWS * (64 * NW);
-    static const int per_cu = [] {
+    constexpr size_t lds = sizeof(float) * (P - 1) * XSLOTS * (64 * NW);
+    const void *k = P == 4 ? reinterpret_cast<const void *>(&k_lp_step4_astrip<COL, NW>)
+                           : reinterpret_cast<const void *>(&k_lp_step5_astrip<COL, NW>);
+    static const int per_cu = [&] {
         int nblk = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, reinterpret_cast<const void *>(&k_lp_step4_astrip<COL, NW>),
-                                                         64 * NW, lds) != hipSuccess ||
-            nblk < 1)
-            nblk = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, k, 64 * NW, lds) != hipSuccess || nblk < 1) nblk = 1;
         return nblk;
     }();
     const int pc = h->tune_fuse_slots > 0 ? std::min(per_cu, h->tune_fuse_slots) : per_cu;
     const int unit = h->nx % 16 == 0 ? 16 : 1;
-    const int nstrip = (h->nx / unit + (64 * NW - 6) / unit - 1) / ((64 * NW - 6) / unit);
+    constexpr int TX = 64 * NW - 2 * (P - 1);
+    const int nstrip = (h->nx / unit + TX / unit - 1) / (TX / unit);
     const int nseg = (int)std::max(1ll, std::min<long long>(h->ny / 16, (long long)h->n_cu * pc / nstrip));
     const int prio = h->tune_fuse_prio >= 0 ? h->tune_fuse_prio : (nstrip * nseg >= 2 * h->n_cu ? 2 : 0);
-    hipLaunchKernelGGL((k_lp_step4_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream, l,
-                       h->tune_fuse_remap, nstrip, nseg, unit, prio);
+    if constexpr (P == 4)
+        hipLaunchKernelGGL((k_lp_step4_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream,
+                           l, h->tune_fuse_remap, nstrip, nseg, unit, prio);
+    else
+        hipLaunchKernelGGL((k_lp_step5_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream,
+                           l, h->tune_fuse_remap, nstrip, nseg, unit, prio);
 }
 
 // Strips of at most 252 positions, edges on 16-position units where the
@@ -630,7 +644,8 @@ void launch_lp_step2_ct(erlbm_lattice *h, const LPArgs &l)
         // three-step walk and the two-step remainder walk; any other shape
         // keeps the reference's order
         if constexpr (sizeof(T) == 4) {
-            if (h->fuse_active == FUSE4_ASTRIP_4W) return launch_lp_astrip4<COL>(h, l);
+            if (h->fuse_active == FUSE5_ASTRIP_4W) return launch_lp_astripP<COL, 5>(h, l);
+            if (h->fuse_active == FUSE4_ASTRIP_4W) return launch_lp_astripP<COL, 4>(h, l);
             if (h->fuse_active == FUSE3_ASTRIP_4W) return launch_lp_astrip<COL>(h, l);
             if (h->fuse_active == FUSE2_STRIP_256x32) return launch_lp_strip<COL, T, 256, 32>(h, l);
         }
@@ -639,7 +654,8 @@ void launch_lp_step2_ct(erlbm_lattice *h, const LPArgs &l)
         if (h->fuse_active == FUSE3_STRIP_256x32) return launch_lp_strip3<COL, 256, 32>(h, l);
         if (h->fuse_active == FUSE3_STRIP_128x32) return launch_lp_strip3<COL, 128, 32>(h, l);
         if (h->fuse_active == FUSE3_ASTRIP_4W) return launch_lp_astrip<COL>(h, l);
-        if (h->fuse_active == FUSE4_ASTRIP_4W) return launch_lp_astrip4<COL>(h, l);
+        if (h->fuse_active == FUSE4_ASTRIP_4W) return launch_lp_astripP<COL, 4>(h, l);
+        if (h->fuse_active == FUSE5_ASTRIP_4W) return launch_lp_astripP<COL, 5>(h, l);
     }
     if (h->fuse_active == FUSE2_STRIP_128x32) launch_lp_strip<COL, T, 128, 32>(h, l);
     else launch_lp_strip<COL, T, 256, 32>(h, l);

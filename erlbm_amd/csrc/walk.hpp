@@ -177,6 +177,15 @@ __device__ __forceinline__ float buf_load_elem<float>(__amdgpu_buffer_rsrc_t r, 
     return buf_load_f32<0>(r, voff, soff);
 }
 
+// Slot of a c_x != 0 population in the rings of the P-step walks (one row
+// per population): 0 .. 5 for populations 1, 2, 5, 6, 7, 8.
+__host__ __device__ constexpr int xslot(int i)
+{
+    return i == 0 ? 0 : xslot(i - 1) + (VSet<9>::cx[i - 1] != 0 ? 1 : 0);
+}
+constexpr int XSLOTS = 6;
+static_assert(xslot(8) == 5 && xslot(5) == 2, "D2Q9 ring slots");
+
 // Four or five steps per launch (P phases): the lane-aligned walk with P-1
 // rings.  Lane t holds position x0 - (P-1) + t in all phases (phase p:
 // lanes p-1 .. w+2P-p-2; the last phase stores lanes P-1 .. w+P-2).  With two
@@ -186,18 +195,22 @@ __device__ __forceinline__ float buf_load_elem<float>(__amdgpu_buffer_rsrc_t r, 
 // between the barriers, phases 3 (row r - 2 DIR) and 5 (row r - 5 DIR)
 // after the second; phase 4 so reads the ring-3 rows phase 3 wrote in the
 // walking steps before, phase 5 the ring-4 rows phase 4 wrote in the same
-// one.  Ring p keeps the c_x != 0 results of phase p (lean layout, 8 rows:
-// 16 KB per ring of FP64 for NW = 4), the c_x = 0 ones stay in registers;
-// a ring's late writes (the populations not read from the newest row)
-// follow its first reader by one interval as in strip3_walk_aligned: rings
-// 1 and 3 after the second barrier (of the same / the next walking step),
-// rings 2 and 4 after the first barrier of the next walking step.  Output
-// rows ya .. ya+rows-1 need step-k rows ya-(P-1) .. ya+rows+P-2: rows + 7
-// (P = 4) or rows + 9 (P = 5) walking steps.
+// one.  Ring p holds the c_x != 0 results of phase p, one row per
+// population (6 x 64 NW values): every ring value has exactly one reader,
+// so each is written in the interval just before the one its reader runs
+// in, from registers — the populations read from the newest row (c_y DIR <
+// 0) at once, those read from the middle row one walking step later, those
+// read from the oldest row two walking steps later — and each slot is free
+// again when the next row's value arrives.  Rings 1 and 3 are written
+// after the second barrier (and ring 1 before the first), rings 2 and 4
+// between the barriers: every ring is written in one interval kind and
+// read in the other.  The c_x = 0 results stay in the lane's registers.
+// Output rows ya .. ya+rows-1 need step-k rows ya-(P-1) .. ya+rows+P-2:
+// rows + 7 (P = 4) or rows + 9 (P = 5) walking steps.
 // T: the population arrays' element type (FP64 storage, or FP32 shifted
 // storage whose collision converts to FP64 and back); Args: the launch's
 // arguments (src, dst, g, step, stop_ro); collide(T (&)[9]) collides one
-// node in place; ring: P-1 rings of XRING_ROWS x 64 NW elements.
+// node in place; ring: P-1 rings of XSLOTS x 64 NW elements.
 template <typename T, int NW, int DIR, int P, class Args, class Collide>
 __device__ __forceinline__ void stripP_walk_aligned(const Args &a, int x0, int w, int ya, int rows, T *ring, int prio,
                                                     Collide collide)
@@ -206,8 +219,8 @@ __device__ __forceinline__ void stripP_walk_aligned(const Args &a, int x0, int w
     constexpr unsigned SZ = sizeof(T);
     using V = VSet<9>;
     constexpr int E = 64 * NW;
-    T *const ring1 = ring, *const ring2 = ring + XRING_ROWS * E, *const ring3 = ring + 2 * XRING_ROWS * E;
-    T *const ring4 = ring + 3 * XRING_ROWS * E;   // P = 5 only
+    T *const ring1 = ring, *const ring2 = ring + XSLOTS * E, *const ring3 = ring + 2 * XSLOTS * E;
+    T *const ring4 = ring + 3 * XSLOTS * E;   // P = 5 only
     const int t = (int)threadIdx.x;
     const int nx = a.g.nx, ny = a.g.ny;
     // phase p covers positions x0-(P-p) .. x0+w-1+(P-p): lanes p-1 .. w+2P-p-2
@@ -228,30 +241,28 @@ __device__ __forceinline__ void stripP_walk_aligned(const Args &a, int x0, int w
             f[i] = buf_load_elem<T>(rs, vo[V::cx[i] + 1] | off, (int)(SZ * at(a.g, i, ys, 0)));
         });
     };
-    auto rrow = [&](int i, int y) { return xring_base<DIR>(i) + (y - ya + P) % xring_depth<DIR>(i); };
-    auto inring = [](int i) { return V::cx[i] != 0; };
-    auto now = [](int i) { return V::cy[i] * DIR < 0; };
-    auto age = [](int i) { return 1 + V::cy[i] * DIR; };
+    auto age = [](int i) { return 1 + V::cy[i] * DIR; };   // 0: newest source row, 1: middle, 2: oldest
     // pull of a phase from the ring of the phase before (c_x != 0) and from
     // that phase's results in registers (c_x = 0: this lane's own, of the
     // newest, middle and oldest source row)
-    auto pull = [&](T (&g)[9], const T *rg, int y, const T (&r0)[9], const T (&r1)[9], const T (&r2)[9]) {
+    auto pull = [&](T (&g)[9], const T *rg, const T (&r0)[9], const T (&r1)[9], const T (&r2)[9]) {
         static_for<9>([&](auto I) {
             constexpr int i = I;
-            if constexpr (V::cx[i] != 0) g[i] = rg[rrow(i, y - V::cy[i]) * E + (t - V::cx[i])];
+            if constexpr (V::cx[i] != 0) g[i] = rg[xslot(i) * E + (t - V::cx[i])];
             else g[i] = age(i) == 0 ? r0[i] : (age(i) == 1 ? r1[i] : r2[i]);
         });
     };
-    auto write_now = [&](T *rg, int y, const T (&g)[9]) {
+    // ring writes of the populations whose reader takes them from the
+    // newest / middle / oldest of its three source rows
+    auto put = [&](T *rg, const T (&g)[9], int which) {
         static_for<9>([&](auto I) {
-            if (inring(I) && now(I)) rg[rrow((int)I, y) * E + t] = g[(int)I];
+            constexpr int i = I;
+            if constexpr (V::cx[i] != 0) {
+                if (age(i) == which) rg[xslot(i) * E + t] = g[i];
+            }
         });
     };
-    auto write_late = [&](T *rg, int y, const T (&g)[9]) {
-        static_for<9>([&](auto I) {
-            if (inring(I) && !now(I)) rg[rrow((int)I, y) * E + t] = g[(int)I];
-        });
-    };
+    auto keep = [](T (&dst)[9], const T (&src)[9]) { static_for<9>([&](auto I) { dst[(int)I] = src[(int)I]; }); };
     auto shift = [](T (&ka)[9], T (&kb)[9], const T (&g)[9]) {   // c_x = 0 results one walking step older
         static_for<9>([&](auto I) {
             if constexpr (V::cx[(int)I] == 0) {
@@ -268,7 +279,9 @@ __device__ __forceinline__ void stripP_walk_aligned(const Args &a, int x0, int w
     const int nrows1 = rows + 2 * (P - 1);           // rows of phase 1
     const int nsteps = P == 4 ? rows + 7 : rows + 9;  // walking steps
     T fq[9];
-    T h2[9], h3[9], h4[9];         // the previous walking step's phase-2 / 3 / 4 rows (late ring writes)
+    T f1[9];                       // phase-1 row of the walking step before (its oldest-row ring writes)
+    T h2[9], h3[9], h4[9];         // phase-2 / 3 / 4 rows of the walking step before
+    T h2p[9], h3p[9], h4p[9];      // the same two walking steps before
     T k1a[9], k1b[9];              // c_x = 0: phase-1 results of the rows one and two walking steps back
     T k2a[9], k2b[9];              // phase 2, likewise
     T k3a[9], k3b[9];              // phase 3, one and two walking steps before h3
@@ -285,26 +298,29 @@ __device__ __forceinline__ void stripP_walk_aligned(const Args &a, int x0, int w
         const bool p1 = act1 && n < nrows1;
         if (p1) {                                                     // phase 1: step k, row r
             collide(f);
-            write_now(ring1, r, f);
+            put(ring1, f, 0);
         }
         lds_barrier();
-        if (act2 && n >= 3) write_late(ring2, r - 2 * DIR, h2);       // the rest of phase-2 row r-2 DIR
+        if (act2 && n >= 3) put(ring2, h2, 1);                        // phase-2 rows of the steps before
+        if (act2 && n >= 4) put(ring2, h2p, 2);
         if constexpr (P == 5) {
-            if (act4 && n >= 8) write_late(ring4, r - 5 * DIR, h4);   // the rest of phase-4 row r-5 DIR
+            if (act4 && n >= 8) put(ring4, h4, 1);                    // phase-4 rows of the steps before
+            if (act4 && n >= 9) put(ring4, h4p, 2);
         }
         if (act2 && n >= 2 && n < nrows1) {                           // phase 2: step k+1, row r-DIR
             T g[9];
-            pull(g, ring1, r - DIR, f, k1a, k1b);
+            pull(g, ring1, f, k1a, k1b);
             collide(g);
-            write_now(ring2, r - DIR, g);
-            static_for<9>([&](auto I) { h2[(int)I] = g[(int)I]; });
+            put(ring2, g, 0);
+            keep(h2p, h2);
+            keep(h2, g);
         }
         shift(k1a, k1b, f);
         const int y4 = n >= 7 ? r - 4 * DIR : ya;                     // phase 4: step k+3, row r-4 DIR
         if constexpr (P == 4) {
             T o[9] = {};
             if (act4 && n >= 7) {
-                pull(o, ring3, y4, h3, k3a, k3b);
+                pull(o, ring3, h3, k3a, k3b);
                 collide(o);
             }
             shift(k3a, k3b, h3);
@@ -312,29 +328,34 @@ __device__ __forceinline__ void stripP_walk_aligned(const Args &a, int x0, int w
         } else {
             if (act4 && n >= 7) {
                 T g[9];
-                pull(g, ring3, y4, h3, k3a, k3b);
+                pull(g, ring3, h3, k3a, k3b);
                 collide(g);
-                write_now(ring4, y4, g);
-                static_for<9>([&](auto I) { h4[(int)I] = g[(int)I]; });
+                put(ring4, g, 0);
+                keep(h4p, h4);
+                keep(h4, g);
             }
             shift(k3a, k3b, h3);
         }
         lds_barrier();
-        if (p1) write_late(ring1, r, f);                              // the rest of phase-1 row r
-        if (act3 && n >= 5) write_late(ring3, r - 3 * DIR, h3);       // the rest of phase-3 row r-3 DIR
+        if (p1) put(ring1, f, 1);                                     // phase-1 rows r and r-DIR
+        if (act1 && n >= 1 && n <= nrows1) put(ring1, f1, 2);
+        keep(f1, f);
+        if (act3 && n >= 5) put(ring3, h3, 1);                        // phase-3 rows of the steps before
+        if (act3 && n >= 6) put(ring3, h3p, 2);
         if (act3 && n >= 4 && n < nrows1) {                           // phase 3: step k+2, row r-2 DIR
             T g[9];
-            pull(g, ring2, r - 2 * DIR, h2, k2a, k2b);
+            pull(g, ring2, h2, k2a, k2b);
             collide(g);
-            write_now(ring3, r - 2 * DIR, g);
-            static_for<9>([&](auto I) { h3[(int)I] = g[(int)I]; });
+            put(ring3, g, 0);
+            keep(h3p, h3);
+            keep(h3, g);
         }
         shift(k2a, k2b, h2);
         if constexpr (P == 5) {                                       // phase 5: step k+4, row r-5 DIR
             const int y5 = n >= 9 ? r - 5 * DIR : ya;
             T o[9] = {};
             if (act5 && n >= 9) {
-                pull(o, ring4, y5, h4, k4a, k4b);
+                pull(o, ring4, h4, k4a, k4b);
                 collide(o);
             }
             shift(k4a, k4b, h4);

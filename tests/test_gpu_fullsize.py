@@ -1,8 +1,8 @@
 """The double-shear-layer workload at its full size: D2Q9 E-RLBM, periodic
 L = 2048 (4,194,304 nodes), tau = 0.5 + 3 U0 L / Re (Re = 80000, U0 = 1/32),
 tau_nh = 0.65, equilibrium initialisation of src/main_double-shear-layer.cpp:
-48-66, 1000 steps, as the default kernels run it (four steps per launch;
-five with the moment-space arithmetic), and on the three- and four-step
+48-66, 1000 steps, as the default kernels run it (five steps per launch in
+FP64, four in FP32-shifted storage), and on the three- and four-step
 walks.
 
   * FP64: bitwise the strict oracle (the reference's operation order, no FMA
@@ -71,10 +71,10 @@ def _vel(f):
     return (f @ c) / f.sum(1)[:, None]
 
 
-@pytest.mark.parametrize("fuse,want_spl", [(None, 4), ("42", 3)])
+@pytest.mark.parametrize("fuse,want_spl", [(None, 5), ("46", 4), ("42", 3)])
 def test_dsl2048_1000_steps_default_kernel_bitwise(gpu, monkeypatch, dsl_setup, strict_1000, fuse, want_spl):
-    """The default kernel (four steps per launch) and the three-step
-    lane-aligned walk."""
+    """The default kernel (five steps per launch) and the four- and
+    three-step lane-aligned walks."""
     f0, _ = dsl_setup
     if fuse:
         monkeypatch.setenv("ERLBM_FUSE", fuse)

@@ -504,7 +504,7 @@ def test_lattice_errors_leave_the_handle_usable(gpu, orc):
 
 def test_headline_default_kernel_full_size(gpu, orc):
     """The headline workload as the bench runs it (D2Q9 double shear,
-    L = 2048, the default multi-step kernel: four steps per launch): 25
+    L = 2048, the default multi-step kernel: five steps per launch): 25
     steps bitwise the strict oracle; total mass conserved to rounding."""
     L, n = 2048, 25
     tau = 0.5 + 3. * (1. / 32.) * L / 80000.
@@ -514,7 +514,7 @@ def test_headline_default_kernel_full_size(gpu, orc):
     with _periodic(gpu, 9, L, tau, 0.65, O.COL_EXT_REG, f0) as lat:
         shape, spl, desc = lat.step_shape()
         print(f"headline default: shape {shape}, {desc}", flush=True)
-        assert spl == 4
+        assert spl == 5
         lat.step(n)
         got = lat.download()
     assert np.array_equal(got, want)
@@ -542,14 +542,13 @@ def test_aligned_walk_issue_priority_bitwise(gpu, orc, monkeypatch, prio):
 
 def test_default_shape_rule(gpu):
     """The default multi-step shape per lattice size (periodic FP64 D2Q9
-    E-RLBM; gfx950 parts have 256 CUs): the four-step lane-aligned walk
-    (five-step with the moment-space arithmetic) where it makes 1.5
-    workgroups per CU or more, else the three-step
+    E-RLBM; gfx950 parts have 256 CUs): the five-step lane-aligned walk
+    where it makes 1.5 workgroups per CU or more, else the three-step
     lane-aligned any-width walk where it loads the CUs no more unevenly
     than the 256-wide walk or makes three workgroups per CU, the 256-wide
     walk at L = 1024, the 64x8 two-step tiles at L = 768, and
     steps_per_launch / descriptions that match."""
-    want = {768: (1, 2), 1000: (42, 3), 1024: (26, 3), 1280: (46, 4), 2048: (46, 4), 3000: (46, 4), 4096: (46, 4)}
+    want = {768: (1, 2), 1000: (42, 3), 1024: (26, 3), 1280: (47, 5), 2048: (47, 5), 3000: (47, 5), 4096: (47, 5)}
     words = {2: "two steps", 3: "three steps", 4: "four steps", 5: "five steps"}
     for L, (shape, spl) in want.items():
         with gpu.Lattice(9, L, L, 1) as lat:
@@ -558,15 +557,15 @@ def test_default_shape_rule(gpu):
             print(f"L={L}: {got}", flush=True)
             assert got[:2] == (shape, spl), L
             assert got[2].startswith(words[spl])
-    # the moment-space arithmetic: five steps per launch where they fill the GPU
+    # the moment-space arithmetic and MRT take the same shapes
     for L, want_shape in ((1024, (26, 3)), (1280, (47, 5)), (2048, (47, 5)), (4096, (47, 5))):
         with gpu.Lattice(9, L, L, 1) as lat:
             lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
             lat.set_arith(gpu.ARITH_MOMENT)
             got = lat.step_shape()
             assert got[:2] == want_shape, (L, got)
-            lat.set_params(0.51, 0.65, 1.0, O.COL_MRT)   # MRT keeps the reference's order: four steps
-            assert lat.step_shape()[:2] == ((26, 3) if L == 1024 else (46, 4)), L
+            lat.set_params(0.51, 0.65, 1.0, O.COL_MRT)
+            assert lat.step_shape()[:2] == want_shape, L
     # shifted storage: FP32 takes the four-step lane-aligned walk, FP16 one
     # step per launch
     for prec, want_shape in ((gpu.FP32_SHIFTED, (46, 4)), (gpu.FP16_SHIFTED, (0, 1))):

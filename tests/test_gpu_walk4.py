@@ -105,21 +105,23 @@ def test_four_step_walk_fma_tolerance_1000_steps(gpu, monkeypatch, shape):
     assert err <= 1e-10
 
 
+@pytest.mark.parametrize("shape", [46, 47])
 @pytest.mark.parametrize("dims", [(1000, 24), (253, 16), (2048, 40), (300, 77)])
-def test_four_step_walk_fp32_storage_bitwise_one_step(gpu, orc, monkeypatch, dims):
-    """FP32 shifted storage (FP64 arithmetic): the four-step walk keeps the
-    FP32 values the one-step launches would store, so it is bitwise the
-    one-step FP32 path, for call lengths leaving every remainder."""
+def test_four_step_walk_fp32_storage_bitwise_one_step(gpu, orc, monkeypatch, shape, dims):
+    """FP32 shifted storage (FP64 arithmetic): the four- and five-step walks
+    keep the FP32 values the one-step launches would store, so they are
+    bitwise the one-step FP32 path, for call lengths leaving every
+    remainder."""
     nx, ny = dims
     f0, _ = _perturbed(orc, nx, ny, nx + 7 * ny)
     out = []
-    for fuse in (str(SHAPE), "0"):
+    for fuse in (str(shape), "0"):
         monkeypatch.setenv("ERLBM_FUSE", fuse)
         with gpu.Lattice(9, nx, ny, 1, precision=gpu.FP32_SHIFTED) as lat:
             lat.set_params(0.52, 0.65, 1.0, O.COL_EXT_REG)
             lat.upload(f0)
-            assert lat.step_shape()[:2] == ((SHAPE, 4) if fuse != "0" else (0, 1))
-            for n in (1, 8, 3, 2, 9):
+            assert lat.step_shape()[:2] == ((shape, STEPS[shape]) if fuse != "0" else (0, 1))
+            for n in (1, 8, 3, 2, 9, 10, 4):
                 lat.step(n)
             out.append(lat.download())
     assert np.array_equal(out[0], out[1])
