@@ -2402,7 +2402,11 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
             {FUSE3_ASTRIP_3W, {"three steps, lane-aligned strips of any width (3 wavefronts)",
                                "three steps, lane-aligned strips of any width (3 wavefronts), XCD runs"}},
             {FUSE3_ASTRIP_2W, {"three steps, lane-aligned strips of any width (2 wavefronts)",
-                               "three steps, lane-aligned strips of any width (2 wavefronts), XCD runs"}}};
+                               "three steps, lane-aligned strips of any width (2 wavefronts), XCD runs"}},
+            {FUSE4_ASTRIP_4W, {"four steps, lane-aligned strips of any width (4 wavefronts)",
+                               "four steps, lane-aligned strips of any width (4 wavefronts), XCD runs"}},
+            {FUSE5_ASTRIP_4W, {"five steps, lane-aligned strips of any width (4 wavefronts)",
+                               "five steps, lane-aligned strips of any width (4 wavefronts), XCD runs"}}};
         for (const Named &nm : fused) {
             if (!fuse_shape_fits(h, nm.f)) continue;
             for (int rm = 0; rm < 2; rm++) {
@@ -2442,11 +2446,18 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
                 if (!fuse_shape_fits(h, fs[k])) continue;
                 for (int rm = 0; rm < 2; rm++) c.push_back({h->tune_ntl, h->tune_block, 1, 0, 0, names[k][rm], fs[k], rm});
             }
-            static const char *aligned[2] = {"three steps, lane-aligned strips of any width (4 wavefronts)",
-                                             "three steps, lane-aligned strips of any width (4 wavefronts), XCD runs"};
-            if (fuse_shape_fits(h, FUSE3_ASTRIP_4W))
-                for (int rm = 0; rm < 2; rm++)
-                    c.push_back({h->tune_ntl, h->tune_block, 1, 0, 0, aligned[rm], FUSE3_ASTRIP_4W, rm});
+            static const char *aligned[3][2] = {
+                {"three steps, lane-aligned strips of any width (4 wavefronts)",
+                 "three steps, lane-aligned strips of any width (4 wavefronts), XCD runs"},
+                {"four steps, lane-aligned strips of any width (4 wavefronts)",
+                 "four steps, lane-aligned strips of any width (4 wavefronts), XCD runs"},
+                {"five steps, lane-aligned strips of any width (4 wavefronts)",
+                 "five steps, lane-aligned strips of any width (4 wavefronts), XCD runs"}};
+            const int af[3] = {FUSE3_ASTRIP_4W, FUSE4_ASTRIP_4W, FUSE5_ASTRIP_4W};
+            for (int k = 0; k < 3; k++)
+                if (fuse_shape_fits(h, af[k]))
+                    for (int rm = 0; rm < 2; rm++)
+                        c.push_back({h->tune_ntl, h->tune_block, 1, 0, 0, aligned[k][rm], af[k], rm});
         }
         c.push_back({h->tune_ntl, h->tune_block, 1, 0, 0, "one node/thread"});
     }

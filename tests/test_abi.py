@@ -95,19 +95,19 @@ def test_errors_are_reported_not_crashing():
 
 
 def test_autotune_candidates_respect_the_31_bit_buffer_limit():
-    """The autotuner offers the raw-buffer strip walks (two- and three-step,
+    """The autotuner offers the raw-buffer strip walks (two- to five-step,
     lane-aligned) only where their byte offsets stay below 2^31, as erlbm_step
     does: an FP32-shifted 8192^2 lattice (2.4 GB per buffer) gets none of
     them, a 2048^2 one gets them all.  Host-side; no device call."""
     from erlbm_amd import lattice as Lm
     from erlbm_amd._lib import FP32_SHIFTED, FP64
-    walks = {11, 13, 22, 24, 42}
+    walks = {11, 13, 22, 24, 42, 46, 47}
     small = set(Lm.autotune_candidates(9, 2048, 2048, 1, FP32_SHIFTED))
     assert walks <= small, small
     big = set(Lm.autotune_candidates(9, 8192, 8192, 1, FP32_SHIFTED))
     assert not (walks & big), big
     # FP64: the raw-buffer walks stop where 8-byte offsets pass 2^31 as well
-    assert {42, 26, 39} <= set(Lm.autotune_candidates(9, 4096, 4096, 1, FP64))
-    assert not ({42, 26, 39, 11, 13, 22, 16} & set(Lm.autotune_candidates(9, 8192, 8192, 1, FP64)))
+    assert {42, 26, 39, 46, 47} <= set(Lm.autotune_candidates(9, 4096, 4096, 1, FP64))
+    assert not ({42, 26, 39, 11, 13, 22, 16, 46, 47} & set(Lm.autotune_candidates(9, 8192, 8192, 1, FP64)))
     # D3Q19: the z-walk tiles where the geometry takes them
     assert {33, 31, 30} <= set(Lm.autotune_candidates(19, 256, 256, 256, FP64))
