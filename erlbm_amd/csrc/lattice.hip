@@ -1450,7 +1450,7 @@ extern "C" int erlbm_upload_aos(erlbm_lattice *h, const double *inif)
 namespace {
 
 // Shapes of the two-step kernel (ERLBM_FUSE selects one; 0 = off).
-enum FuseKind { FUSE_TILE, FUSE_STRIP, FUSE_WAVE, FUSE_STRIP3, FUSE_ZWALK, FUSE_GSTRIP3, FUSE_GSTRIP4, FUSE_GSTRIP5 };
+enum FuseKind { FUSE_TILE, FUSE_STRIP, FUSE_WAVE, FUSE_STRIP3, FUSE_ZWALK, FUSE_GSTRIP3, FUSE_GSTRIP4, FUSE_GSTRIP5, FUSE_GSTRIP6 };
 struct FuseShape {
     int tx, ty;       // tile; (strip) strip width and rows walked; (wave) rows walked
     FuseKind kind;
@@ -1483,9 +1483,9 @@ constexpr FuseShape FUSE_SHAPES[] = {
     // in registers), 2, 3, 4, 8, 5 and 6 wavefronts per workgroup
     {124, 0, FUSE_GSTRIP3},            {188, 0, FUSE_GSTRIP3},           {252, 0, FUSE_GSTRIP3},
     {508, 0, FUSE_GSTRIP3},            {316, 0, FUSE_GSTRIP3},           {380, 0, FUSE_GSTRIP3},
-    // four and five steps per launch: lane-aligned strips of any width up to
-    // 64 NW - 6 / 64 NW - 8, four wavefronts
-    {250, 0, FUSE_GSTRIP4},            {248, 0, FUSE_GSTRIP5}};
+    // four, five and six steps per launch: lane-aligned strips of any width
+    // up to 64 NW - 2 (steps - 1), four wavefronts
+    {250, 0, FUSE_GSTRIP4},            {248, 0, FUSE_GSTRIP5},           {246, 0, FUSE_GSTRIP6}};
 constexpr int N_FUSE_SHAPES = sizeof(FUSE_SHAPES) / sizeof(FUSE_SHAPES[0]);
 
 // Steps one launch of the shape advances.
@@ -1493,6 +1493,7 @@ int fuse_steps(int f)
 {
     if (f <= 0 || f >= N_FUSE_SHAPES) return 2;
     const FuseKind k = FUSE_SHAPES[f].kind;
+    if (k == FUSE_GSTRIP6) return 6;
     return k == FUSE_GSTRIP5 ? 5 : (k == FUSE_GSTRIP4 ? 4 : (k == FUSE_STRIP3 || k == FUSE_GSTRIP3 ? 3 : 2));
 }
 
@@ -1516,7 +1517,7 @@ bool fuse_geometry_fits(const erlbm_lattice *h, const FuseShape &fs)
     }
     if (h->nz != 1) return false;
     if (fs.kind == FUSE_WAVE) return h->nx >= 64 && h->extent * 8 < (1ll << 31);   // 31-bit buffer offsets
-    if (fs.kind == FUSE_GSTRIP3 || fs.kind == FUSE_GSTRIP4 || fs.kind == FUSE_GSTRIP5)
+    if (fs.kind == FUSE_GSTRIP3 || fs.kind == FUSE_GSTRIP4 || fs.kind == FUSE_GSTRIP5 || fs.kind == FUSE_GSTRIP6)
         return h->extent * h->elem < (1ll << 31) && h->nx >= 16 && h->ny >= 16;
     if ((fs.kind == FUSE_STRIP || fs.kind == FUSE_STRIP3) && h->elem == 8 && h->extent * 8 >= (1ll << 31)) return false;
     if (fs.ty == 0) return h->nx % fs.tx == 0 && h->ny >= 16;
@@ -1880,6 +1881,18 @@ __global__ __launch_bounds__(64 * NW) void k_step5_astrip(StepArgs a, int remap,
     astripP_workgroup<double, NW, 5>(a, remap, nstrip, nseg, unit, prio, rings, collide);
 }
 
+template <int COL, int NW>
+__global__ __launch_bounds__(64 * NW) void k_step6_astrip(StepArgs a, int remap, int nstrip, int nseg, int unit,
+                                                         int prio)
+{
+    extern __shared__ double rings[];
+    auto collide = [&](double (&f)[9]) {
+        double vx, vy, vz;
+        collide_sel<9, COL>(f, a.k, vx, vy, vz);
+    };
+    astripP_workgroup<double, NW, 6>(a, remap, nstrip, nseg, unit, prio, rings, collide);
+}
+
 // Strips of at most 64 NW - 4 positions, as few as the width allows;
 // segments as for launch_strip3_t with SEG = 0.
 template <int NW, bool ALIGNED>
@@ -2059,6 +2072,25 @@ void launch_astrip5_t(erlbm_lattice *h, const StepArgs &a)
                        h->tune_fuse_remap, nstrip, nseg, unit, walk_prio(h, nstrip * nseg));
 }
 
+// Six-step lane-aligned walk: five such rings (60 KB for NW = 4).
+template <int COL, int NW>
+void launch_astrip6_t(erlbm_lattice *h, const StepArgs &a)
+{
+    constexpr size_t lds = sizeof(double) * 5 * XSLOTS * (64 * NW);
+    static const int per_cu = [] {
+        const void *k = reinterpret_cast<const void *>(&k_step6_astrip<COL, NW>);
+        if constexpr (lds >= 65536) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int nblk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, k, 64 * NW, lds) != hipSuccess || nblk < 1) nblk = 1;
+        return nblk;
+    }();
+    const int unit = astrip_unit(h);
+    int nstrip, nseg;
+    strip3_grid(h, 64 * NW - 10, true, per_cu, nstrip, nseg, unit);
+    hipLaunchKernelGGL((k_step6_astrip<COL, NW>), dim3((unsigned)(nstrip * nseg)), dim3(64 * NW), lds, h->stream, a,
+                       h->tune_fuse_remap, nstrip, nseg, unit, walk_prio(h, nstrip * nseg));
+}
+
 template <int COL, int NW>
 void launch_astrip4_t(erlbm_lattice *h, const StepArgs &a)
 {
@@ -2109,6 +2141,7 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
         case FUSE3_ASTRIP_4W: launch_gstrip3_t<COL, 4, true>(h, a); return;
         case FUSE4_ASTRIP_4W: launch_astrip4_t<COL, 4>(h, a); return;
         case FUSE5_ASTRIP_4W: launch_astrip5_t<COL, 4>(h, a); return;
+        case FUSE6_ASTRIP_4W: launch_astrip6_t<COL, 4>(h, a); return;
         case FUSE2_STRIP_256x32: launch_strip_t<COL, 256, 32>(h, a); return;
         default: launch_fused_c<(COL & ~COL_MOMENT)>(h, a); return;
         }
@@ -2145,6 +2178,7 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
     case FUSE3_ASTRIP_6W: launch_gstrip3_t<COL, 6, true>(h, a); return;
     case FUSE4_ASTRIP_4W: launch_astrip4_t<COL, 4>(h, a); return;
     case FUSE5_ASTRIP_4W: launch_astrip5_t<COL, 4>(h, a); return;
+    case FUSE6_ASTRIP_4W: launch_astrip6_t<COL, 4>(h, a); return;
     case 16: launch_wave_t<COL, 1, 4, 1>(h, a, 2); return;
     case 17: launch_wave_t<COL, 1, 4, 1>(h, a, 1); return;
     case 18: launch_wave_t<COL, 2, 4, 1>(h, a, 2); return;
@@ -2237,12 +2271,22 @@ int dispatch_step(erlbm_lattice *h, const StepArgs &a, bool stream)
 
 namespace {
 
-// Four-step shape for a remainder of four steps after five-step launches:
+// Five-step shape for a remainder of five steps after six-step launches:
+// the five-step walk where the size rule would take it.  0: none.
+long long astrip5_workgroups(const erlbm_lattice *h);
+int remainder5_shape(const erlbm_lattice *h)
+{
+    if (fuse_steps(h->fuse_active) != 6) return 0;
+    const int f = FUSE5_ASTRIP_4W;
+    return fuse_shape_fits(h, f) && 2 * astrip5_workgroups(h) >= 3ll * h->n_cu ? f : 0;
+}
+
+// Four-step shape for a remainder of four steps after five- or six-step launches:
 // the four-step walk where the size rule would take it.  0: none.
 long long astrip4_workgroups(const erlbm_lattice *h);
 int remainder4_shape(const erlbm_lattice *h)
 {
-    if (fuse_steps(h->fuse_active) != 5) return 0;
+    if (fuse_steps(h->fuse_active) < 5) return 0;
     const int f = FUSE4_ASTRIP_4W;
     return fuse_shape_fits(h, f) && 2 * astrip4_workgroups(h) >= 3ll * h->n_cu ? f : 0;
 }
@@ -2305,6 +2349,7 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
     const int rem2 = fuse ? remainder2_shape(h) : 0;
     const int rem3 = fuse ? remainder3_shape(h) : 0;
     const int rem4 = fuse ? remainder4_shape(h) : 0;
+    const int rem5 = fuse ? remainder5_shape(h) : 0;
     for (int s = 0; s < n_steps;) {
         const long long k = h->launched;
         const int cur = (h->base + (int)(k & 1)) & 1;
@@ -2325,6 +2370,16 @@ extern "C" int erlbm_step(erlbm_lattice *h, int n_steps, int energy_interval, do
             h->launched += spl;
             if (spl % 2 == 0) h->base ^= 1;
             s += spl;
+            continue;
+        }
+        if (rem5 && k > 0 && n_steps - s == 5) {
+            const int keep = h->fuse_active;
+            h->fuse_active = rem5;
+            int rc = launch_fused(h, a);
+            h->fuse_active = keep;
+            if (rc) return rc;
+            h->launched += 5;
+            s += 5;
             continue;
         }
         if (rem4 && k > 0 && n_steps - s == 4) {
@@ -2505,6 +2560,7 @@ std::string fuse_shape_desc(int f, int remap)
     case FUSE_ZWALK: d = "two steps, " + tx + "x" + ty + " tiles walked along z"; break;
     case FUSE_GSTRIP4: d = "four steps, lane-aligned strips of any width (" + std::to_string((fs.tx + 6) / 64) + " wavefronts)"; break;
     case FUSE_GSTRIP5: d = "five steps, lane-aligned strips of any width (" + std::to_string((fs.tx + 8) / 64) + " wavefronts)"; break;
+    case FUSE_GSTRIP6: d = "six steps, lane-aligned strips of any width (" + std::to_string((fs.tx + 10) / 64) + " wavefronts)"; break;
     }
     return remap ? d + ", XCD runs" : d;
 }

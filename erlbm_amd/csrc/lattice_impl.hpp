@@ -132,8 +132,8 @@ constexpr int FUSE3_GSTRIP_1W = 36, FUSE3_GSTRIP_2W = 37, FUSE3_GSTRIP_3W = 38, 
 // the same with the lanes aligned across the walk's phases, 2, 3, 4, 8 wavefronts
 constexpr int FUSE3_ASTRIP_2W = 40, FUSE3_ASTRIP_3W = 41, FUSE3_ASTRIP_4W = 42, FUSE3_ASTRIP_8W = 43,
               FUSE3_ASTRIP_5W = 44, FUSE3_ASTRIP_6W = 45;
-// D2Q9 four and five steps per launch, lane-aligned strips of any width, 4 wavefronts
-constexpr int FUSE4_ASTRIP_4W = 46, FUSE5_ASTRIP_4W = 47;
+// D2Q9 four, five and six steps per launch, lane-aligned strips of any width, 4 wavefronts
+constexpr int FUSE4_ASTRIP_4W = 46, FUSE5_ASTRIP_4W = 47, FUSE6_ASTRIP_4W = 48;
 
 struct erlbm_lattice {
     int q = 0, nx = 0, ny = 0, nz = 0;

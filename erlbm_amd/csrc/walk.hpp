@@ -186,7 +186,7 @@ __host__ __device__ constexpr int xslot(int i)
 constexpr int XSLOTS = 6;
 static_assert(xslot(8) == 5 && xslot(5) == 2, "D2Q9 ring slots");
 
-// Four or five steps per launch (P phases): the lane-aligned walk with P-1
+// Four to six steps per launch (P phases): the lane-aligned walk with P-1
 // rings.  Lane t holds position x0 - (P-1) + t in all phases (phase p:
 // lanes p-1 .. w+2P-p-2; the last phase stores lanes P-1 .. w+P-2).  With two
 // barriers per walking step the phases alternate between the two intervals
@@ -195,18 +195,19 @@ static_assert(xslot(8) == 5 && xslot(5) == 2, "D2Q9 ring slots");
 // between the barriers, phases 3 (row r - 2 DIR) and 5 (row r - 5 DIR)
 // after the second; phase 4 so reads the ring-3 rows phase 3 wrote in the
 // walking steps before, phase 5 the ring-4 rows phase 4 wrote in the same
-// one.  Ring p holds the c_x != 0 results of phase p, one row per
+// one, phase 6 (between the barriers, row r - 7 DIR) the ring-5 rows phase
+// 5 wrote in the walking steps before.  Ring p holds the c_x != 0 results of phase p, one row per
 // population (6 x 64 NW values): every ring value has exactly one reader,
 // so each is written in the interval just before the one its reader runs
 // in, from registers — the populations read from the newest row (c_y DIR <
 // 0) at once, those read from the middle row one walking step later, those
 // read from the oldest row two walking steps later — and each slot is free
-// again when the next row's value arrives.  Rings 1 and 3 are written
+// again when the next row's value arrives.  Rings 1, 3 and 5 are written
 // after the second barrier (and ring 1 before the first), rings 2 and 4
 // between the barriers: every ring is written in one interval kind and
 // read in the other.  The c_x = 0 results stay in the lane's registers.
 // Output rows ya .. ya+rows-1 need step-k rows ya-(P-1) .. ya+rows+P-2:
-// rows + 7 (P = 4) or rows + 9 (P = 5) walking steps.
+// rows + 7 (P = 4), rows + 9 (P = 5) or rows + 12 (P = 6) walking steps.
 // T: the population arrays' element type (FP64 storage, or FP32 shifted
 // storage whose collision converts to FP64 and back); Args: the launch's
 // arguments (src, dst, g, step, stop_ro); collide(T (&)[9]) collides one
@@ -215,18 +216,19 @@ template <typename T, int NW, int DIR, int P, class Args, class Collide>
 __device__ __forceinline__ void stripP_walk_aligned(const Args &a, int x0, int w, int ya, int rows, T *ring, int prio,
                                                     Collide collide)
 {
-    static_assert(P == 4 || P == 5, "four or five phases");
+    static_assert(P >= 4 && P <= 6, "four, five or six phases");
     constexpr unsigned SZ = sizeof(T);
     using V = VSet<9>;
     constexpr int E = 64 * NW;
     T *const ring1 = ring, *const ring2 = ring + XSLOTS * E, *const ring3 = ring + 2 * XSLOTS * E;
-    T *const ring4 = ring + 3 * XSLOTS * E;   // P = 5 only
+    T *const ring4 = ring + 3 * XSLOTS * E;   // P >= 5
+    T *const ring5 = ring + 4 * XSLOTS * E;   // P = 6
     const int t = (int)threadIdx.x;
     const int nx = a.g.nx, ny = a.g.ny;
     // phase p covers positions x0-(P-p) .. x0+w-1+(P-p): lanes p-1 .. w+2P-p-2
     auto act = [&](int p) { return t >= p - 1 && t < w + 2 * P - p - 1; };
-    const bool act1 = act(1), act2 = act(2), act3 = act(3), act4 = act(4), act5 = act(5);
-    const bool actl = P == 4 ? act4 : act5;   // lanes of the last phase: the stores
+    const bool act1 = act(1), act2 = act(2), act3 = act(3), act4 = act(4), act5 = act(5), act6 = act(6);
+    const bool actl = P == 4 ? act4 : (P == 5 ? act5 : act6);   // lanes of the last phase: the stores
     const int x1 = act1 ? wrap(x0 - (P - 1) + t, nx) : 0;
     const __amdgpu_buffer_rsrc_t rs = pop_buffer<SZ>(a.src, a.g), rd = pop_buffer<SZ>(a.dst, a.g);
     const unsigned oob1 = act1 ? 0u : BUF_OOB;
@@ -277,15 +279,16 @@ __device__ __forceinline__ void stripP_walk_aligned(const Args &a, int x0, int w
     };
     const int r0 = DIR > 0 ? ya - (P - 1) : ya + rows + (P - 2);
     const int nrows1 = rows + 2 * (P - 1);           // rows of phase 1
-    const int nsteps = P == 4 ? rows + 7 : rows + 9;  // walking steps
+    const int nsteps = P == 4 ? rows + 7 : (P == 5 ? rows + 9 : rows + 12);  // walking steps
     T fq[9];
     T f1[9];                       // phase-1 row of the walking step before (its oldest-row ring writes)
-    T h2[9], h3[9], h4[9];         // phase-2 / 3 / 4 rows of the walking step before
-    T h2p[9], h3p[9], h4p[9];      // the same two walking steps before
+    T h2[9], h3[9], h4[9], h5[9];  // phase-2 / 3 / 4 / 5 rows of the walking step before
+    T h2p[9], h3p[9], h4p[9], h5p[9];   // the same two walking steps before
     T k1a[9], k1b[9];              // c_x = 0: phase-1 results of the rows one and two walking steps back
     T k2a[9], k2b[9];              // phase 2, likewise
     T k3a[9], k3b[9];              // phase 3, one and two walking steps before h3
-    T k4a[9], k4b[9];              // phase 4, likewise (P = 5)
+    T k4a[9], k4b[9];              // phase 4, likewise (P >= 5)
+    T k5a[9], k5b[9];              // phase 5, one and two walking steps before h5 (P = 6)
     load_row(r0, fq, false);
     if (a.step >= a.stop_ro[ST_STOP_STEP]) return;
     nine_dropped_stores<T>(rd);
@@ -303,7 +306,7 @@ __device__ __forceinline__ void stripP_walk_aligned(const Args &a, int x0, int w
         lds_barrier();
         if (act2 && n >= 3) put(ring2, h2, 1);                        // phase-2 rows of the steps before
         if (act2 && n >= 4) put(ring2, h2p, 2);
-        if constexpr (P == 5) {
+        if constexpr (P >= 5) {
             if (act4 && n >= 8) put(ring4, h4, 1);                    // phase-4 rows of the steps before
             if (act4 && n >= 9) put(ring4, h4p, 2);
         }
@@ -336,6 +339,16 @@ __device__ __forceinline__ void stripP_walk_aligned(const Args &a, int x0, int w
             }
             shift(k3a, k3b, h3);
         }
+        if constexpr (P == 6) {                                       // phase 6: step k+5, row r-7 DIR
+            const int y6 = n >= 12 ? r - 7 * DIR : ya;
+            T o[9] = {};
+            if (act6 && n >= 12) {
+                pull(o, ring5, h5, k5a, k5b);
+                collide(o);
+            }
+            shift(k5a, k5b, h5);
+            store(o, y6, act6 && n >= 12);
+        }
         lds_barrier();
         if (p1) put(ring1, f, 1);                                     // phase-1 rows r and r-DIR
         if (act1 && n >= 1 && n <= nrows1) put(ring1, f1, 2);
@@ -360,6 +373,18 @@ __device__ __forceinline__ void stripP_walk_aligned(const Args &a, int x0, int w
             }
             shift(k4a, k4b, h4);
             store(o, y5, act5 && n >= 9);
+        } else if constexpr (P == 6) {
+            if (act5 && n >= 10) put(ring5, h5, 1);                   // phase-5 rows of the steps before
+            if (act5 && n >= 11) put(ring5, h5p, 2);
+            if (act5 && n >= 9) {                                     // phase 5: step k+4, row r-5 DIR
+                T g[9];
+                pull(g, ring4, h4, k4a, k4b);
+                collide(g);
+                put(ring5, g, 0);
+                keep(h5p, h5);
+                keep(h5, g);
+            }
+            shift(k4a, k4b, h4);
         }
     }
 }

@@ -56,10 +56,10 @@ def tg_oracle(orc):
 
 
 @pytest.mark.parametrize("col", sorted(CASES))
-@pytest.mark.parametrize("shape", [42, 40, 26, 39, 11, 46, 47])
+@pytest.mark.parametrize("shape", [42, 40, 26, 39, 11, 46, 47, 48])
 def test_moment_d2q9_walks_within_tolerance(gpu, monkeypatch, tg_oracle, col, shape):
     """The three-step walks (lane-aligned with 4 and 2 wavefronts, 256-wide,
-    any-width), the two-step strip walk and the four- and five-step
+    any-width), the two-step strip walk and the four-, five- and six-step
     lane-aligned walks, every operator."""
     f0, want = tg_oracle
     tau, tnh = CASES[col]

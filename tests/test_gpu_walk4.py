@@ -1,6 +1,6 @@
-"""Four and five steps per launch (k_step4_astrip, ERLBM_FUSE=46;
-k_step5_astrip, 47): the lane-aligned strip walk with a third (and fourth)
-LDS ring.  Same arithmetic as the one-step kernel,
+"""Four, five and six steps per launch (k_step4_astrip, ERLBM_FUSE=46;
+k_step5_astrip, 47; k_step6_astrip, 48): the lane-aligned strip walk with
+three to five LDS rings.  Same arithmetic as the one-step kernel,
 so every population is bitwise the strict oracle, for every collision model,
 on ragged widths and heights (strips of any width, segments of both walking
 directions), for call lengths that leave one- to four-step remainders after
@@ -14,7 +14,7 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 SHAPE = 46
-STEPS = {46: 4, 47: 5}
+STEPS = {46: 4, 47: 5, 48: 6}
 
 
 def _perturbed(orc, nx, ny, seed):
@@ -24,7 +24,7 @@ def _perturbed(orc, nx, ny, seed):
     return feq[None] * (1 + 1e-2 * rng.uniform(-1, 1, (nf, 9))), st
 
 
-@pytest.mark.parametrize("shape", [46, 47])
+@pytest.mark.parametrize("shape", [46, 47, 48])
 @pytest.mark.parametrize("col", [O.COL_BGK, O.COL_MRT, O.COL_REG, O.COL_EXT_REG])
 @pytest.mark.parametrize("dims", [(1000, 24), (16, 16), (253, 16), (2048, 40), (272, 17), (4000, 16), (300, 77),
                                   (130, 33)])
@@ -47,13 +47,13 @@ def test_four_step_walk_ragged_bitwise(gpu, orc, monkeypatch, shape, col, dims):
             assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, done, col, tau, tnh, nthreads=4)), done
 
 
-@pytest.mark.parametrize("shape", [46, 47])
+@pytest.mark.parametrize("shape", [46, 47, 48])
 def test_four_step_walk_full_size_with_remainders(gpu, orc, monkeypatch, shape):
     """The double-shear workload at L = 2048: calls of 1, 4, 7 (4 + 3: the
     three-step remainder launch), 6 (4 + 2: the two-step remainder), 5, 3
-    and, for five steps per launch, 9 (5 + 4: the four-step remainder) and
-    10 steps, bitwise the strict oracle."""
-    L, calls = 2048, (1, 4, 7, 6, 5, 3, 9, 10)
+    and 9, 10, 11 and 8 steps (the four- and five-step remainders after
+    five- and six-step launches), bitwise the strict oracle."""
+    L, calls = 2048, (1, 4, 7, 6, 5, 3, 9, 10, 11, 8)
     tau = 0.5 + 3. * (1. / 32.) * L / 80000.
     f0 = gpu.init_double_shear(L)
     ini, st, nf = orc.stream_map(9, L, L, 1)
@@ -70,7 +70,7 @@ def test_four_step_walk_full_size_with_remainders(gpu, orc, monkeypatch, shape):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("shape", [46, 47])
+@pytest.mark.parametrize("shape", [46, 47, 48])
 @pytest.mark.parametrize("prio", ["0", "1", "2"])
 def test_four_step_walk_issue_priority_bitwise(gpu, orc, monkeypatch, shape, prio):
     nx, ny = 1000, 40
@@ -84,7 +84,7 @@ def test_four_step_walk_issue_priority_bitwise(gpu, orc, monkeypatch, shape, pri
         assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, 13, O.COL_EXT_REG, 0.51, 0.65, nthreads=16))
 
 
-@pytest.mark.parametrize("shape", [46, 47])
+@pytest.mark.parametrize("shape", [46, 47, 48])
 def test_four_step_walk_fma_tolerance_1000_steps(gpu, monkeypatch, shape):
     """The 1e-10 tolerance against the FMA-contracted oracle after 1000
     steps (D2Q9 double shear, L = 512)."""
