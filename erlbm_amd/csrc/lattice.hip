@@ -1565,13 +1565,14 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
 // The multi-step shape a call of erlbm_step uses (0: one step per launch).
 // Without an explicit ERLBM_FUSE, a shape must give every CU a workgroup
 // (small lattices have too few strips or tiles to fill the chip: there the
-// one-step kernel is faster).  FP64: the four-step lane-aligned walk, else
-// the three-step walk, lane-aligned on strips of any width or on 256-wide
-// strips (the rule below), else on strips of any width (four wavefronts,
-// lanes shifted), else the two-step walk, else the 64x8 tiles, else the
-// wavefront strips (any width from 64 up); shifted FP32 storage: the
-// four-step lane-aligned walk, else the three-step one, else the two-step
-// strip walk.
+// one-step kernel is faster).  FP64: the five-step lane-aligned walk, else
+// the four-step one, else the three-step walk, lane-aligned on strips of any
+// width or on 256-wide strips (the rule below), else on strips of any width
+// (four wavefronts, lanes shifted), else the two-step walk, else the 64x8
+// tiles, else the wavefront strips (any width from 64 up); shifted FP32
+// storage: the five-step lane-aligned walk (moment-space arithmetic only),
+// else the four-step one, else the three-step one, else the two-step strip
+// walk.
 long long strip3_workgroups(const erlbm_lattice *h, int f);
 long long astrip4_workgroups(const erlbm_lattice *h);
 long long astrip5_workgroups(const erlbm_lattice *h);
@@ -1601,9 +1602,12 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
     // FP64: the five-step walk where it makes at least 1.5 workgroups per CU
     // (MLUPS, five-step / four-step, exact then moment-space arithmetic:
     // L = 1280 80.2 / 79.9, 110.8 / 100.2; 2048 106.8 / 104.8, 147.5 / 129.2;
-    // 4096 122.2 / 117.0, 165.7 / 142.3)
-    if (h->elem == 8 && first == FUSE3_STRIP_SIZED && fuse_shape_fits(h, FUSE5_ASTRIP_4W) &&
-        2 * astrip5_workgroups(h) >= 3ll * h->n_cu)
+    // 4096 122.2 / 117.0, 165.7 / 142.3); FP32 storage in the moment-space
+    // arithmetic likewise (L = 1280 116.7 / 115.1, 2048 173.5 / 170.0, 4096
+    // 204.4 / 196.3), in the exact one not (80.2 / 77.8 four-step against
+    // five-step at L = 1280, 114.1 / 113.7 at 2048)
+    const bool five = (h->elem == 8 && first == FUSE3_STRIP_SIZED) || (h->elem == 4 && h->arith == ERLBM_ARITH_MOMENT);
+    if (five && fuse_shape_fits(h, FUSE5_ASTRIP_4W) && 2 * astrip5_workgroups(h) >= 3ll * h->n_cu)
         return FUSE5_ASTRIP_4W;
     // FP32 storage likewise (L = 2048, MLUPS, four-step / three-step, exact
     // and moment-space arithmetic: 111.6 / 107.7, 162.6 / 155.2; L = 4096

@@ -128,7 +128,7 @@ def test_dsl2048_fp32_shifted_moment_arith_1000_steps(gpu, dsl_setup, strict_100
     f0, _ = dsl_setup
     got, (shape, spl, desc) = _gpu_run(gpu, f0, gpu.FP32_SHIFTED, gpu.ARITH_MOMENT)
     print(f"GPU FP32 kernel, moment-space arithmetic: {desc} (shape {shape}, {spl} steps per launch)", flush=True)
-    assert spl == 4
+    assert spl == 5
     exact, _ = _gpu_run(gpu, f0, gpu.FP32_SHIFTED)
     u, uw, ue = _vel(got), _vel(strict_1000), _vel(exact)
     err = float(np.sqrt(np.sum((u - uw) ** 2) / np.sum(uw ** 2)))

@@ -566,12 +566,19 @@ def test_default_shape_rule(gpu):
             assert got[:2] == want_shape, (L, got)
             lat.set_params(0.51, 0.65, 1.0, O.COL_MRT)
             assert lat.step_shape()[:2] == want_shape, L
-    # shifted storage: FP32 takes the four-step lane-aligned walk, FP16 one
-    # step per launch
+    # shifted storage: FP32 takes the four-step lane-aligned walk (the
+    # five-step one in the moment-space arithmetic), FP16 one step per launch
     for prec, want_shape in ((gpu.FP32_SHIFTED, (46, 4)), (gpu.FP16_SHIFTED, (0, 1))):
         with gpu.Lattice(9, 2048, 2048, 1, precision=prec) as lat:
             lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
             assert lat.step_shape()[:2] == want_shape, prec
+    for L, want_shape in ((1024, (42, 3)), (1280, (47, 5)), (2048, (47, 5)), (4096, (47, 5))):
+        with gpu.Lattice(9, L, L, 1, precision=gpu.FP32_SHIFTED) as lat:
+            lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
+            lat.set_arith(gpu.ARITH_MOMENT)
+            assert lat.step_shape()[:2] == want_shape, L
+            lat.set_arith(gpu.ARITH_EXACT)
+            assert lat.step_shape()[1] <= 4, L
     # a stop test since the upload: one step per launch
     with gpu.Lattice(9, 2048, 2048, 1) as lat:
         lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
