@@ -49,8 +49,8 @@ struct TinyGeom {
     int nvp;                     // LDS plane stride (voxels, padded)
     int general;                 // solids or boundary columns present
     int inlet_x, outlet_x, outlet_dx;
-    const int *vox_of_fluid;     // [n_fluid]
-    const int *fluid_of_vox;     // [nx*ny], -1 = solid
+    const int *vox_of_fluid;     // [n_fluid]: voxel of thread p (boundary columns last)
+    const int *fluid_of_vox;     // [nx*ny]: index in fluid order (ascending voxels), -1 = solid
     const uint32_t *nbr;         // [nx*ny]: bit i = x - c_i solid, FLUID
     const double *sin_tab;       // sin(2 pi x / L), x < max(nx, ny)
     const double *cos_tab;
@@ -278,18 +278,20 @@ __device__ __forceinline__ double ordered_sum(const double *v, int n)
 // Energy of the gathered state, ordered over the sites; `scratch` aliases the
 // population LDS, so every gather of this step must be complete (barrier).
 template <class P>
-__device__ __forceinline__ double ordered_energy(typename P::T (&f)[Q], bool active, double rho0, double *scratch,
-                                                 int n_fluid, double *bcast)
+__device__ __forceinline__ double ordered_energy(typename P::T (&f)[Q], bool active, int rank, double rho0,
+                                                 double *scratch, int n_fluid, double *bcast)
 {
     __syncthreads();
-    if (active) scratch[threadIdx.x] = P::ke(f, rho0);   // fluid order = thread order
+    if (active) scratch[rank] = P::ke(f, rho0);   // rank: this site's index in fluid order
     __syncthreads();
     if (threadIdx.x == 0) bcast[0] = ordered_sum(scratch, n_fluid);
     __syncthreads();
     return bcast[0];
 }
 
-// One job in one workgroup; thread p owns fluid site p (voxel vox_of_fluid[p]).
+// One job in one workgroup; thread p owns voxel vox_of_fluid[p] (sites of the
+// boundary columns on the last threads), whose index in fluid order is
+// fluid_of_vox of that voxel.
 // GENERAL: solids and/or boundary columns.
 template <class P, int COL, bool GENERAL>
 __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typename P::T *sf, double *bcast)
@@ -303,11 +305,12 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
     const auto k = P::consts(J);
     const double rho0 = J.rho0;
 
-    int my_v = 0, my_x = 0, my_y = 0;
+    int my_v = 0, my_x = 0, my_y = 0, my_rank = tid;
     uint32_t my_bounce = 0, out_bounce = 0;
     bool out_fluid = false;
     if (active) {
         my_v = GENERAL ? a.g.vox_of_fluid[tid] : tid;
+        if constexpr (GENERAL) my_rank = a.g.fluid_of_vox[my_v];
         my_x = my_v % nx;
         my_y = my_v / nx;
         if constexpr (GENERAL) {
@@ -342,7 +345,7 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
             }
             P::init(f, rho0, ux, uy);
         }
-        if (J.check_every > 0) e0 = ordered_energy<P>(f, active, rho0, scratch, nf, bcast);
+        if (J.check_every > 0) e0 = ordered_energy<P>(f, active, my_rank, rho0, scratch, nf, bcast);
         __syncthreads();
         if (active)
             for (int i = 0; i < Q; i++) sf[i * nvp + my_v] = f[i];
@@ -380,7 +383,7 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
         }
         if (J.check_every > 0 && s > 0 && s % J.check_every == 0) {
             // (energy tests run on lattices without an outlet column; checked on the host)
-            const double e = ordered_energy<P>(f, active, rho0, scratch, nf, bcast);
+            const double e = ordered_energy<P>(f, active, my_rank, rho0, scratch, nf, bcast);
             e_last = e;
             e_max = (e > e_max || !isfinite(e)) ? e : e_max;
             if (e > J.energy_factor * e0 || !isfinite(e)) {
@@ -430,9 +433,10 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
         for (int i = 0; i < Q; i++) slot[i * nvp + my_v] = sf[i * nvp + my_v];
     double margin = 0.0;
     if (J.dens_limit > 0.0) {
-        // block max of the per-thread |rho| maxima (NaN propagates)
+        // block max of the per-site |rho| maxima in fluid order (NaN
+        // propagates; threads without a site hold 0 beyond the sites)
         __syncthreads();
-        scratch[tid] = r_max;
+        scratch[active ? my_rank : tid] = r_max;
         __syncthreads();
         if (tid == 0) {
             double m = 0.0;
@@ -672,6 +676,12 @@ extern "C" int erlbm_batch_create(int nx, int ny, const int *mask, int inlet_x, 
             vof.push_back(v);
         }
     const int nf = (int)vof.size();
+    // Thread order: the sites of the inlet and outlet columns last, so that
+    // only the last wavefronts hold the lanes that run the boundary
+    // conditions (a divergent branch costs every wavefront holding one of
+    // its lanes); fov keeps the fluid order, which the ordered sums follow.
+    std::stable_partition(vof.begin(), vof.end(),
+                          [&](int v) { return v % nx != inlet_x && v % nx != outlet_x; });
     if (nf < 1 || nf > TPB || nv > 400) {
         set_last_error_msg("erlbm_batch_create: a tiny lattice needs 1..256 fluid sites in at most 400 voxels");
         return ERLBM_ERR_ARG;
