@@ -292,15 +292,19 @@ __device__ __forceinline__ double ordered_energy(typename P::T (&f)[Q], bool act
 // One job in one workgroup; thread p owns voxel vox_of_fluid[p] (sites of the
 // boundary columns on the last threads), whose index in fluid order is
 // fluid_of_vox of that voxel.
-// GENERAL: solids and/or boundary columns.
-template <class P, int COL, bool GENERAL>
+// GENERAL: solids and/or boundary columns.  NS > 0: a periodic NS x NS
+// lattice, sizes known at compile time (the Taylor-Green maps' 16 x 16), so
+// a pull's LDS address is one of nine neighbour offsets plus a constant
+// population offset the LDS instruction carries.
+template <class P, int COL, bool GENERAL, int NS = 0>
 __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typename P::T *sf, double *bcast)
 {
     using T = typename P::T;
     using V = VSet<Q>;
+    static_assert(NS == 0 || (!GENERAL && NS * NS <= TPB && NS * NS % 8 == 0), "compile-time size: periodic lattices");
     const int tid = threadIdx.x;
     const int nf = a.g.n_fluid;
-    const int nx = a.g.nx, ny = a.g.ny, nvp = a.g.nvp;
+    const int nx = NS ? NS : a.g.nx, ny = NS ? NS : a.g.ny, nvp = NS ? NS * NS : a.g.nvp;
     const bool active = tid < nf;
     const auto k = P::consts(J);
     const double rho0 = J.rho0;
@@ -365,13 +369,27 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
     // run, so they are computed once.
     const bool outlet_copy = GENERAL && active && my_x == a.g.outlet_x && out_fluid;
     const int gx = outlet_copy ? my_x + a.g.outlet_dx : my_x;
-    int off[Q];
-    if (active) pull_offsets(nvp, gx, my_y, nx, ny, outlet_copy ? out_bounce : my_bounce, off);
+    int off[NS ? 1 : Q];
+    int nb9[9];   // NS > 0: voxel of x - c_x, y - c_y for (c_x + 1) * 3 + c_y + 1
+    if constexpr (NS > 0) {
+        const int xm = gx == 0 ? NS - 1 : gx - 1, xp = gx == NS - 1 ? 0 : gx + 1;
+        const int ym = my_y == 0 ? NS - 1 : my_y - 1, yp = my_y == NS - 1 ? 0 : my_y + 1;
+        const int xs[3] = {xp, gx, xm}, ys[3] = {yp, my_y, ym};
+        for (int p = 0; p < 9; p++) nb9[p] = xs[p / 3] + NS * ys[p % 3];
+    } else if (active) {
+        pull_offsets(nvp, gx, my_y, nx, ny, outlet_copy ? out_bounce : my_bounce, off);
+    }
     for (; s < end; s++) {
         double fator = 0.0;   // outlet column: rho0 / rho(upstream neighbour)
         if (active) {
             if (s > 0) {
-                static_for<Q>([&](auto I) { f[(int)I] = sf[off[(int)I]]; });
+                if constexpr (NS > 0)
+                    static_for<Q>([&](auto I) {
+                        constexpr int i = I;
+                        f[i] = sf[i * NS * NS + nb9[(V::cx[i] + 1) * 3 + V::cy[i] + 1]];
+                    });
+                else
+                    static_for<Q>([&](auto I) { f[(int)I] = sf[off[(int)I]]; });
             } else {                 // first step after the initial state: no stream
                 static_for<Q>([&](auto I) { f[(int)I] = sf[(int)I * nvp + gx + nx * my_y]; });
             }
@@ -467,15 +485,24 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
 }
 
 // Wavefronts per SIMD the compiler must leave room for: three for the FP64
-// periodic (Taylor-Green) lattices -- 168 VGPRs instead of 192, a few
-// spills, 9% faster over the full E-RLBM map -- and the natural two
-// otherwise (the boundary-column code of the obstacle channel spills 51
-// VGPRs at three and runs 14% slower).
-template <class P, bool GENERAL>
-constexpr int tiny_waves_per_eu() { return std::is_same<P, F64>::value && !GENERAL ? 3 : 2; }
+// periodic lattices -- at run-time sizes 168 VGPRs instead of 192, a few
+// spills, 9% faster over the full E-RLBM map -- and for the 16 x 16 ones
+// four under RLBM (128 VGPRs, 40 bytes spilled: TG RLBM map 55.7k -> 62.4k
+// points/s; E-RLBM at four spills 148 bytes and runs 2% slower than at
+// three, 162 VGPRs, no spill); three for FP32 storage of the 16 x 16 ones
+// (E-RLBM 172 -> 168 VGPRs: its TG map 35.5k -> 40.8k points/s); the
+// natural two otherwise (the boundary-column code of the obstacle channel
+// spills 51 VGPRs at three and runs 14% slower).
+template <class P, int COL, bool GENERAL, int NS>
+constexpr int tiny_waves_per_eu()
+{
+    if (GENERAL) return 2;
+    if (std::is_same<P, F64>::value) return NS > 0 && COL == COL_REG ? 4 : 3;
+    return std::is_same<P, F32>::value && NS > 0 ? 3 : 2;
+}
 
-template <class P, int COL, bool GENERAL>
-__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(tiny_waves_per_eu<P, GENERAL>())))
+template <class P, int COL, bool GENERAL, int NS = 0>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(tiny_waves_per_eu<P, COL, GENERAL, NS>())))
 void k_tiny(RunArgs a)
 {
     extern __shared__ double lds_raw[];   // Q * nvp populations of type P::T
@@ -491,7 +518,7 @@ void k_tiny(RunArgs a)
         if (k >= n_jobs) return;
         const int j = a.job_idx[k];
         const MapJob J = a.jobs[j];
-        run_job<P, COL, GENERAL>(a, J, &a.results[j], sf, bcast);
+        run_job<P, COL, GENERAL, NS>(a, J, &a.results[j], sf, bcast);
     }
 }
 
@@ -524,12 +551,14 @@ struct erlbm_batch {
 
 namespace {
 
-template <class P, int COL, bool GENERAL>
+template <class P, int COL, bool GENERAL, int NS = 0>
 int launch_group(erlbm_batch *b, const RunArgs &a, unsigned int *counter)
 {
+    if constexpr (!GENERAL && NS == 0)   // the Taylor-Green maps' lattice: sizes compiled in
+        if (a.g.nx == 16 && a.g.ny == 16) return launch_group<P, COL, false, 16>(b, a, counter);
     int per_cu = 0;
     const size_t lds = sizeof(typename P::T) * Q * (size_t)a.g.nvp;
-    ERLBM_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, GENERAL>, TPB, lds));
+    ERLBM_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, GENERAL, NS>, TPB, lds));
     const int want = b->wg_per_cu[P::code == ERLBM_FP64 ? 0 : 1];
     if (want > 0 && want < per_cu) per_cu = want;
     if (per_cu < 1) per_cu = 1;
@@ -538,7 +567,7 @@ int launch_group(erlbm_batch *b, const RunArgs &a, unsigned int *counter)
     if (grid < 1) return ERLBM_OK;
     RunArgs ra = a;
     ra.next_job = counter;
-    hipLaunchKernelGGL((k_tiny<P, COL, GENERAL>), dim3(grid), dim3(TPB), lds, b->stream, ra);
+    hipLaunchKernelGGL((k_tiny<P, COL, GENERAL, NS>), dim3(grid), dim3(TPB), lds, b->stream, ra);
     ERLBM_HIP_TRY(hipGetLastError());
     return ERLBM_OK;
 }
@@ -1073,8 +1102,9 @@ int occupancy_of(const erlbm_batch *b)
     int per_cu = 0;
     const size_t nvp = (size_t)(b->nx * b->ny + 7) / 8 * 8;
     const size_t lds = sizeof(typename P::T) * Q * nvp;
-    hipError_t e = b->general
-                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, true>, TPB, lds)
+    hipError_t e = b->general ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, true>, TPB, lds)
+                   : (b->nx == 16 && b->ny == 16)
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, false, 16>, TPB, lds)
                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, false>, TPB, lds);
     return e == hipSuccess && per_cu > 0 ? per_cu : 1;
 }
