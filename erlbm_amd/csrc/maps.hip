@@ -292,19 +292,22 @@ __device__ __forceinline__ double ordered_energy(typename P::T (&f)[Q], bool act
 // One job in one workgroup; thread p owns voxel vox_of_fluid[p] (sites of the
 // boundary columns on the last threads), whose index in fluid order is
 // fluid_of_vox of that voxel.
-// GENERAL: solids and/or boundary columns.  NS > 0: a periodic NS x NS
-// lattice, sizes known at compile time (the Taylor-Green maps' 16 x 16), so
-// a pull's LDS address is one of nine neighbour offsets plus a constant
-// population offset the LDS instruction carries.
-template <class P, int COL, bool GENERAL, int NS = 0>
+// GENERAL: solids and/or boundary columns.  NX, NY > 0: the lattice sizes
+// known at compile time (the Taylor-Green maps' 16 x 16, the obstacle map's
+// 22 x 12 channel), so a pull's LDS address is one of nine neighbour offsets
+// plus a constant population offset the LDS instruction carries; halfway
+// bounce-back then goes through ghost values (below) instead of per-site
+// pull offsets.
+template <class P, int COL, bool GENERAL, int NX = 0, int NY = 0>
 __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typename P::T *sf, double *bcast)
 {
     using T = typename P::T;
     using V = VSet<Q>;
-    static_assert(NS == 0 || (!GENERAL && NS * NS <= TPB && NS * NS % 8 == 0), "compile-time size: periodic lattices");
+    static_assert((NX == 0) == (NY == 0) && NX * NY <= 400, "compile-time sizes: both or neither");
+    constexpr int NVP = (NX * NY + 7) / 8 * 8;
     const int tid = threadIdx.x;
     const int nf = a.g.n_fluid;
-    const int nx = NS ? NS : a.g.nx, ny = NS ? NS : a.g.ny, nvp = NS ? NS * NS : a.g.nvp;
+    const int nx = NX ? NX : a.g.nx, ny = NY ? NY : a.g.ny, nvp = NX ? NVP : a.g.nvp;
     const bool active = tid < nf;
     const auto k = P::consts(J);
     const double rho0 = J.rho0;
@@ -369,24 +372,44 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
     // run, so they are computed once.
     const bool outlet_copy = GENERAL && active && my_x == a.g.outlet_x && out_fluid;
     const int gx = outlet_copy ? my_x + a.g.outlet_dx : my_x;
-    int off[NS ? 1 : Q];
-    int nb9[9];   // NS > 0: voxel of x - c_x, y - c_y for (c_x + 1) * 3 + c_y + 1
-    if constexpr (NS > 0) {
-        const int xm = gx == 0 ? NS - 1 : gx - 1, xp = gx == NS - 1 ? 0 : gx + 1;
-        const int ym = my_y == 0 ? NS - 1 : my_y - 1, yp = my_y == NS - 1 ? 0 : my_y + 1;
+    int off[NX ? 1 : Q];
+    int nb9[9];   // NX > 0: voxel of x - c_x, y - c_y for (c_x + 1) * 3 + c_y + 1
+    if constexpr (NX > 0) {
+        const int xm = gx == 0 ? NX - 1 : gx - 1, xp = gx == NX - 1 ? 0 : gx + 1;
+        const int ym = my_y == 0 ? NY - 1 : my_y - 1, yp = my_y == NY - 1 ? 0 : my_y + 1;
         const int xs[3] = {xp, gx, xm}, ys[3] = {yp, my_y, ym};
-        for (int p = 0; p < 9; p++) nb9[p] = xs[p / 3] + NS * ys[p % 3];
+        for (int p = 0; p < 9; p++) nb9[p] = xs[p / 3] + NX * ys[p % 3];
     } else if (active) {
         pull_offsets(nvp, gx, my_y, nx, ny, outlet_copy ? out_bounce : my_bounce, off);
+    }
+    // Ghost values (GENERAL, NX > 0): a site whose pull of population i
+    // would come from a solid voxel x - c_i writes its own post-collision
+    // f_opp(i) into that voxel's slot of population i -- which only this
+    // site's pull reads -- so every pull reads x - c_i alike.  The outlet
+    // column's sites gather at their upstream neighbour, whose own thread
+    // writes the ghosts that gather reads.
+    const bool ghosts = GENERAL && NX > 0 && active && gx == my_x && my_bounce != 0;
+    auto write_ghosts = [&](auto own) {
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            if constexpr (V::cx[i] != 0 || V::cy[i] != 0)
+                if (my_bounce & (1u << i)) sf[i * NVP + nb9[(V::cx[i] + 1) * 3 + V::cy[i] + 1]] = own(V::opp[i]);
+        });
+    };
+    if constexpr (GENERAL && NX > 0) {
+        if (resume) {   // the ghosts of the saved state
+            if (ghosts) write_ghosts([&](int i) { return sf[i * NVP + my_v]; });
+            __syncthreads();
+        }
     }
     for (; s < end; s++) {
         double fator = 0.0;   // outlet column: rho0 / rho(upstream neighbour)
         if (active) {
             if (s > 0) {
-                if constexpr (NS > 0)
+                if constexpr (NX > 0)
                     static_for<Q>([&](auto I) {
                         constexpr int i = I;
-                        f[i] = sf[i * NS * NS + nb9[(V::cx[i] + 1) * 3 + V::cy[i] + 1]];
+                        f[i] = sf[i * NVP + nb9[(V::cx[i] + 1) * 3 + V::cy[i] + 1]];
                     });
                 else
                     static_for<Q>([&](auto I) { f[(int)I] = sf[off[(int)I]]; });
@@ -439,6 +462,8 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
         __syncthreads();  // every gather of this step is done
         if (active)
             for (int i = 0; i < Q; i++) sf[i * nvp + my_v] = f[i];
+        if constexpr (GENERAL && NX > 0)
+            if (ghosts) write_ghosts([&](int i) { return f[i]; });
         const int any_bad = __syncthreads_or(bad ? 1 : 0);
         if (any_bad) {
             unstable = 1;
@@ -493,16 +518,16 @@ __device__ void run_job(const RunArgs &a, const MapJob &J, MapResult *res, typen
 // (E-RLBM 172 -> 168 VGPRs: its TG map 35.5k -> 40.8k points/s); the
 // natural two otherwise (the boundary-column code of the obstacle channel
 // spills 51 VGPRs at three and runs 14% slower).
-template <class P, int COL, bool GENERAL, int NS>
+template <class P, int COL, bool GENERAL, int NX>
 constexpr int tiny_waves_per_eu()
 {
-    if (GENERAL) return 2;
-    if (std::is_same<P, F64>::value) return NS > 0 && COL == COL_REG ? 4 : 3;
-    return std::is_same<P, F32>::value && NS > 0 ? 3 : 2;
+    if (GENERAL) return NX > 0 && !std::is_same<P, F32A>::value ? 3 : 2;
+    if (std::is_same<P, F64>::value) return NX > 0 && COL == COL_REG ? 4 : 3;
+    return std::is_same<P, F32>::value && NX > 0 ? 3 : 2;
 }
 
-template <class P, int COL, bool GENERAL, int NS = 0>
-__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(tiny_waves_per_eu<P, COL, GENERAL, NS>())))
+template <class P, int COL, bool GENERAL, int NX = 0, int NY = 0>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(tiny_waves_per_eu<P, COL, GENERAL, NX>())))
 void k_tiny(RunArgs a)
 {
     extern __shared__ double lds_raw[];   // Q * nvp populations of type P::T
@@ -518,7 +543,7 @@ void k_tiny(RunArgs a)
         if (k >= n_jobs) return;
         const int j = a.job_idx[k];
         const MapJob J = a.jobs[j];
-        run_job<P, COL, GENERAL, NS>(a, J, &a.results[j], sf, bcast);
+        run_job<P, COL, GENERAL, NX, NY>(a, J, &a.results[j], sf, bcast);
     }
 }
 
@@ -551,14 +576,16 @@ struct erlbm_batch {
 
 namespace {
 
-template <class P, int COL, bool GENERAL, int NS = 0>
+template <class P, int COL, bool GENERAL, int NX = 0, int NY = 0>
 int launch_group(erlbm_batch *b, const RunArgs &a, unsigned int *counter)
 {
-    if constexpr (!GENERAL && NS == 0)   // the Taylor-Green maps' lattice: sizes compiled in
-        if (a.g.nx == 16 && a.g.ny == 16) return launch_group<P, COL, false, 16>(b, a, counter);
+    if constexpr (NX == 0) {   // the maps' lattices: sizes compiled in
+        if (!GENERAL && a.g.nx == 16 && a.g.ny == 16) return launch_group<P, COL, false, 16, 16>(b, a, counter);
+        if (GENERAL && a.g.nx == 22 && a.g.ny == 12) return launch_group<P, COL, true, 22, 12>(b, a, counter);
+    }
     int per_cu = 0;
     const size_t lds = sizeof(typename P::T) * Q * (size_t)a.g.nvp;
-    ERLBM_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, GENERAL, NS>, TPB, lds));
+    ERLBM_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, GENERAL, NX, NY>, TPB, lds));
     const int want = b->wg_per_cu[P::code == ERLBM_FP64 ? 0 : 1];
     if (want > 0 && want < per_cu) per_cu = want;
     if (per_cu < 1) per_cu = 1;
@@ -567,7 +594,7 @@ int launch_group(erlbm_batch *b, const RunArgs &a, unsigned int *counter)
     if (grid < 1) return ERLBM_OK;
     RunArgs ra = a;
     ra.next_job = counter;
-    hipLaunchKernelGGL((k_tiny<P, COL, GENERAL, NS>), dim3(grid), dim3(TPB), lds, b->stream, ra);
+    hipLaunchKernelGGL((k_tiny<P, COL, GENERAL, NX, NY>), dim3(grid), dim3(TPB), lds, b->stream, ra);
     ERLBM_HIP_TRY(hipGetLastError());
     return ERLBM_OK;
 }
@@ -705,12 +732,6 @@ extern "C" int erlbm_batch_create(int nx, int ny, const int *mask, int inlet_x, 
             vof.push_back(v);
         }
     const int nf = (int)vof.size();
-    // Thread order: the sites of the inlet and outlet columns last, so that
-    // only the last wavefronts hold the lanes that run the boundary
-    // conditions (a divergent branch costs every wavefront holding one of
-    // its lanes); fov keeps the fluid order, which the ordered sums follow.
-    std::stable_partition(vof.begin(), vof.end(),
-                          [&](int v) { return v % nx != inlet_x && v % nx != outlet_x; });
     if (nf < 1 || nf > TPB || nv > 400) {
         set_last_error_msg("erlbm_batch_create: a tiny lattice needs 1..256 fluid sites in at most 400 voxels");
         return ERLBM_ERR_ARG;
@@ -725,6 +746,14 @@ extern "C" int erlbm_batch_create(int nx, int ny, const int *mask, int inlet_x, 
         }
         nbr[v] = bits;
     }
+    // Thread order: the sites of the inlet and outlet columns and those next
+    // to a solid last, so that only the last wavefronts hold the lanes that
+    // run the boundary conditions and write ghost values (a divergent branch
+    // costs every wavefront holding one of its lanes); fov keeps the fluid
+    // order, which the ordered sums follow.
+    std::stable_partition(vof.begin(), vof.end(), [&](int v) {
+        return v % nx != inlet_x && v % nx != outlet_x && (nbr[v] & ~FLUID) == 0;
+    });
     auto *b = new erlbm_batch();
     b->nx = nx;
     b->ny = ny;
@@ -1102,10 +1131,11 @@ int occupancy_of(const erlbm_batch *b)
     int per_cu = 0;
     const size_t nvp = (size_t)(b->nx * b->ny + 7) / 8 * 8;
     const size_t lds = sizeof(typename P::T) * Q * nvp;
-    hipError_t e = b->general ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, true>, TPB, lds)
-                   : (b->nx == 16 && b->ny == 16)
-                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, false, 16>, TPB, lds)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, false>, TPB, lds);
+    const bool tg = !b->general && b->nx == 16 && b->ny == 16, ch = b->general && b->nx == 22 && b->ny == 12;
+    hipError_t e = tg   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, false, 16, 16>, TPB, lds)
+                   : ch ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, true, 22, 12>, TPB, lds)
+                   : b->general ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, true>, TPB, lds)
+                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiny<P, COL, false>, TPB, lds);
     return e == hipSuccess && per_cu > 0 ? per_cu : 1;
 }
 
