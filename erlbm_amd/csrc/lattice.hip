@@ -1568,8 +1568,9 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
 // one-step kernel is faster).  FP64: the five-step lane-aligned walk, else
 // the four-step one, else the three-step walk, lane-aligned on strips of any
 // width or on 256-wide strips (the rule below), else on strips of any width
-// (four wavefronts, lanes shifted), else the two-step walk, else the 64x8
-// tiles, else the wavefront strips (any width from 64 up); shifted FP32
+// (four wavefronts, lanes shifted), else the two-step walk, else 32x8 tiles
+// where 64x8 ones do not fit or make fewer than two workgroups per CU, else
+// the 64x8 tiles, else the wavefront strips (any width from 64 up); shifted FP32
 // storage: the five-step lane-aligned walk (moment-space arithmetic only),
 // else the four-step one, else the three-step one, else the two-step strip
 // walk.
@@ -1629,6 +1630,14 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
             return FUSE3_ASTRIP_4W;
     }
     for (int f : {first, FUSE3_GSTRIP_4W, FUSE2_STRIP_256x32, 1, 16}) {
+        // 32x8 tiles where 64x8 ones do not fit or make fewer than two
+        // workgroups per CU (MLUPS, 32x8 / the shape before: L = 128 7.2 /
+        // 4.9 (one step per launch), 256 26.7 / 17.2 (wavefront strips),
+        // 384 41.0 / 34.7 (64x8), 448 49.1 / 46.8, 480 54.1 / 37.0; at 512
+        // 52.4 against 57.9 for 64x8, at 576 45.8 / 49.6)
+        if (f == 1 && h->elem == 8 && fuse_shape_fits(h, 3) &&
+            (!fuse_shape_fits(h, 1) || fuse_workgroups(h, FUSE_SHAPES[1]) < 2ll * h->n_cu))
+            return 3;
         if (!fuse_shape_fits(h, f)) continue;
         // strips of any width: from 95% of a workgroup per CU (L = 1000:
         // 248 workgroups, 61.1k MLUPS against 46.4k for the wavefront

@@ -418,7 +418,8 @@ def test_any_width_strips_ragged_bitwise(gpu, orc, monkeypatch, col, shape, dims
 
 @pytest.mark.parametrize("shape,dims", [(13, (384, 96)), (11, (768, 64)), (12, (768, 48)), (1, (192, 40)), (16, (1000, 24)),
                                         (20, (384, 80)), (19, (768, 40)), (22, (768, 64)), (24, (384, 96)), (25, (1536, 64)),
-                                        (26, (768, 83)), (27, (384, 50)), (29, (512, 16))])
+                                        (26, (768, 83)), (27, (384, 50)), (29, (512, 16)), (3, (480, 40)),
+                                        (3, (96, 24))])
 def test_two_step_kernels_non_power_of_two(gpu, orc, monkeypatch, shape, dims):
     """The two-step kernels on widths and heights that are no power of two
     (row and plane offsets then share bits with column offsets): bitwise
@@ -546,9 +547,11 @@ def test_default_shape_rule(gpu):
     where it makes 1.5 workgroups per CU or more, else the three-step
     lane-aligned any-width walk where it loads the CUs no more unevenly
     than the 256-wide walk or makes three workgroups per CU, the 256-wide
-    walk at L = 1024, the 64x8 two-step tiles at L = 768, and
-    steps_per_launch / descriptions that match."""
-    want = {768: (1, 2), 1000: (42, 3), 1024: (26, 3), 1280: (47, 5), 2048: (47, 5), 3000: (47, 5), 4096: (47, 5)}
+    walk at L = 1024, the 64x8 two-step tiles at L = 512 .. 768, the 32x8
+    ones below (fewer than two 64x8 workgroups per CU) and where 64x8 tiles
+    do not fit (L = 480), and steps_per_launch / descriptions that match."""
+    want = {64: (3, 2), 128: (3, 2), 256: (3, 2), 384: (3, 2), 448: (3, 2), 480: (3, 2), 512: (1, 2), 576: (1, 2),
+            768: (1, 2), 1000: (42, 3), 1024: (26, 3), 1280: (47, 5), 2048: (47, 5), 3000: (47, 5), 4096: (47, 5)}
     words = {2: "two steps", 3: "three steps", 4: "four steps", 5: "five steps"}
     for L, (shape, spl) in want.items():
         with gpu.Lattice(9, L, L, 1) as lat:
