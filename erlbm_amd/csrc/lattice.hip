@@ -1577,6 +1577,7 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
 long long strip3_workgroups(const erlbm_lattice *h, int f);
 long long astrip4_workgroups(const erlbm_lattice *h);
 long long astrip5_workgroups(const erlbm_lattice *h);
+int astrip_unit(const erlbm_lattice *h);
 
 int fuse_shape(const erlbm_lattice *h, int energy_interval)
 {
@@ -1608,8 +1609,25 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
     // 204.4 / 196.3), in the exact one not (80.2 / 77.8 four-step against
     // five-step at L = 1280, 114.1 / 113.7 at 2048)
     const bool five = (h->elem == 8 && first == FUSE3_STRIP_SIZED) || (h->elem == 4 && h->arith == ERLBM_ARITH_MOMENT);
-    if (five && fuse_shape_fits(h, FUSE5_ASTRIP_4W) && 2 * astrip5_workgroups(h) >= 3ll * h->n_cu)
-        return FUSE5_ASTRIP_4W;
+    if (five && fuse_shape_fits(h, FUSE5_ASTRIP_4W)) {
+        // FP64 below 1.5 workgroups per CU: from 0.75 per CU in the
+        // moment-space arithmetic, from 0.85 in the exact one where 256-wide
+        // strips do not divide the width (MLUPS, five-step / the shape
+        // picked before, exact then moment-space: L = 768 (0.75 per CU)
+        // 43.0 / 52.7, 58.2 / 52.9; 896 59.3 / 53.6, 76.6 / 53.7; 960 67.3 /
+        // 56.3, 87.3 / 56.4; 1088 59.1 / 55.8, 83.0 / 69.2; 1152 66.0 /
+        // 61.8, 91.5 / 73.5; at 1024, 256-wide strips, 52.6 / 60.1, 74.4 /
+        // 74.0; at 704 (0.52 per CU) 36.6 / 48.7, 47.4 / 48.7) -- where its
+        // strips are no more than the three-step walk's (L = 1000: five
+        // strips of 200 against four of 250, 48.6 / 65.8, 69.4 / 80.7)
+        const long long w = astrip5_workgroups(h), n = h->n_cu;
+        const int u = astrip_unit(h), units = h->nx / u;
+        const bool strips = (units + 248 / u - 1) / (248 / u) <= (units + 252 / u - 1) / (252 / u);
+        const bool moment = h->elem == 8 && h->arith == ERLBM_ARITH_MOMENT && h->col != ERLBM_COL_MRT;
+        if (2 * w >= 3 * n || (strips && moment && 4 * w >= 3 * n) ||
+            (strips && h->elem == 8 && 20 * w >= 17 * n && h->nx % 256 != 0))
+            return FUSE5_ASTRIP_4W;
+    }
     // FP32 storage likewise (L = 2048, MLUPS, four-step / three-step, exact
     // and moment-space arithmetic: 111.6 / 107.7, 162.6 / 155.2; L = 4096
     // 125.8 / 122.8, 193.7 / 180.4; L = 1280 78.7 / 79.4, 112.0 / 109.2)
