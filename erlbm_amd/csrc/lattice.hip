@@ -1572,8 +1572,7 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
 // where 64x8 ones do not fit or make fewer than two workgroups per CU, else
 // the 64x8 tiles, else the wavefront strips (any width from 64 up); shifted FP32
 // storage: the five-step lane-aligned walk (moment-space arithmetic only),
-// else the four-step one, else the three-step one, else the two-step strip
-// walk.
+// else the four-step one, else one step per launch.
 long long strip3_workgroups(const erlbm_lattice *h, int f);
 long long astrip4_workgroups(const erlbm_lattice *h);
 long long astrip5_workgroups(const erlbm_lattice *h);
@@ -1591,10 +1590,7 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
         return 0;
     }
     if (!(h->q == 9 && h->nz == 1)) return 0;
-    // FP32 storage: the lane-aligned three-step walk (L = 2048: 44.9 us/step
-    // against 49.8 for the two-step walk and 50.0 for the 256-wide
-    // three-step one), else the two-step walk
-    const int first = h->elem == 8 ? h->tune_fuse : FUSE3_ASTRIP_4W;
+    const int first = h->tune_fuse;
     // FP64: the four-step lane-aligned walk where it makes at least 1.5
     // workgroups per CU (MLUPS, four-step / three-step rule below, exact and
     // moment-space arithmetic: L = 1280 81.0 / 74.3, 100.5 / 87.0; 1536 94.5 /
@@ -1623,7 +1619,7 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
         const long long w = astrip5_workgroups(h), n = h->n_cu;
         const int u = astrip_unit(h), units = h->nx / u;
         const bool strips = (units + 248 / u - 1) / (248 / u) <= (units + 252 / u - 1) / (252 / u);
-        const bool moment = h->elem == 8 && h->arith == ERLBM_ARITH_MOMENT && h->col != ERLBM_COL_MRT;
+        const bool moment = h->arith == ERLBM_ARITH_MOMENT && h->col != ERLBM_COL_MRT;
         if (2 * w >= 3 * n || (strips && moment && 4 * w >= 3 * n) ||
             (strips && h->elem == 8 && 20 * w >= 17 * n && h->nx % 256 != 0))
             return FUSE5_ASTRIP_4W;
@@ -1634,6 +1630,12 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
     if (((h->elem == 8 && first == FUSE3_STRIP_SIZED) || h->elem == 4) && fuse_shape_fits(h, FUSE4_ASTRIP_4W) &&
         2 * astrip4_workgroups(h) >= 3ll * h->n_cu)
         return FUSE4_ASTRIP_4W;
+    // FP32 storage below that: one step per launch (MLUPS, one step / the
+    // three-step walk, exact arithmetic: L = 1024 64.5 / 51.5, 1152 66.8 /
+    // 65.1; moment-space, L = 512, 43.7 / 23.9; the five-step walk from
+    // 0.75 workgroups per CU in the moment-space arithmetic, as above: 768
+    // 58.3 / 56.7, 896 78.5 / 60.8, 1024 75.0 / 70.9, 1152 94.9 / 89.4)
+    if (h->elem == 4) return 0;
     // FP64: the lane-aligned walk on strips of any width (four wavefronts,
     // four workgroups per CU) before the 256-wide walk where it puts no more
     // workgroups on a CU than that one, or makes three per CU or more

@@ -578,7 +578,8 @@ def test_default_shape_rule(gpu):
         with gpu.Lattice(9, 2048, 2048, 1, precision=prec) as lat:
             lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
             assert lat.step_shape()[:2] == want_shape, prec
-    for L, want_shape in ((1024, (42, 3)), (1280, (47, 5)), (2048, (47, 5)), (4096, (47, 5))):
+    for L, want_shape in ((512, (0, 1)), (768, (47, 5)), (1024, (47, 5)), (1280, (47, 5)), (2048, (47, 5)),
+                          (4096, (47, 5))):
         with gpu.Lattice(9, L, L, 1, precision=gpu.FP32_SHIFTED) as lat:
             lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
             lat.set_arith(gpu.ARITH_MOMENT)
