@@ -1620,6 +1620,14 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
         const int u = astrip_unit(h), units = h->nx / u;
         const bool strips = (units + 248 / u - 1) / (248 / u) <= (units + 252 / u - 1) / (252 / u);
         const bool moment = h->arith == ERLBM_ARITH_MOMENT && h->col != ERLBM_COL_MRT;
+        // between one and 1.5 workgroups per CU in the exact arithmetic some
+        // CUs walk two five-step workgroups and others one: the two-wavefront
+        // lane-aligned three-step walk balances them (MLUPS, it / five-step:
+        // L = 1088 65.9 / 59.0, 1100 65.3 / 58.7, 1152 68.8 / 65.7, 1200
+        // 72.8 / 70.9)
+        if (h->elem == 8 && !moment && strips && h->nx % 256 != 0 && w > n && 2 * w < 3 * n &&
+            fuse_shape_fits(h, FUSE3_ASTRIP_2W))
+            return FUSE3_ASTRIP_2W;
         if (2 * w >= 3 * n || (strips && moment && 4 * w >= 3 * n) ||
             (strips && h->elem == 8 && 20 * w >= 17 * n && h->nx % 256 != 0))
             return FUSE5_ASTRIP_4W;

@@ -547,11 +547,15 @@ def test_default_shape_rule(gpu):
     where it makes 1.5 workgroups per CU or more, else the three-step
     lane-aligned any-width walk where it loads the CUs no more unevenly
     than the 256-wide walk or makes three workgroups per CU, the 256-wide
-    walk at L = 1024, the 64x8 two-step tiles at L = 512 .. 768, the 32x8
+    walk at L = 1024, the five-step walk from 0.85 workgroups per CU where
+    256-wide strips do not divide the width (L = 896) and the two-wavefront
+    three-step walk between one and 1.5 per CU (L = 1152), the 64x8
+    two-step tiles at L = 512 .. 768, the 32x8
     ones below (fewer than two 64x8 workgroups per CU) and where 64x8 tiles
     do not fit (L = 480), and steps_per_launch / descriptions that match."""
     want = {64: (3, 2), 128: (3, 2), 256: (3, 2), 384: (3, 2), 448: (3, 2), 480: (3, 2), 512: (1, 2), 576: (1, 2),
-            768: (1, 2), 1000: (42, 3), 1024: (26, 3), 1280: (47, 5), 2048: (47, 5), 3000: (47, 5), 4096: (47, 5)}
+            768: (1, 2), 896: (47, 5), 1000: (42, 3), 1024: (26, 3), 1152: (40, 3), 1280: (47, 5), 2048: (47, 5),
+            3000: (47, 5), 4096: (47, 5)}
     words = {2: "two steps", 3: "three steps", 4: "four steps", 5: "five steps"}
     for L, (shape, spl) in want.items():
         with gpu.Lattice(9, L, L, 1) as lat:
