@@ -456,6 +456,28 @@ def test_default_multi_step_on_ragged_width(gpu, orc):
         assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, 9, O.COL_EXT_REG, 0.51, 0.65, nthreads=16))
 
 
+@pytest.mark.parametrize("L,want", [(64, 3), (480, 3), (896, 47), (1152, 40)])
+def test_default_rule_picks_bitwise(gpu, orc, L, want):
+    """The shapes the default rule picks at small and mid sizes (32x8 tiles,
+    the five-step walk below 1.5 workgroups per CU, the two-wavefront
+    three-step walk between one and 1.5), for calls leaving every
+    remainder: bitwise the strict oracle."""
+    rng = np.random.default_rng(L)
+    ini, st, nf = orc.stream_map(9, L, L, 1)
+    feq = orc.dist_eq(9, 0.03, -0.02, 0.0, 1.0)
+    f0 = feq[None] * (1 + 1e-2 * rng.uniform(-1, 1, (nf, 9)))
+    calls = (1, 4, 7, 5, 3, 2)
+    with gpu.Lattice(9, L, L, 1) as lat:
+        lat.set_params(0.51, 0.65, 1.0, O.COL_EXT_REG)
+        lat.upload(f0)
+        assert lat.step_shape()[0] == want
+        for n in calls:
+            lat.step(n)
+        assert lat.status() == (sum(calls), 0)
+        got = lat.download()
+    assert np.array_equal(got, orc.run_steps(9, f0, st, sum(calls), O.COL_EXT_REG, 0.51, 0.65, nthreads=16))
+
+
 @pytest.mark.parametrize("q,dims,seed", [(9, (300, 77, 1), 1), (9, (5, 3, 1), 2), (19, (37, 19, 11), 3),
                                          (19, (4, 3, 2), 4)])
 def test_ragged_sizes_random_solids(gpu, orc, q, dims, seed):
