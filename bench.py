@@ -55,6 +55,9 @@ FP64_INSTR_PER_NODE_STEP = 175
 # range tests of 1/rho per six collisions of the walking loop)
 FP64_INSTR_PER_NODE_STEP_MOMENT = 85
 FP64_VALU_PEAK_T = 39.3
+# VALU lane-instructions per node-step of the Taylor-Green E-RLBM map kernel
+# (D3Q19 16x16x1, FP64: SQ_INSTS_VALU x 64 / node-steps of the full map)
+MAP_TG_VALU_PER_NODE_STEP = 413
 # rocprofv3 PMC traffic per launch (FETCH_SIZE + WRITE_SIZE, calibrated against
 # a plain copy kernel), measured on the FP64 step kernel
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "dsl2048_d2q9_fp64_traffic.json")
@@ -257,6 +260,13 @@ def maps_section(E):
              "MLUPS": round(float(np.sum(res["steps"].astype(float)) * nf / dt) / 1e6, 1),
              "unstable": int(res["unstable"].sum()), "agreement_vs_published_data": round(agree, 5),
              "autotune": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in pick.items()}}
+        if prec == "fp64" and name == "tg_erlbm065":
+            # against the vector-ALU bound: VALU lane-instructions per node-step
+            # of this map's kernel (rocprofv3 SQ_INSTS_VALU, profiles/
+            # tg_map_erlbm065_fp64_k_tiny.md) at 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+            rate = r["node_steps_per_s"] * MAP_TG_VALU_PER_NODE_STEP / 1e12
+            r["valu"] = {"instr_per_node_step_profiled": MAP_TG_VALU_PER_NODE_STEP, "achieved_T_per_s": round(rate, 2),
+                         "peak_T_per_s": FP64_VALU_PEAK_T, "frac": round(rate / FP64_VALU_PEAK_T, 4)}
         if prec == "fp64":
             fp64[name] = res["unstable"]
             out[name] = r
