@@ -2487,6 +2487,7 @@ std::vector<LaunchShape> candidate_shapes(const erlbm_lattice *h)
             {11, {"two steps, 256-wide strips x 32 rows", "two steps, 256-wide strips x 32 rows, XCD runs"}},
             {13, {"two steps, 128-wide strips x 32 rows", "two steps, 128-wide strips x 32 rows, XCD runs"}},
             {1, {"two steps, 64x8 tiles", "two steps, 64x8 tiles, XCD runs"}},
+            {3, {"two steps, 32x8 tiles", "two steps, 32x8 tiles, XCD runs"}},
             {7, {"two steps, 64x4 tiles", "two steps, 64x4 tiles, XCD runs"}},
             {16, {"two steps, wavefront strips", "two steps, wavefront strips, XCD runs"}},
             {22, {"three steps, 256-wide strips x 32 rows", "three steps, 256-wide strips x 32 rows, XCD runs"}},

@@ -108,6 +108,8 @@ def test_autotune_candidates_respect_the_31_bit_buffer_limit():
     assert not (walks & big), big
     # FP64: the raw-buffer walks stop where 8-byte offsets pass 2^31 as well
     assert {42, 26, 39, 46, 47} <= set(Lm.autotune_candidates(9, 4096, 4096, 1, FP64))
+    # small FP64 lattices: the tiles the default rule takes there among them
+    assert {1, 3, 7} <= set(Lm.autotune_candidates(9, 256, 256, 1, FP64))
     assert not ({42, 26, 39, 11, 13, 22, 16, 46, 47} & set(Lm.autotune_candidates(9, 8192, 8192, 1, FP64)))
     # D3Q19: the z-walk tiles where the geometry takes them
     assert {33, 31, 30} <= set(Lm.autotune_candidates(19, 256, 256, 256, FP64))
