@@ -2184,6 +2184,8 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
         case FUSE5_ASTRIP_4W: launch_astrip5_t<COL, 4>(h, a); return;
         case FUSE6_ASTRIP_4W: launch_astrip6_t<COL, 4>(h, a); return;
         case FUSE2_STRIP_256x32: launch_strip_t<COL, 256, 32>(h, a); return;
+        case 1: launch_fused_t<COL, 64, 8>(h, a); return;   // the size rule's two-step tiles
+        case 3: launch_fused_t<COL, 32, 8>(h, a); return;
         default: launch_fused_c<(COL & ~COL_MOMENT)>(h, a); return;
         }
     }
