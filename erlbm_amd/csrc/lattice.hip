@@ -1578,6 +1578,14 @@ long long astrip4_workgroups(const erlbm_lattice *h);
 long long astrip5_workgroups(const erlbm_lattice *h);
 int astrip_unit(const erlbm_lattice *h);
 
+// Output columns the five-step walk's strips offer for this width (strips of
+// at most 248 positions, edges on the lane-aligned walks' unit).
+long long fw_cap(const erlbm_lattice *h)
+{
+    const int u = astrip_unit(h), units = h->nx / u;
+    return (long long)((units + 248 / u - 1) / (248 / u)) * (248 / u) * u;
+}
+
 int fuse_shape(const erlbm_lattice *h, int energy_interval)
 {
     if (!(h->all_fluid && !h->forced && h->inlet_x < 0 && h->outlet_x < 0 && h->dens_limit <= 0.0 &&
@@ -1628,15 +1636,21 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
         if (h->elem == 8 && !moment && strips && h->nx % 256 != 0 && w > n && 2 * w < 3 * n &&
             fuse_shape_fits(h, FUSE3_ASTRIP_2W))
             return FUSE3_ASTRIP_2W;
-        if (2 * w >= 3 * n || (strips && moment && 4 * w >= 3 * n) ||
-            (strips && h->elem == 8 && 20 * w >= 17 * n && h->nx % 256 != 0))
+        // in the exact arithmetic only where the strips use at least 80% of
+        // the lanes (512 x 2048: three strips of about 170 on 256 lanes, 54.4
+        // against 59.3 for the 256-wide three-step walk; the moment-space
+        // walk, bound by HBM, stays the fastest there, 79.0 / 71.5)
+        const bool lanes = moment || h->elem != 8 || 5 * h->nx >= 4 * fw_cap(h);
+        if (lanes && (2 * w >= 3 * n || (strips && moment && 4 * w >= 3 * n) ||
+                      (strips && h->elem == 8 && 20 * w >= 17 * n && h->nx % 256 != 0)))
             return FUSE5_ASTRIP_4W;
     }
     // FP32 storage likewise (L = 2048, MLUPS, four-step / three-step, exact
     // and moment-space arithmetic: 111.6 / 107.7, 162.6 / 155.2; L = 4096
     // 125.8 / 122.8, 193.7 / 180.4; L = 1280 78.7 / 79.4, 112.0 / 109.2)
-    if (((h->elem == 8 && first == FUSE3_STRIP_SIZED) || h->elem == 4) && fuse_shape_fits(h, FUSE4_ASTRIP_4W) &&
-        2 * astrip4_workgroups(h) >= 3ll * h->n_cu)
+    if (((h->elem == 8 && first == FUSE3_STRIP_SIZED && ((h->arith == ERLBM_ARITH_MOMENT && h->col != ERLBM_COL_MRT) || 5 * h->nx >= 4 * fw_cap(h))) ||
+         h->elem == 4) &&
+        fuse_shape_fits(h, FUSE4_ASTRIP_4W) && 2 * astrip4_workgroups(h) >= 3ll * h->n_cu)
         return FUSE4_ASTRIP_4W;
     // FP32 storage below that: one step per launch (MLUPS, one step / the
     // three-step walk, exact arithmetic: L = 1024 64.5 / 51.5, 1152 66.8 /
