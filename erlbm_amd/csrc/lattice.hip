@@ -1569,8 +1569,9 @@ bool fuse_shape_fits(const erlbm_lattice *h, int f)
 // the four-step one, else the three-step walk, lane-aligned on strips of any
 // width or on 256-wide strips (the rule below), else on strips of any width
 // (four wavefronts, lanes shifted), else the two-step walk, else 32x8 tiles
-// where 64x8 ones do not fit or make fewer than two workgroups per CU, else
-// the 64x8 tiles, else the wavefront strips (any width from 64 up); shifted FP32
+// where 64x8 ones do not fit or make fewer than two workgroups per CU, 64x4
+// ones where they make 3 to 4.5, else the 64x8 tiles, else the wavefront
+// strips (any width from 64 up); shifted FP32
 // storage: the five-step lane-aligned walk (moment-space arithmetic only),
 // else the four-step one, else one step per launch.
 long long strip3_workgroups(const erlbm_lattice *h, int f);
@@ -1680,6 +1681,14 @@ int fuse_shape(const erlbm_lattice *h, int energy_interval)
         if (f == 1 && h->elem == 8 && fuse_shape_fits(h, 3) &&
             (!fuse_shape_fits(h, 1) || fuse_workgroups(h, FUSE_SHAPES[1]) < 2ll * h->n_cu))
             return 3;
+        // 64x4 tiles where 64x8 ones make 3 to 4.5 workgroups per CU (MLUPS,
+        // 64x4 / 64x8, exact then moment-space: L = 640 45.8 / 42.6, 48.9 /
+        // 45.3; 704 51.1 / 48.7, 55.4 / 53.0; at 576 (2.5 per CU) 45.4 /
+        // 49.4, at 768 (4.5) 52.3 / 52.9)
+        if (f == 1 && h->elem == 8 && fuse_shape_fits(h, 1) && fuse_shape_fits(h, 7)) {
+            const long long w1 = fuse_workgroups(h, FUSE_SHAPES[1]);
+            if (w1 >= 3ll * h->n_cu && 2 * w1 < 9ll * h->n_cu) return 7;
+        }
         if (!fuse_shape_fits(h, f)) continue;
         // strips of any width: from 95% of a workgroup per CU (L = 1000:
         // 248 workgroups, 61.1k MLUPS against 46.4k for the wavefront
@@ -2200,6 +2209,7 @@ void launch_fused_c(erlbm_lattice *h, const StepArgs &a)
         case FUSE2_STRIP_256x32: launch_strip_t<COL, 256, 32>(h, a); return;
         case 1: launch_fused_t<COL, 64, 8>(h, a); return;   // the size rule's two-step tiles
         case 3: launch_fused_t<COL, 32, 8>(h, a); return;
+        case 7: launch_fused_t<COL, 64, 4>(h, a); return;
         default: launch_fused_c<(COL & ~COL_MOMENT)>(h, a); return;
         }
     }

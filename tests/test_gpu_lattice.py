@@ -456,10 +456,10 @@ def test_default_multi_step_on_ragged_width(gpu, orc):
         assert np.array_equal(lat.download(), orc.run_steps(9, f0, st, 9, O.COL_EXT_REG, 0.51, 0.65, nthreads=16))
 
 
-@pytest.mark.parametrize("L,want", [(64, 3), (480, 3), (896, 47), (1152, 40)])
+@pytest.mark.parametrize("L,want", [(64, 3), (480, 3), (640, 7), (896, 47), (1152, 40)])
 def test_default_rule_picks_bitwise(gpu, orc, L, want):
     """The shapes the default rule picks at small and mid sizes (32x8 tiles,
-    the five-step walk below 1.5 workgroups per CU, the two-wavefront
+    64x4 tiles, the five-step walk below 1.5 workgroups per CU, the two-wavefront
     three-step walk between one and 1.5), for calls leaving every
     remainder: bitwise the strict oracle."""
     rng = np.random.default_rng(L)
@@ -576,7 +576,7 @@ def test_default_shape_rule(gpu):
     ones below (fewer than two 64x8 workgroups per CU) and where 64x8 tiles
     do not fit (L = 480), and steps_per_launch / descriptions that match."""
     want = {64: (3, 2), 128: (3, 2), 256: (3, 2), 384: (3, 2), 448: (3, 2), 480: (3, 2), 512: (1, 2), 576: (1, 2),
-            768: (1, 2), 896: (47, 5), 1000: (42, 3), 1024: (26, 3), 1152: (40, 3), 1280: (47, 5), 2048: (47, 5),
+            640: (7, 2), 704: (7, 2), 768: (1, 2), 896: (47, 5), 1000: (42, 3), 1024: (26, 3), 1152: (40, 3), 1280: (47, 5), 2048: (47, 5),
             3000: (47, 5), 4096: (47, 5)}
     words = {2: "two steps", 3: "three steps", 4: "four steps", 5: "five steps"}
     for L, (shape, spl) in want.items():

@@ -56,11 +56,11 @@ def tg_oracle(orc):
 
 
 @pytest.mark.parametrize("col", sorted(CASES))
-@pytest.mark.parametrize("shape", [42, 40, 26, 39, 11, 46, 47, 48, 1, 3])
+@pytest.mark.parametrize("shape", [42, 40, 26, 39, 11, 46, 47, 48, 1, 3, 7])
 def test_moment_d2q9_walks_within_tolerance(gpu, monkeypatch, tg_oracle, col, shape):
     """The three-step walks (lane-aligned with 4 and 2 wavefronts, 256-wide,
     any-width), the two-step strip walk, the four-, five- and six-step
-    lane-aligned walks and the 64x8 and 32x8 two-step tiles, every
+    lane-aligned walks and the 64x8, 32x8 and 64x4 two-step tiles, every
     operator."""
     f0, want = tg_oracle
     tau, tnh = CASES[col]
