@@ -201,7 +201,7 @@ def precision_sweep(n: int = 400, r_max: int = 120) -> dict:
     map, and the cheapest mixed scheme that reaches 99.9% agreement -- the
     FP32 map, then the FP64 re-run of every point within r map cells of the
     FP32 map's stability boundary, r the smallest that reaches 99.9% with the
-    FP64 map as the judge (a best case: a service would not know the FP64
+    FP64 map as the arbiter (a best case: a service would not know the FP64
     map); the re-run is priced at the FP64 map's time per node-step."""
     import time
     out = {}
